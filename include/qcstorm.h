@@ -1,0 +1,294 @@
+/*
+ * qcstorm.h -- C-ABI of the MI355X (gfx950) localization library libqcstorm.so.
+ *
+ * The library replaces the localization hot path of QC-STORM: ROI extraction from
+ * uint16 sCMOS frames, weighted-likelihood pre-estimation, BFGS Poisson-MLE Gaussian
+ * fitting (2D / astigmatic 3D, 1-3 emitters) with CRLB, frame-sorted compaction of
+ * the 12-float localization records and super-resolution splat rendering.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and returns an
+ * int status (QCS_OK == 0).  qcs_last_error() gives a thread-local message for the
+ * last failing call.  Pointers named d_* are device (HBM) pointers; the others are
+ * host pointers.  `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   DLL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/bfgsMLE_dll
+ *   REL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/x64/Release
+ *   IJ  = Plugin for ImageJ/QC-STORM_ CPPDLL/QC-STORM_ CPPDLL
+ *   MM  = Plugin for Micro-Manager/QC-STORM CPPDLL/QC-STORM CPPDLL
+ */
+#ifndef QCSTORM_H
+#define QCSTORM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints and continues; see DLL/cudaWrapper/cudaWrapper.cu:129-139) */
+#define QCS_OK               0
+#define QCS_ERR_ARG          1  /* invalid argument (NULL pointer, bad size, unsupported ROI size) */
+#define QCS_ERR_HIP          2  /* HIP runtime error */
+#define QCS_ERR_CAPACITY     3  /* output capacity too small; partial results are reported */
+#define QCS_ERR_STATE        4  /* call not valid in the current handle state */
+#define QCS_ERR_NOMEM        5  /* allocation failed */
+#define QCS_ERR_NODEVICE     6  /* no usable gfx950 device */
+
+/* ---- localization types (DLL/bfgsMLE_Loc_LD/bfgs_CommonPara.h:274-277) */
+#define QCS_LOCTYPE_GS2D     0
+#define QCS_LOCTYPE_AS3D     1
+#define QCS_LOCTYPE_DH3D     2  /* accepted and fitted as GS2D, as LocType_IsGS2D does */
+
+/* ---- 12-float localization record (bfgs_CommonPara.h:292-309) */
+#define QCS_REC_FLOATS       12
+#define QCS_POS_PPHO         0   /* peak photon */
+#define QCS_POS_XPOS         1   /* x, pixel (pixel i spans [i, i+1)) */
+#define QCS_POS_YPOS         2
+#define QCS_POS_ZPOS         3   /* z, nm (AS3D) */
+#define QCS_POS_SIGX         4   /* sigma x, pixel */
+#define QCS_POS_SIGY         5
+#define QCS_POS_TPHO         6   /* total photon */
+#define QCS_POS_BAKG         7   /* background photon per pixel */
+#define QCS_POS_PSNR         8   /* peak SNR */
+#define QCS_POS_CRBX         9   /* CRLB x, nm */
+#define QCS_POS_CRBY         10  /* CRLB y, nm */
+#define QCS_POS_FRME         11  /* frame number */
+
+/* ---- ROI record: uint16[R*(R+1)] = R*R raw pixels row-major, then
+ *      [Xc, Yc_in_frame, frame % 65536, frame / 65536, FitType, 0...]
+ *      (DLL/SubregionExtraction/LDROIExtraction_ROIExtraction.cu:257-302) */
+#define QCS_ROI_WORDS(R)     ((R) * ((R) + 1))
+#define QCS_ROITYPE_SINGLE   0
+#define QCS_ROITYPE_MULTI    1
+
+/* ---- WLE pre-estimation record: float[6] per ROI
+ *      (DLL/WLEParaEstimation/WLEParaEstimation_Parameters.h:20-34) */
+#define QCS_WLE_FLOATS       6
+#define QCS_WLE_NEARDIST     0
+#define QCS_WLE_SIGMAL       1
+#define QCS_WLE_SIGMAR       2
+#define QCS_WLE_SIGMAU       3
+#define QCS_WLE_SIGMAD       4
+#define QCS_WLE_FLUOTYPE     5
+#define QCS_MOLTYPE_MLEFIT   0
+#define QCS_MOLTYPE_WLEFIT   1
+#define QCS_MOLTYPE_MULTIFIT 2
+
+/* ---- capacities (bfgs_CommonPara.h:315-316) */
+#define QCS_POINT_NUM_TH     20480
+#define QCS_MAX_POINT_NUM    (20480 * 5)
+
+/* ---- rendering modes (DLL/ImgRender/ImageRender_base.h:36-39) */
+#define QCS_RENDER_PEAKPHOTON_CRLB   0
+#define QCS_RENDER_1PHOTON_CRLB      1
+#define QCS_RENDER_1PHOTON_FIXED     2
+#define QCS_RENDER_PEAKPHOTON_1PIXEL 3
+
+/* ---- fit arithmetic.  EXACT evaluates the reference schedule with the library's own
+ *      deterministic FP32 exp/log (bit-identical to the CPU oracle); FAST uses the gfx950
+ *      hardware exp2/log2 instructions (same schedule, tolerance-level agreement). */
+#define QCS_MATH_EXACT       0
+#define QCS_MATH_FAST        1
+
+/* Run configuration.  Mirrors LocalizationPara (DLL/bfgsMLE_Loc_LD/bfgs_base.h:26-110)
+ * minus the GUI-only fields; defaults are LocalizationPara::LocalizationPara()
+ * (DLL/bfgsMLE_Loc_LD/bfgs_base.cu:18-86). */
+typedef struct qcs_params {
+    /* camera */
+    float Offset;            /* ADU */
+    float KAdc;              /* e-/ADU */
+    float QE;                /* e-/photon */
+    float ReadNoise_e;       /* e- */
+    /* localization */
+    int   ROISize;           /* odd, 5..17 */
+    int   LocType;           /* QCS_LOCTYPE_* */
+    int   MultiEmitterFitEn;
+    int   WLEEn;
+    int   ConsecFitEn;       /* accepted for interface parity; consecutive merging is not on this path */
+    float ConsecFit_DistanceTh_nm;
+    /* rendering */
+    float PixelSize;         /* nm */
+    float PixelZoom;
+    float SNR_th;
+    int   ColorMode_3D;
+    /* image */
+    int   ImageWidth;
+    int   ImageHigh;
+    int   TotalFrameNum;
+    int   SRImageWidth;
+    int   SRImageHigh;
+    /* astigmatic 3D calibration: z = sum p_k d^k, d = sigx^2 - sigy^2 (px^2) */
+    float MinZDepth;
+    float MaxZDepth;
+    float ZDepthCorrFactor;
+    float p4_XGY, p3_XGY, p2_XGY, p1_XGY, p0_XGY;
+    float p4_XLY, p3_XLY, p2_XLY, p1_XLY, p0_XLY;
+    /* rendering selection (ImageRenderData_TypeDef::FluoRenderTop arguments) */
+    int   RenderingMode;     /* QCS_RENDER_* */
+    float FixedLocPrec;      /* nm, for QCS_RENDER_1PHOTON_FIXED */
+    /* library controls (no reference counterpart) */
+    int   FitMath;           /* QCS_MATH_EXACT (default) or QCS_MATH_FAST */
+    int   MaxROIPerBatch;    /* ROI list capacity per extraction batch; default QCS_MAX_POINT_NUM */
+    int   BatchedImageNum;   /* frames stacked per extraction batch (IJJ QC_STORM_.java:138-142); default 1 */
+} qcs_params;
+
+/* Fill `p` with LocalizationPara() defaults (bfgs_base.cu:18-86) and the library controls. */
+void qcs_params_default(qcs_params *p);
+/* SRImageWidth/High = ((int)(W*PixelZoom)+3)/4*4 (bfgs_base.cu:89-98). */
+void qcs_params_update_sr_size(qcs_params *p);
+
+/* Thread-local text of the last error. */
+const char *qcs_last_error(void);
+/* Library version string. */
+const char *qcs_version(void);
+
+/* Number of HIP devices; <0 on runtime error.  Replaces CudaDeviceNum (REL/bfgs_MLE_dll.h:806-821). */
+int qcs_device_count(void);
+
+/* ========================================================================================
+ * Stage API (device pointers).  These replace the methods of the static-library classes the
+ * CPPDLL calls (REL/bfgs_MLE_dll.h:222-373), operating on HBM-resident data.  A context owns
+ * the per-configuration scratch (filter taps, capacity buffers); it is not re-entrant: use one
+ * context per host thread.
+ * ====================================================================================== */
+typedef struct qcs_ctx qcs_ctx;
+
+/* Create a context bound to the current HIP device.  Replaces the Init(LocalizationPara&)
+ * methods (LDROIExtraction.cu:219-286, bfgs_top.cu:234-293, ZeroLocalizationsRemove.cu:104-122). */
+int qcs_ctx_create(const qcs_params *p, qcs_ctx **out);
+/* Re-configure; reallocates only when sizes change (IJ/LocResource.cpp:219-224 re-init rule). */
+int qcs_ctx_set_params(qcs_ctx *ctx, const qcs_params *p);
+int qcs_ctx_destroy(qcs_ctx *ctx);
+
+/* ROI extraction over `nframes` uint16 frames of ImageWidth x ImageHigh stacked vertically
+ * (LDROIExtractData_TypeDef::ExtractMolecules, LDROIExtraction.cu:30-80, with WLE
+ * pre-estimation WLEParaEstimation.cpp:20-40).  Frames are processed in batches of
+ * BatchedImageNum; the candidate list of each batch is in raster order (frame, y, x).
+ * Writes up to `cap` ROI records to d_rois and their WLE records to d_wle; *n_out receives
+ * the number written.  Returns QCS_ERR_CAPACITY (with *n_out = cap) when more were found;
+ * *n_found (may be NULL) receives the number found. */
+int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame,
+                    uint16_t *d_rois, float *d_wle, int cap, int *n_out, int *n_found,
+                    void *stream);
+
+/* WLE pre-estimation only (WLEParameterEstimation_TypeDef::WLEParameterEstimate,
+ * WLEParaEstimation.cpp:20-40): PSF widths, per-frame nearest-neighbour distance, type. */
+int qcs_wle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_wle, void *stream);
+
+/* BFGS MLE fitting + CRLB (LDLocData_TypeDef::BFGS_MLELocalization, bfgs_top.cu:26-204).
+ * Row i of d_loc (12 floats) belongs to ROI i; in multi-emitter mode accepted second
+ * emitters are appended at rows >= n.  `loc_cap` rows must be available (>= n; 2n covers
+ * every append).  *n_rows receives the number of rows written (oValidFluoNum).
+ * d_wle may be NULL when WLEEn == 0 and MultiEmitterFitEn == 0. */
+int qcs_fit_dev(qcs_ctx *ctx, const uint16_t *d_rois, const float *d_wle, int n,
+                float *d_loc, int loc_cap, int *n_rows, void *stream);
+
+/* Fit-stage counters of the last qcs_fit_dev call (FitRatio_* of bfgs_top.cu:68-173). */
+typedef struct qcs_fit_stats {
+    int   n_in, n_single, n_2e, n_3e, n_rejected, n_rows;
+    float FitRatio_1E, FitRatio_2E, FitRatio_3E;
+    float FitRatio_Final_1E, FitRatio_Final_2E, FitRatio_Final_3E, FitRatio_Final_4E;
+} qcs_fit_stats;
+int qcs_fit_stats_get(qcs_ctx *ctx, qcs_fit_stats *out);
+
+/* CRLB only (LocPrecCalc_GaussianCRLB.cu:23-40): fills CRBX/CRBY of rows [0, n). */
+int qcs_crlb_dev(qcs_ctx *ctx, float *d_loc, int n, void *stream);
+
+/* Frame-sorted removal of rejected rows (ZeroLocalizationsRemovel_TypeDef::RemoveZero_SortFrame,
+ * ZeroLocalizationsRemove.cu:42-61,161-186): keeps rows with (int)PPho > 1 whose frame lies in
+ * [first, end], ordered by frame, original order within a frame.  *n_out receives the count. */
+int qcs_compact_sorted_dev(qcs_ctx *ctx, const float *d_loc, int n, int first_frame, int end_frame,
+                           float *d_out, int *n_out, void *stream);
+/* Unsorted removal (RemoveZeroWithoutSortFrame, ZeroLocalizationsRemove.cu:64-102): keeps rows
+ * with PPho > 1.0f in original order. */
+int qcs_compact_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_out, int *n_out, void *stream);
+
+/* Splat-render rows into the float SR image d_img[SRImageHigh][SRImageWidth] (accumulating),
+ * d_zimg (3D z-weighted plane, may be NULL for 2D) -- FluoRenderWithPrec_2D/3D
+ * (ImageRender_2d.cu:55-155, ImageRender_3d.cu:60-147). */
+int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream);
+
+/* ========================================================================================
+ * Host-pointer stage API.  Same semantics as the *_dev calls, with the H2D / D2H copies the
+ * reference performs inside the class methods (e.g. bfgs_top.cu:42-47,201-202).
+ * ====================================================================================== */
+int qcs_fit_rois(qcs_ctx *ctx, const uint16_t *rois, const float *wle, int n,
+                 float *loc, int loc_cap, int *n_rows);
+int qcs_extract(qcs_ctx *ctx, const uint16_t *frames, int nframes, int start_frame,
+                uint16_t *rois, float *wle, int cap, int *n_out, int *n_found);
+
+/* ========================================================================================
+ * Online API.  Mirrors the lm_* JNI entry points (IJ/QC_STORM_CPPDLL.h:15-174,
+ * MM/QC_STORM_CPPDLL.cpp:38-924): frames are queued and localized by an internal worker
+ * thread; localizations are frame-sorted and accumulated; the SR image is rendered on device.
+ * ====================================================================================== */
+typedef struct qcs_session qcs_session;
+
+/* lm_SetLocPara / lm_SetLocPara3D / lm_SetImagePara + lm_StartLocThread. */
+int qcs_session_create(const qcs_params *p, qcs_session **out);
+/* lm_FeedImageData(short[], n): copies `nframes` frames into a bounded queue (blocks while the
+ * queue holds `max_queued_frames` frames, the reference's back-pressure). */
+int qcs_session_feed(qcs_session *s, const uint16_t *frames, int nframes);
+/* lm_StopLocThread: process everything queued, then stop the worker. */
+int qcs_session_finish(qcs_session *s);
+/* lm_IsLocFinish: 1 when the worker has drained the queue after finish, else 0. */
+int qcs_session_is_finished(qcs_session *s);
+/* Localizations accumulated so far (frame-sorted).  Copies min(count, cap) rows into `out12`
+ * (may be NULL to query) and returns the total count (<0 on error). */
+long long qcs_session_fetch_locs(qcs_session *s, float *out12, long long cap);
+/* lm_GetSMLMImage: copies the SRImageWidth*SRImageHigh float image. */
+int qcs_session_get_sr_image(qcs_session *s, float *img);
+/* lm_GetMaxDispVal: max of (int) SR image value. */
+int qcs_session_get_max_disp_val(qcs_session *s, int *val);
+/* lm_ReleaseResource. */
+int qcs_session_destroy(qcs_session *s);
+
+/* lm_LocBatchedImg (MM/QC_STORM_CPPDLL.cpp:857-924): synchronous extract + fit + frame-sorted
+ * compaction of `nframes` frames; returns the number of rows (<0 on error) and copies up to
+ * `cap` of them into out12.  Frames are numbered from `start_frame`. */
+long long qcs_loc_batch_sync(qcs_ctx *ctx, const uint16_t *frames, int nframes, int start_frame,
+                             float *out12, long long cap);
+
+/* ========================================================================================
+ * Synthetic data (deterministic simulator; no reference counterpart).  Philox4x32-10 keyed
+ * by (seed, unit, pixel); Poisson by inversion (lambda < 10) or PTRS (Hoermann 1993).
+ * ====================================================================================== */
+typedef struct qcs_sim_params {
+    int      width, height;        /* ROI size (R x R) or frame size */
+    float    offset, kadc, qe;     /* ADU = min(65535, floor(e/kadc + 0.5) + offset) */
+    float    photons_min, photons_max;   /* log-uniform */
+    float    sigma_min, sigma_max;       /* uniform (2D) */
+    float    bg_min, bg_max;             /* uniform, photons/px */
+    float    density_per_px;             /* frames: emitters per pixel per frame */
+    float    center_jitter;              /* ROIs: centre uniform within +-jitter of R/2 */
+    int      astig;                      /* 1: sigma_x/sigma_y from z (astigmatic model) */
+    float    z_min, z_max, s0, c_nm, d_nm;   /* astigmatic model */
+    int      n_per_unit_max;             /* ROIs: 1..n emitters (multi-emitter ROIs) */
+    float    min_sep, max_sep;           /* ROIs with >1 emitter: separation range, px */
+} qcs_sim_params;
+void qcs_sim_params_default(qcs_sim_params *sp);
+
+/* Ground truth per emitter: [unit, x, y, z, photons, bg, sigx, sigy] (8 floats). */
+#define QCS_TRUTH_FLOATS 8
+
+/* Simulate `n` independent R x R ROI records (headers: Xc = Yc = 100, frame = first_frame + i)
+ * and their ground truth (n_per_unit_max floats-of-8 per ROI, unused slots zero).  Host and
+ * device versions produce identical records. */
+int qcs_sim_rois_host(const qcs_sim_params *sp, uint64_t seed, int n, int first_frame,
+                      uint16_t *rois, float *truth);
+int qcs_sim_rois_dev(const qcs_sim_params *sp, uint64_t seed, int n, int first_frame,
+                     uint16_t *d_rois, float *d_truth, void *stream);
+/* Number of emitters the simulator places in frame `frame` (deterministic). */
+int qcs_sim_frame_emitters(const qcs_sim_params *sp, uint64_t seed, int frame, float *truth, int cap);
+/* Simulate `nframes` frames width x height starting at frame index `first_frame`. */
+int qcs_sim_frames_host(const qcs_sim_params *sp, uint64_t seed, int first_frame, int nframes,
+                        uint16_t *frames);
+int qcs_sim_frames_dev(const qcs_sim_params *sp, uint64_t seed, int first_frame, int nframes,
+                       uint16_t *d_frames, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QCSTORM_H */

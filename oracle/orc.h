@@ -1,0 +1,122 @@
+/*
+ * orc.h -- CPU oracle for the QC-STORM localization hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the tests, the smoke check and the
+ * benchmark's CPU-baseline leg compare the HIP library against.  The product library
+ * (libqcstorm.so) never links, loads or calls it.
+ *
+ * It is a plain-C restatement of the reference algorithm, function by function (each function
+ * cites the reference file:line it follows).  Paths are relative to
+ *   DLL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/bfgsMLE_dll
+ * Arithmetic: FP32 with every multiply-add written as an explicit fmaf (built with
+ * -ffp-contract=off), IEEE division and sqrt.  Transcendentals: the reference's __expf/__logf
+ * cannot be reproduced; the oracle offers
+ *   ORC_TMODE_DET  -- orc_qexpf/orc_qlogf, deterministic FP32 routines (<= 2 ulp, pinned
+ *                     against double-precision exp/log by the tests);
+ *   ORC_TMODE_LIBM -- the C library's expf/logf.
+ * and two evaluations of the Gaussian model inside the fit loss:
+ *   ORC_MODEL_SEPARABLE -- exp(-s dx^2) * exp(-s dy^2) (one exp per row/column);
+ *   ORC_MODEL_LITERAL   -- exp(-s (dx^2 + dy^2)) per pixel, as written in the reference.
+ * Both are the same function up to FP32 rounding.  Every fitter also exists in FP64
+ * (the same schedule in double precision) to measure FP32 error.
+ */
+#ifndef QCS_ORACLE_H
+#define QCS_ORACLE_H
+
+#include <stdint.h>
+#include "../include/qcstorm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_TMODE_DET        0
+#define ORC_TMODE_LIBM       1
+#define ORC_MODEL_SEPARABLE  0
+#define ORC_MODEL_LITERAL    1
+
+void  orc_set_modes(int tmode, int model);
+void  orc_get_modes(int *tmode, int *model);
+
+/* deterministic FP32 exp / natural log */
+float orc_qexpf(float x);
+float orc_qlogf(float x);
+void  orc_qexpf_v(const float *x, float *y, long n);
+void  orc_qlogf_v(const float *x, float *y, long n);
+
+/* ---------------- extraction (DLL/SubregionExtraction) ---------------- */
+void  orc_filter_taps(int roi_size, int loc_type, int multi, float taps[5]);
+void  orc_remove_background(const uint16_t *in, uint16_t *out, int w, int bh);
+float orc_image_variance(const uint16_t *img, int w, int bh);
+void  orc_conv_h(const uint16_t *in, uint16_t *out, int w, int bh, const float taps[5]);
+void  orc_conv_v(const uint16_t *in, uint16_t *out, int w, int bh, const float taps[5]);
+/* returns the number of candidates found; writes min(found, cap) (x, y) pairs in raster order
+ * and the mark image (uint8, w*bh). */
+int   orc_find_rois(const uint16_t *filt, int w, int bh, int h, float variance, int loc_type,
+                    int multi, int roi_size, uint8_t *mark, int *xy, int cap);
+void  orc_classify_rois(const uint8_t *mark, const int *xy, int n, int multi, int roi_size, int w,
+                        int *fit_type);
+void  orc_cut_rois(const uint16_t *raw, const int *xy, const int *fit_type, int n, int roi_size,
+                   int w, int bh, int h, int start_frame, uint16_t *rois);
+/* Whole extraction of nframes frames in batches of p->BatchedImageNum, followed by the WLE
+ * pre-estimation of each batch (the reference always runs it: WLE_ENABLE 1).  Each batch keeps
+ * at most p->MaxROIPerBatch candidates.  Returns the number of ROI records written (<= cap);
+ * *n_found (may be NULL) receives the number of candidates found.  `filtered` (may be NULL)
+ * receives the detection image of the last batch, `variance_out` (may be NULL) the per-batch
+ * variances. */
+int   orc_extract(const qcs_params *p, const uint16_t *frames, int nframes, int start_frame,
+                  uint16_t *rois, float *wle, int cap, int *n_found, uint16_t *filtered,
+                  float *variance_out);
+
+/* ---------------- WLE pre-estimation (DLL/WLEParaEstimation) ---------------- */
+void  orc_spline_interp(const float *x_i, const float *y_i, int n_in, const float *x_o, float *y_o,
+                        int n_out);
+void  orc_wle_psf_width(const uint16_t *rois, int n, int roi_size, float *wle);
+void  orc_wle_nearest(const uint16_t *rois, int n, int roi_size, float *wle);
+void  orc_wle_classify(const uint16_t *rois, int n, int roi_size, int loc_type, float *wle);
+void  orc_wle(const uint16_t *rois, int n, int roi_size, int loc_type, float *wle);
+
+/* ---------------- fitting (DLL/bfgsMLE_Loc_LD, DLL/bfgsMLE_Loc_HD, DLL/bfgsMLE_Loc_top) ---- */
+typedef struct orc_fit_stats {
+    int n_in, n_single, n_2e, n_3e, n_rejected, n_rows;
+} orc_fit_stats;
+/* full cascade + CRLB; returns rows written (oValidFluoNum) */
+int   orc_fit(const qcs_params *p, const uint16_t *rois, const float *wle, int n, float *loc,
+              int loc_cap, orc_fit_stats *st);
+/* same schedule in FP64 (transcendentals from libm, double); loc in double */
+int   orc_fit_f64(const qcs_params *p, const uint16_t *rois, const float *wle, int n, double *loc,
+                  int loc_cap, orc_fit_stats *st);
+/* single-kernel entry points: fit the listed ROIs with one model and return the raw fitted
+ * parameter vectors (scaled units, as Ininf at the end of the BFGS loop); `iters` overrides
+ * the reference iteration count when > 0.  model: 0 GS2D-1E, 1 AS3D-1E, 2 GS2D-2E, 3 GS2D-3E,
+ * 4 AS3D-2E.  Returns the parameter count. */
+int   orc_fit_params(const qcs_params *p, int model, const uint16_t *rois, const float *wle, int n,
+                     float *theta, int iters);
+int   orc_fit_params_f64(const qcs_params *p, int model, const uint16_t *rois, const float *wle,
+                         int n, double *theta, int iters);
+/* loss value of `model` at theta for ROI 0 (scaled units); FP32 and FP64 */
+float orc_loss(const qcs_params *p, int model, const uint16_t *roi, const float *wle, const float *theta);
+
+/* ---------------- CRLB (DLL/bfgsMLE_Loc_top/LocPrecCalc_GaussianCRLB.h) ---------------- */
+void  orc_crlb(const qcs_params *p, float *loc, int n);
+void  orc_crlb_f64(const qcs_params *p, double *loc, int n);
+
+/* ---------------- compaction / render ---------------- */
+int   orc_compact_sorted(const float *loc, int n, int first, int end, float *out);
+int   orc_compact(const float *loc, int n, float *out);
+void  orc_render(const qcs_params *p, const float *loc, int n, float *img, float *zimg);
+
+/* ---------------- FP64 maximum-likelihood truth tools ---------------- */
+/* Converged unweighted Poisson MLE of K emitters (K = 1..3) sharing sigma (2D) or with
+ * per-emitter sigma_x/sigma_y (astig = 1) and one background, in photon units, pixel-edge
+ * coordinates.  theta layout: [N_k, x_k, y_k, (sx_k, sy_k | sigma shared)]..., bg.
+ * Returns iterations used (<0 on failure).  loglik receives the final log-likelihood. */
+int   orc_mle_f64(const double *img, int R, int K, int astig, double *theta, int max_iter,
+                  double *loglik);
+/* Full-Fisher CRLB (standard deviations) at theta for the same models. */
+int   orc_fisher_crlb_f64(int R, int K, int astig, const double *theta, double *crlb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

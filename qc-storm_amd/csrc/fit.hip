@@ -1,0 +1,544 @@
+// fit.hip -- BFGS Poisson-MLE Gaussian fitting and CRLB kernels for gfx950.
+//
+// One ROI per lane of a wave64.  The ROI's photo-electron pixels q and log q live in VGPRs for the
+// whole fit; the Gaussian model is evaluated separably (one exp per ROI row and column), so a
+// pixel evaluation of the loss costs one FMA for the model, one log and three FMAs for the
+// weighted Poisson deviance.  The reference's fixed schedule (gradient, 2 + 10 line-search
+// evaluations, gradient, BFGS update; DLL/bfgsMLE_Loc_LD/bfgsMLE_GS2D_core.h:313-426) is run as a
+// uniform state machine over its loss evaluations, so the loss body is emitted once and every
+// lane of the wave executes the same instruction stream (no divergence: the schedule does not
+// depend on the data; the line-search side each lane updates is a per-lane select).
+//
+// Reference kernels (DLL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/bfgsMLE_dll):
+//   bfgsMLELoc_Gauss2D   DLL/bfgsMLE_Loc_LD/bfgsMLE_GS2D_core.h:56-310
+//   bfgsMLELoc_AS3D      DLL/bfgsMLE_Loc_LD/bfgsMLE_AS3D_core.h:57-337
+//   GaussianCRLB_Calc_top DLL/bfgsMLE_Loc_top/LocPrecCalc_GaussianCRLB.h:64-243
+#include "qcs_internal.h"
+#include "qcs_devmath.h"
+
+namespace qcs {
+
+enum { MODEL_GS2D = 0, MODEL_AS3D = 1 };
+
+template <int MODEL> struct Traits;
+template <> struct Traits<MODEL_GS2D> {   // bfgsMLE_GS2D_core.h:25-37
+    static constexpr int P = 5, ITERS = 8;
+    static constexpr float CLAMP = 0.0f;
+};
+template <> struct Traits<MODEL_AS3D> {   // bfgsMLE_AS3D_core.h:25-38, loss clamp :501-505
+    static constexpr int P = 6, ITERS = 11;
+    static constexpr float CLAMP = 0.01f;
+};
+
+FitConst make_fit_const(const qcs_params &p)
+{
+    FitConst c;
+    c.Offset = p.Offset; c.KAdc = p.KAdc; c.QE = p.QE; c.ReadNoise_e = p.ReadNoise_e; c.PixelSize = p.PixelSize;
+    c.MultiEmitterFitEn = p.MultiEmitterFitEn; c.WLEEn = p.WLEEn; c.ZDepthCorrFactor = p.ZDepthCorrFactor;
+    c.pG[0] = p.p4_XGY; c.pG[1] = p.p3_XGY; c.pG[2] = p.p2_XGY; c.pG[3] = p.p1_XGY; c.pG[4] = p.p0_XGY;
+    c.pL[0] = p.p4_XLY; c.pL[1] = p.p3_XLY; c.pL[2] = p.p2_XLY; c.pL[3] = p.p1_XLY; c.pL[4] = p.p0_XLY;
+    return c;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weighted Poisson deviance of the one-emitter model (separable evaluation).  Clamps p in place
+// as the reference's loss does with its argument array.
+template <int MODEL, int R, bool FAST>
+__device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
+                                         const float (&wy)[R], float (&p)[Traits<MODEL>::P])
+{
+    constexpr int P = Traits<MODEL>::P;
+#pragma unroll
+    for (int i = 0; i < P; i++) p[i] = (p[i] < Traits<MODEL>::CLAMP) ? Traits<MODEL>::CLAMP : p[i];
+    const float a = p[0] * 128.0f;
+    const float sx = p[3] * 0.1f;
+    const float sy = (MODEL == MODEL_AS3D) ? p[4] * 0.1f : sx;
+    const float b = p[P - 1] * 64.0f;
+    float ex[R], ay[R];
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        const float dx = ((float)c + 0.5f) - p[1];
+        ex[c] = vexp<FAST>((-(dx * dx)) * sx);
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const float dy = ((float)r + 0.5f) - p[2];
+        ay[r] = a * vexp<FAST>((-(dy * dy)) * sy);
+    }
+    float total = 0.0f;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            const float I = __builtin_fmaf(ay[r], ex[c], b);
+            const float qq = q[r * R + c];
+            const float t = __builtin_fmaf(-qq, vlog<FAST>(I) - lq[r * R + c], I - qq);
+            total = __builtin_fmaf(wy[r] * wx[c], t, total);
+        }
+    }
+    return total;
+}
+
+template <int P>
+__device__ __forceinline__ void bfgs_update(float (&D)[P * P], const float (&s)[P], const float (&y)[P])
+{
+    // ConstructD0 (bfgsMLE_core.h:87-182)
+    float sy = 0.0f;
+#pragma unroll
+    for (int k = 0; k < P; k++) sy = __builtin_fmaf(s[k], y[k], sy);
+    sy = 1.0f / sy;
+    float ty[P], ts[P];
+#pragma unroll
+    for (int k = 0; k < P; k++) { ty[k] = y[k] * sy; ts[k] = s[k] * sy; }
+    float A[P * P], T[P * P];
+#pragma unroll
+    for (int r = 0; r < P; r++)
+#pragma unroll
+        for (int c = 0; c < P; c++) A[r * P + c] = __builtin_fmaf(-s[r], ty[c], (r == c) ? 1.0f : 0.0f);
+#pragma unroll
+    for (int r = 0; r < P; r++)
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < P; k++) acc = __builtin_fmaf(A[r * P + k], D[k * P + c], acc);
+            T[r * P + c] = acc;
+        }
+#pragma unroll
+    for (int r = 0; r < P; r++)
+#pragma unroll
+        for (int c = 0; c < P; c++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < P; k++) acc = __builtin_fmaf(T[r * P + k], A[c * P + k], acc);
+            D[r * P + c] = acc;
+        }
+#pragma unroll
+    for (int r = 0; r < P; r++)
+#pragma unroll
+        for (int c = 0; c < P; c++) D[r * P + c] = D[r * P + c] + s[r] * ts[c];
+}
+
+template <int P>
+__device__ __forceinline__ void normalize_dir(float (&d)[P])
+{
+    float tot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < P; k++) tot = tot + __builtin_fabsf(d[k]);
+    tot = (float)P / tot;
+#pragma unroll
+    for (int k = 0; k < P; k++) d[k] = d[k] * tot;
+}
+
+enum { PH_G0 = 0, PH_G1 = 1, PH_L0 = 2, PH_L1 = 3, PH_BS = 4 };
+
+// Fixed-schedule BFGS (MLELocalization_GS2D, bfgsMLE_GS2D_core.h:313-426, with the gradient of
+// NumericalGradient_GS2D :429-454).  x holds the initial vector on entry, the result on exit.
+// Schedule: gradient (2P evaluations); per iteration: probes at 1e-4 and 1, ten bisection
+// evaluations (eleven halvings of [1e-5, 1]), step, clamp >= 0.01, and -- except after the
+// last iteration -- gradient, BFGS update and new direction.
+template <int MODEL, int R, bool FAST>
+__device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
+                                            const float (&wy)[R], float (&x)[Traits<MODEL>::P])
+{
+    constexpr int P = Traits<MODEL>::P, IT = Traits<MODEL>::ITERS;
+    constexpr int NEVAL = 2 * P + IT * 12 + (IT - 1) * 2 * P;
+    float D[P * P], g[P], gprev[P], s[P], d[P], t[P];
+#pragma unroll
+    for (int r = 0; r < P; r++)
+#pragma unroll
+        for (int c = 0; c < P; c++) D[r * P + c] = (r == c) ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < P; k++) { t[k] = x[k]; g[k] = 0.0f; gprev[k] = 0.0f; s[k] = 0.0f; d[k] = 0.0f; }
+    float L0 = 0.0f, L1 = 0.0f, lo = 0.0f, hi = 0.0f, ln = 0.0f;
+    bool sel = false;
+    int phase = PH_G0, gi = 0, bsn = 0;
+    bool first = true;
+
+    for (int e = 0; e < NEVAL; e++) {
+        float pr[P];
+        if (phase <= PH_G1) {
+            const float delta = (phase == PH_G0) ? -0.002f : 0.004f;
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                if (k == gi) t[k] = t[k] + delta;
+                pr[k] = t[k];
+            }
+        } else {
+            float step = (phase == PH_L0) ? 0.0001f : 1.0f;
+            if (phase == PH_BS) step = sel ? hi : lo;
+#pragma unroll
+            for (int k = 0; k < P; k++) pr[k] = __builtin_fmaf(d[k], step, x[k]);
+        }
+
+        const float L = loss_1e<MODEL, R, FAST>(q, lq, wx, wy, pr);
+
+        if (phase == PH_G0) {
+#pragma unroll
+            for (int k = 0; k < P; k++) t[k] = pr[k];
+            ln = L;
+            phase = PH_G1;
+        } else if (phase == PH_G1) {
+#pragma unroll
+            for (int k = 0; k < P; k++) {
+                t[k] = pr[k];
+                if (k == gi) {
+                    g[k] = (L - ln) * 250.0f;
+                    t[k] = t[k] - 0.002f;
+                }
+            }
+            gi++;
+            phase = PH_G0;
+            if (gi == P) {
+                if (first) {
+                    first = false;
+#pragma unroll
+                    for (int k = 0; k < P; k++) d[k] = -g[k];
+                } else {
+                    float y[P];
+#pragma unroll
+                    for (int k = 0; k < P; k++) y[k] = g[k] - gprev[k];
+                    bfgs_update<P>(D, s, y);
+#pragma unroll
+                    for (int r = 0; r < P; r++) {     // MatMulVector (bfgsMLE_core.h:184-204)
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < P; c++) acc = __builtin_fmaf(-D[r * P + c], g[c], acc);
+                        d[r] = acc;
+                    }
+                }
+                normalize_dir<P>(d);
+                lo = 0.00001f;
+                hi = 1.0f;
+                phase = PH_L0;
+            }
+        } else if (phase == PH_L0) {
+            L0 = L;
+            phase = PH_L1;
+        } else {
+            if (phase == PH_L1) {
+                L1 = L;
+                bsn = 0;
+            } else {
+                if (sel) L1 = L; else L0 = L;
+                bsn++;
+            }
+            sel = (L0 < L1);
+            const float mid = (lo + hi) * 0.5f;
+            if (sel) hi = mid; else lo = mid;
+            phase = PH_BS;
+            if (bsn == 10) {
+                const float scale = (lo + hi) * 0.5f;
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    s[k] = d[k] * scale;
+                    x[k] = x[k] + s[k];
+                }
+#pragma unroll
+                for (int k = 0; k < P; k++) {
+                    x[k] = (x[k] < 0.01f) ? 0.01f : x[k];   // IninfConstrain (bfgsMLE_core.h:76-85)
+                    gprev[k] = g[k];
+                    t[k] = x[k];
+                }
+                gi = 0;
+                phase = PH_G0;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-ROI setup shared by the one-emitter kernels: photo-electrons, logs, header.
+template <int R>
+__device__ __forceinline__ void load_roi(const uint16_t *__restrict__ rec, const FitConst &fc, float (&q)[R * R],
+                                         float (&lq)[R * R], int &xo, int &yo, int &fr)
+{
+    // the record is R*(R+1) 16-bit words; read it as 32-bit words (R*(R+1) is even)
+    constexpr int NW = R * (R + 1) / 2;
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(rec);
+    uint32_t v[NW];
+#pragma unroll
+    for (int k = 0; k < NW; k++) v[k] = __builtin_nontemporal_load(w + k);
+#pragma unroll
+    for (int i = 0; i < R * R; i++) {
+        const uint32_t word = v[i >> 1];
+        const float adu = (float)((i & 1) ? (word >> 16) : (word & 0xffffu));
+        const float e = (adu - fc.Offset) * fc.KAdc;     // bfgsMLE_GS2D_core.h:144
+        q[i] = (e > 1.0f) ? e : 1.0f;
+        lq[i] = qlogf(q[i]);
+    }
+    auto word_at = [&](int idx) -> int {
+        const uint32_t word = v[idx >> 1];
+        return (int)((idx & 1) ? (word >> 16) : (word & 0xffffu));
+    };
+    xo = word_at(R * R + 0);
+    yo = word_at(R * R + 1);
+    fr = word_at(R * R + 2) + 65536 * word_at(R * R + 3);
+}
+
+// WLEWeightsCalc (bfgsMLE_core.h:206-238), separable form: w(r, c) = wy[r] * wx[c]
+template <int R>
+__device__ __forceinline__ void wle_weights(float sgx, float sgy, bool mlefit, float (&wx)[R], float (&wy)[R])
+{
+    const float ctr = (float)R / 2.0f;
+    const float sx1 = 1.0f / ((2.0f * sgx) * sgx);
+    const float sy1 = 1.0f / ((2.0f * sgy) * sgy);
+#pragma unroll
+    for (int c = 0; c < R; c++) {
+        const float d = ((float)c + 0.5f) - ctr;
+        wx[c] = mlefit ? 1.0f : qexpf(-((d * d) * sx1));
+        wy[c] = mlefit ? 1.0f : qexpf(-((d * d) * sy1));
+    }
+}
+
+// Border sums of the ROI (PreEstimation_GS2D, bfgsMLE_GS2D_core.h:529-546).
+template <int R>
+__device__ __forceinline__ float border_bkg(const float (&q)[R * R])
+{
+    float m1 = 0.0f, m2 = 0.0f, m3 = 0.0f, m4 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        m1 = m1 + q[k * R + 0];
+        m2 = m2 + q[k * R + R - 1];
+        m3 = m3 + q[0 * R + k];
+        m4 = m4 + q[(R - 1) * R + k];
+    }
+    m1 = m1 < m2 ? m1 : m2;
+    m3 = m3 < m4 ? m3 : m4;
+    return ((m1 + m3) / 2.0f) / (float)R;
+}
+
+__device__ __forceinline__ float zpoly(const float (&pc)[5], float d, float corr)
+{
+    // bfgsMLE_AS3D_core.h:266-279 with d^k formed by multiplication
+    const float d2 = d * d, d3 = d2 * d, d4 = d2 * d2;
+    float z = pc[0] * d4;
+    z = __builtin_fmaf(pc[1], d3, z);
+    z = __builtin_fmaf(pc[2], d2, z);
+    z = __builtin_fmaf(pc[3], d, z);
+    z = z + pc[4];
+    return z * corr;
+}
+
+// One-emitter fitting kernel.  Lane i fits ROI idx[i] (or i when idx is null) and writes its
+// 12-float row; flag[roi] (if non-null) is set to 1 when a rejected fit qualifies for the
+// two-emitter stage (valid background and SNR > 6, bfgsMLE_GS2D_core.h:279-297).
+template <int MODEL, int R, bool FAST>
+__global__ void __launch_bounds__(64) fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
+                                                    const int *__restrict__ idx, int n, float *__restrict__ loc,
+                                                    uint8_t *__restrict__ flag, FitConst fc)
+{
+    constexpr int P = Traits<MODEL>::P;
+    constexpr int HALF = R / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int roi = idx ? idx[i] : i;
+
+    float q[R * R], lq[R * R];
+    int xo, yo, fr;
+    load_roi<R>(rois + (size_t)roi * (R * (R + 1)), fc, q, lq, xo, yo, fr);
+
+    const float *wp = wle + (size_t)roi * QCS_WLE_FLOATS;
+    const float sl = wp[1], sr = wp[2], su = wp[3], sd = wp[4];
+    const int mt = (int)wp[5];
+    const bool mlefit = (mt == QCS_MOLTYPE_MLEFIT);
+
+    const float bkg = border_bkg<R>(q);
+    const float amp = q[HALF * R + HALF] - bkg;
+    float x[P], wx[R], wy[R];
+    if (MODEL == MODEL_GS2D) {
+        // bfgsMLELoc_Gauss2D :188-228, PreEstimation_GS2D :548-574
+        float ws = fminf(fminf(sl, sr), fminf(su, sd));
+        ws = fminf(ws, ((float)R / 2.0f) / 2.35f);
+        const float ps = 1.0f / ((2.0f * ws) * ws);
+        x[0] = amp * 0.0078125f;
+        x[1] = (float)HALF + 0.5f;
+        x[2] = (float)HALF + 0.5f;
+        x[3] = ps * 10.0f;
+        x[4] = bkg * 0.015625f;
+        if (!mlefit) {
+            ws = ws / 1.2f;
+            ws = fminf(ws, ((float)R / 2.0f) / 2.35f);
+        }
+        if (fc.WLEEn == 0 && mt != QCS_MOLTYPE_MULTIFIT) ws = (float)R;
+        wle_weights<R>(ws, ws, mlefit, wx, wy);
+    } else {
+        // bfgsMLELoc_AS3D :188-228, PreEstimation_AS3D :578-606
+        float wsx = fminf(sl, sr), wsy = fminf(su, sd);
+        x[0] = amp * 0.0078125f;
+        x[1] = (float)HALF + 0.5f;
+        x[2] = (float)HALF + 0.5f;
+        x[3] = 10.0f * (0.5f / (wsx * wsx));
+        x[4] = 10.0f * (0.5f / (wsy * wsy));
+        x[5] = bkg * 0.015625f;
+        if (!mlefit) {
+            const float cap = ((float)R / 1.6f) / 2.35f;
+            wsx = fminf(wsx, cap);
+            wsy = fminf(wsy, cap);
+        }
+        if (fc.WLEEn == 0 && mt != QCS_MOLTYPE_MULTIFIT) { wsx = (float)R; wsy = (float)R; }
+        wle_weights<R>(wsx, wsy, mlefit, wx, wy);
+    }
+
+    bfgs_fit_1e<MODEL, R, FAST>(q, lq, wx, wy, x);
+
+    float row[QCS_REC_FLOATS];
+    bool ok;
+    float snr;
+    bool valid;
+    const float QE = fc.QE;
+    if (MODEL == MODEL_GS2D) {
+        // bfgsMLELoc_Gauss2D :236-308
+        const float lb = (float)R / 2.0f - 1.5f, ub = (float)R / 2.0f + 1.5f;
+        const float smax = fc.MultiEmitterFitEn ? ((float)R / 2.0f) / 2.35f : ((float)R / 1.5f) / 2.35f;
+        const float pp = (x[0] * 128.0f) / QE;
+        const float bg = (x[4] * 64.0f) / QE;
+        float fx = x[1], fy = x[2];
+        float sg = sqrtf(0.5f / (x[3] * 0.1f));
+        fx = (fx != fx) ? -1.0f : fx;
+        fy = (fy != fy) ? -1.0f : fy;
+        sg = (sg != sg) ? -1.0f : sg;
+        snr = (sqrtf(QE) * pp) / sqrtf(pp + bg);
+        const float tp = (((pp * 2.0f) * 3.141593f) * sg) * sg;
+        valid = !(bg != bg);
+        ok = valid && snr > 4.0f && sg <= smax && sg >= 0.6f && fx > lb && fx < ub && fy > lb && fy < ub;
+        row[0] = pp; row[1] = (fx - (float)HALF) + (float)xo; row[2] = (fy - (float)HALF) + (float)yo;
+        row[3] = 0.0f; row[4] = sg; row[5] = sg; row[6] = tp; row[7] = bg; row[8] = snr;
+        row[9] = 0.0f; row[10] = 0.0f; row[11] = (float)fr;
+    } else {
+        // bfgsMLELoc_AS3D :237-335
+        const float lb = (float)R / 2.0f - 2.5f, ub = (float)R / 2.0f + 2.5f;
+        const float smax = fc.MultiEmitterFitEn ? ((float)R / 2.0f) / 2.35f : ((float)R / 1.5f) / 2.35f;
+        const float pe = x[0] * 128.0f, be = x[5] * 64.0f;
+        float fx = x[1], fy = x[2];
+        float sx = sqrtf(0.5f / (x[3] * 0.1f));
+        float sy = sqrtf(0.5f / (x[4] * 0.1f));
+        fx = (fx != fx) ? -1.0f : fx;
+        fy = (fy != fy) ? -1.0f : fy;
+        sx = (sx != sx) ? -1.0f : sx;
+        sy = (sy != sy) ? -1.0f : sy;
+        snr = pe / sqrtf(pe + be);
+        const float pp = pe / QE, bg = be / QE;
+        const float tp = (((pp * 2.0f) * 3.141593f) * sx) * sy;
+        const float dif = (sx * sx) - (sy * sy);
+        const float z = (dif >= 0.0f) ? zpoly(fc.pG, dif, fc.ZDepthCorrFactor) : zpoly(fc.pL, dif, fc.ZDepthCorrFactor);
+        const float ms = (sx + sy) / 2.0f;
+        valid = !(bg != bg);
+        ok = valid && snr > 3.5f && ms > 0.8f && ms < smax && fx > lb && fx < ub && fy > lb && fy < ub;
+        row[0] = pp; row[1] = (fx - (float)HALF) + (float)xo; row[2] = (fy - (float)HALF) + (float)yo;
+        row[3] = z; row[4] = sx; row[5] = sy; row[6] = tp; row[7] = bg; row[8] = snr;
+        row[9] = 0.0f; row[10] = 0.0f; row[11] = (float)fr;
+    }
+    if (!ok) {
+#pragma unroll
+        for (int k = 0; k < QCS_REC_FLOATS; k++) row[k] = 0.0f;
+    }
+    float4 *dst = reinterpret_cast<float4 *>(loc + (size_t)roi * QCS_REC_FLOATS);
+    dst[0] = make_float4(row[0], row[1], row[2], row[3]);
+    dst[1] = make_float4(row[4], row[5], row[6], row[7]);
+    dst[2] = make_float4(row[8], row[9], row[10], row[11]);
+    if (flag) flag[roi] = (!ok && valid && snr > 6.0f) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Diagonal-Fisher CRLB of x and y (GaussianCRLB_Calc_top, LocPrecCalc_GaussianCRLB.h:64-128):
+// numeric second derivatives of the model and its log with dh = 0.02 on a G x G grid.
+template <int G>
+__device__ __forceinline__ float crlb_psf(int r, int c, float peak, float ix, float iy, float x0, float y0, float B)
+{
+    const float cy = (float)(r - G / 2), cx = (float)(c - G / 2);
+    const float dx = cx - x0, dy = cy - y0;
+    return __builtin_fmaf(peak, qexpf(-__builtin_fmaf(dx * dx, ix, (dy * dy) * iy)), B);
+}
+
+template <int G>
+__global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n, float QE, float RN, float PS)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float *row = loc + (size_t)i * QCS_REC_FLOATS;
+    float te = row[6] * QE, be = row[7] * QE;
+    const float sx = row[4], sy = row[5];
+    float stdx = 100.0f, stdy = 100.0f;
+    if (te > 1.0f && be > 1.0f && (double)sx > 0.1 && (double)sy > 0.1 && sx < 10.0f && sy < 10.0f) {
+        be = __builtin_fmaf(be, QE, RN * RN) / QE;
+        const float pi = 3.141592654f;
+        const float peak = (1.0f / (((2.0f * pi) * sx) * sy)) * te;
+        const float ix = 1.0f / ((2.0f * sx) * sx);
+        const float iy = 1.0f / ((2.0f * sy) * sy);
+        const float dh = 0.02f;
+        const float xp = 0.0f + dh, xn = (0.0f + dh) - 2.0f * dh;
+        float fx = 0.0f, fy = 0.0f;
+        for (int r = 0; r < G; r++) {
+#pragma unroll
+            for (int c = 0; c < G; c++) {
+                const float m0 = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, 0.0f, be);
+                const float l0 = qlogf(m0);
+                {
+                    const float mp = crlb_psf<G>(r, c, peak, ix, iy, xp, 0.0f, be);
+                    const float mn = crlb_psf<G>(r, c, peak, ix, iy, xn, 0.0f, be);
+                    const float d2 = ((__builtin_fmaf(-2.0f, m0, mp) + mn) / dh) / dh;
+                    const float d2l = ((__builtin_fmaf(-2.0f, l0, qlogf(mp)) + qlogf(mn)) / dh) / dh;
+                    fx = fx + __builtin_fmaf(-m0, d2l, d2);
+                }
+                {
+                    const float mp = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, xp, be);
+                    const float mn = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, xn, be);
+                    const float d2 = ((__builtin_fmaf(-2.0f, m0, mp) + mn) / dh) / dh;
+                    const float d2l = ((__builtin_fmaf(-2.0f, l0, qlogf(mp)) + qlogf(mn)) / dh) / dh;
+                    fy = fy + __builtin_fmaf(-m0, d2l, d2);
+                }
+            }
+        }
+        fx = (fx == 0.0f) ? 1.0f : fx;
+        fy = (fy == 0.0f) ? 1.0f : fy;
+        stdx = sqrtf(1.0f / fx) * PS;
+        stdy = sqrtf(1.0f / fy) * PS;
+    }
+    row[9] = stdx;
+    row[10] = stdy;
+}
+
+// ---------------------------------------------------------------------------------------------
+template <int MODEL, bool FAST>
+static int launch_1e_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, const float *wle, const int *idx, int n,
+                       float *loc, uint8_t *flag, const FitConst &fc)
+{
+    switch (R) {
+    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
+    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
+    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
+    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the one-emitter fitter", R);
+    }
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n, float *d_loc,
+                  uint8_t *d_flag, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    const FitConst fc = make_fit_const(p);
+    const dim3 grid((n + 63) / 64);
+    const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
+    const bool fast = (p.FitMath == QCS_MATH_FAST);
+    if (as3d)
+        return fast ? launch_1e_r<MODEL_AS3D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc)
+                    : launch_1e_r<MODEL_AS3D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc);
+    return fast ? launch_1e_r<MODEL_GS2D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc)
+                : launch_1e_r<MODEL_GS2D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc);
+}
+
+int crlb_launch(const qcs_params &p, float *d_loc, int n, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    const dim3 grid((n + 63) / 64);
+    if (p.LocType == QCS_LOCTYPE_AS3D)
+        hipLaunchKernelGGL(crlb_kernel<15>, grid, dim3(64), 0, s, d_loc, n, p.QE, p.ReadNoise_e, p.PixelSize);
+    else
+        hipLaunchKernelGGL(crlb_kernel<9>, grid, dim3(64), 0, s, d_loc, n, p.QE, p.ReadNoise_e, p.PixelSize);
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+}  // namespace qcs

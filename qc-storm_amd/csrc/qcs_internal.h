@@ -1,0 +1,69 @@
+// qcs_internal.h -- shared declarations of libqcstorm (not part of the public C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "../../include/qcstorm.h"
+
+namespace qcs {
+
+// Thread-local error text; returns `code`.
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define QCS_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t _e = (call);                                              \
+        if (_e != hipSuccess) return qcs::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// Scalar fit configuration passed to kernels by value (CoreFittingPara, bfgs_base.h:113-141).
+struct FitConst {
+    float Offset, KAdc, QE, ReadNoise_e, PixelSize;
+    int MultiEmitterFitEn, WLEEn;
+    float ZDepthCorrFactor;
+    float pG[5];   // p4..p0 for sigma_x >= sigma_y
+    float pL[5];   // p4..p0 for sigma_x <  sigma_y
+};
+FitConst make_fit_const(const qcs_params &p);
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned host buffer.
+struct HostBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+int validate_params(const qcs_params &p);
+
+// fit stage internals (fit.hip)
+int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n,
+                  float *d_loc, uint8_t *d_flag, hipStream_t s);
+int crlb_launch(const qcs_params &p, float *d_loc, int n, hipStream_t s);
+
+}  // namespace qcs
+
+struct qcs_ctx {
+    qcs_params p;
+    int device = 0;
+    qcs_fit_stats stats{};
+    // extraction scratch
+    qcs::DevBuf ex_bkg, ex_tmp, ex_det, ex_mark, ex_cnt, ex_xy, ex_var, ex_frames;
+    // fit scratch
+    qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;
+    // host staging for the host-pointer API
+    qcs::DevBuf h_rois_dev, h_wle_dev, h_loc_dev;
+    qcs::HostBuf pin_cnt;
+    float taps[5];
+};

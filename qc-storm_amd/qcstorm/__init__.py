@@ -1,0 +1,289 @@
+"""ctypes binding of libqcstorm.so (the MI355X localization library).
+
+The library's C-ABI is declared in include/qcstorm.h; this module mirrors its structs and
+prototypes so Python callers (tests, benchmark, notebooks) can drive it with numpy arrays
+(host API) or torch tensors (device API, HBM-resident data).  Loading fails loudly when the
+built library is missing: there is no Python or CPU fallback for the localization path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libqcstorm.so")
+
+QCS_OK = 0
+QCS_ERR_ARG, QCS_ERR_HIP, QCS_ERR_CAPACITY, QCS_ERR_STATE, QCS_ERR_NOMEM, QCS_ERR_NODEVICE = 1, 2, 3, 4, 5, 6
+LOCTYPE_GS2D, LOCTYPE_AS3D, LOCTYPE_DH3D = 0, 1, 2
+MATH_EXACT, MATH_FAST = 0, 1
+REC_FLOATS = 12
+WLE_FLOATS = 6
+TRUTH_FLOATS = 8
+MAX_POINT_NUM = 20480 * 5
+REC_FIELDS = ("PPho", "X", "Y", "Z", "SigX", "SigY", "TPho", "Bkg", "PSNR", "CrbX", "CrbY", "Frame")
+
+
+def roi_words(R):
+    return R * (R + 1)
+
+
+class QcsParams(C.Structure):
+    _fields_ = [
+        ("Offset", C.c_float), ("KAdc", C.c_float), ("QE", C.c_float), ("ReadNoise_e", C.c_float),
+        ("ROISize", C.c_int), ("LocType", C.c_int), ("MultiEmitterFitEn", C.c_int), ("WLEEn", C.c_int),
+        ("ConsecFitEn", C.c_int), ("ConsecFit_DistanceTh_nm", C.c_float),
+        ("PixelSize", C.c_float), ("PixelZoom", C.c_float), ("SNR_th", C.c_float), ("ColorMode_3D", C.c_int),
+        ("ImageWidth", C.c_int), ("ImageHigh", C.c_int), ("TotalFrameNum", C.c_int),
+        ("SRImageWidth", C.c_int), ("SRImageHigh", C.c_int),
+        ("MinZDepth", C.c_float), ("MaxZDepth", C.c_float), ("ZDepthCorrFactor", C.c_float),
+        ("p4_XGY", C.c_float), ("p3_XGY", C.c_float), ("p2_XGY", C.c_float), ("p1_XGY", C.c_float),
+        ("p0_XGY", C.c_float),
+        ("p4_XLY", C.c_float), ("p3_XLY", C.c_float), ("p2_XLY", C.c_float), ("p1_XLY", C.c_float),
+        ("p0_XLY", C.c_float),
+        ("RenderingMode", C.c_int), ("FixedLocPrec", C.c_float),
+        ("FitMath", C.c_int), ("MaxROIPerBatch", C.c_int), ("BatchedImageNum", C.c_int),
+    ]
+
+
+class QcsSimParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int), ("offset", C.c_float), ("kadc", C.c_float), ("qe", C.c_float),
+        ("photons_min", C.c_float), ("photons_max", C.c_float), ("sigma_min", C.c_float), ("sigma_max", C.c_float),
+        ("bg_min", C.c_float), ("bg_max", C.c_float), ("density_per_px", C.c_float), ("center_jitter", C.c_float),
+        ("astig", C.c_int), ("z_min", C.c_float), ("z_max", C.c_float), ("s0", C.c_float), ("c_nm", C.c_float),
+        ("d_nm", C.c_float), ("n_per_unit_max", C.c_int), ("min_sep", C.c_float), ("max_sep", C.c_float),
+    ]
+
+
+class QcsFitStats(C.Structure):
+    _fields_ = [
+        ("n_in", C.c_int), ("n_single", C.c_int), ("n_2e", C.c_int), ("n_3e", C.c_int), ("n_rejected", C.c_int),
+        ("n_rows", C.c_int),
+        ("FitRatio_1E", C.c_float), ("FitRatio_2E", C.c_float), ("FitRatio_3E", C.c_float),
+        ("FitRatio_Final_1E", C.c_float), ("FitRatio_Final_2E", C.c_float), ("FitRatio_Final_3E", C.c_float),
+        ("FitRatio_Final_4E", C.c_float),
+    ]
+
+
+_P = C.POINTER
+_vp, _i, _f, _u64, _ll = C.c_void_p, C.c_int, C.c_float, C.c_uint64, C.c_longlong
+
+# name -> (restype, argtypes); every symbol declared in include/qcstorm.h
+PROTOTYPES = {
+    "qcs_params_default": (None, [_P(QcsParams)]),
+    "qcs_params_update_sr_size": (None, [_P(QcsParams)]),
+    "qcs_last_error": (C.c_char_p, []),
+    "qcs_version": (C.c_char_p, []),
+    "qcs_device_count": (_i, []),
+    "qcs_ctx_create": (_i, [_P(QcsParams), _P(_vp)]),
+    "qcs_ctx_set_params": (_i, [_vp, _P(QcsParams)]),
+    "qcs_ctx_destroy": (_i, [_vp]),
+    "qcs_extract_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _i, _P(_i), _P(_i), _vp]),
+    "qcs_wle_dev": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "qcs_fit_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i), _vp]),
+    "qcs_fit_stats_get": (_i, [_vp, _P(QcsFitStats)]),
+    "qcs_crlb_dev": (_i, [_vp, _vp, _i, _vp]),
+    "qcs_compact_sorted_dev": (_i, [_vp, _vp, _i, _i, _i, _vp, _P(_i), _vp]),
+    "qcs_compact_dev": (_i, [_vp, _vp, _i, _vp, _P(_i), _vp]),
+    "qcs_render_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "qcs_fit_rois": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i)]),
+    "qcs_extract": (_i, [_vp, _vp, _i, _i, _vp, _vp, _i, _P(_i), _P(_i)]),
+    "qcs_session_create": (_i, [_P(QcsParams), _P(_vp)]),
+    "qcs_session_feed": (_i, [_vp, _vp, _i]),
+    "qcs_session_finish": (_i, [_vp]),
+    "qcs_session_is_finished": (_i, [_vp]),
+    "qcs_session_fetch_locs": (_ll, [_vp, _vp, _ll]),
+    "qcs_session_get_sr_image": (_i, [_vp, _vp]),
+    "qcs_session_get_max_disp_val": (_i, [_vp, _P(_i)]),
+    "qcs_session_destroy": (_i, [_vp]),
+    "qcs_loc_batch_sync": (_ll, [_vp, _vp, _i, _i, _vp, _ll]),
+    "qcs_sim_params_default": (None, [_P(QcsSimParams)]),
+    "qcs_sim_rois_host": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp, _vp]),
+    "qcs_sim_rois_dev": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp, _vp, _vp]),
+    "qcs_sim_frame_emitters": (_i, [_P(QcsSimParams), _u64, _i, _vp, _i]),
+    "qcs_sim_frames_host": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp]),
+    "qcs_sim_frames_dev": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libqcstorm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libqcstorm.so not built at %s (run __graft_entry__.build())" % LIB_PATH)
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            fn = getattr(l, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def missing_symbols():
+    """Names declared in the header that the built library does not export."""
+    l = C.CDLL(LIB_PATH)
+    return [n for n in PROTOTYPES if getattr(l, n, None) is None]
+
+
+class QcsError(RuntimeError):
+    def __init__(self, rc, where):
+        msg = lib().qcs_last_error().decode(errors="replace")
+        super().__init__("%s failed (rc=%d): %s" % (where, rc, msg))
+        self.rc = rc
+
+
+def check(rc, where):
+    if rc != QCS_OK:
+        raise QcsError(rc, where)
+    return rc
+
+
+def default_params(**kw):
+    p = QcsParams()
+    lib().qcs_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise AttributeError("qcs_params has no field %r" % k)
+        setattr(p, k, v)
+    lib().qcs_params_update_sr_size(C.byref(p))
+    return p
+
+
+def default_sim(**kw):
+    sp = QcsSimParams()
+    lib().qcs_sim_params_default(C.byref(sp))
+    for k, v in kw.items():
+        if not hasattr(sp, k):
+            raise AttributeError("qcs_sim_params has no field %r" % k)
+        setattr(sp, k, v)
+    return sp
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def sim_rois_host(sp, seed, n, first_frame=1):
+    R = sp.width
+    M = max(1, sp.n_per_unit_max)
+    rois = np.zeros((n, roi_words(R)), np.uint16)
+    truth = np.zeros((n, M, TRUTH_FLOATS), np.float32)
+    check(lib().qcs_sim_rois_host(C.byref(sp), seed, n, first_frame, _ptr(rois), _ptr(truth)), "qcs_sim_rois_host")
+    return rois, truth
+
+
+def sim_frames_host(sp, seed, first_frame, nframes):
+    frames = np.zeros((nframes, sp.height, sp.width), np.uint16)
+    check(lib().qcs_sim_frames_host(C.byref(sp), seed, first_frame, nframes, _ptr(frames)), "qcs_sim_frames_host")
+    return frames
+
+
+def frame_emitters(sp, seed, frame):
+    n = lib().qcs_sim_frame_emitters(C.byref(sp), seed, frame, None, 0)
+    t = np.zeros((max(n, 0), TRUTH_FLOATS), np.float32)
+    lib().qcs_sim_frame_emitters(C.byref(sp), seed, frame, _ptr(t), n)
+    return t
+
+
+class Context:
+    """A qcs_ctx bound to the current HIP device."""
+
+    def __init__(self, params):
+        self.params = params
+        h = C.c_void_p()
+        check(lib().qcs_ctx_create(C.byref(params), C.byref(h)), "qcs_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().qcs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        check(lib().qcs_ctx_set_params(self.h, C.byref(params)), "qcs_ctx_set_params")
+        self.params = params
+
+    # ---- host-pointer API -------------------------------------------------------------------
+    def fit_rois(self, rois, wle, loc_cap=None):
+        n = rois.shape[0]
+        cap = loc_cap or 2 * max(n, 1)
+        loc = np.zeros((cap, REC_FLOATS), np.float32)
+        rows = C.c_int()
+        rois = np.ascontiguousarray(rois, np.uint16)
+        wle = np.ascontiguousarray(wle, np.float32)
+        check(lib().qcs_fit_rois(self.h, _ptr(rois), _ptr(wle), n, _ptr(loc), cap, C.byref(rows)), "qcs_fit_rois")
+        return loc[:rows.value].copy()
+
+    def extract(self, frames, start_frame=1, cap=None):
+        frames = np.ascontiguousarray(frames, np.uint16)
+        nf = frames.shape[0]
+        R = self.params.ROISize
+        cap = cap or MAX_POINT_NUM
+        rois = np.zeros((cap, roi_words(R)), np.uint16)
+        wle = np.zeros((cap, WLE_FLOATS), np.float32)
+        n_out, n_found = C.c_int(), C.c_int()
+        rc = lib().qcs_extract(self.h, _ptr(frames), nf, start_frame, _ptr(rois), _ptr(wle), cap, C.byref(n_out),
+                               C.byref(n_found))
+        if rc not in (QCS_OK, QCS_ERR_CAPACITY):
+            check(rc, "qcs_extract")
+        return rois[:n_out.value].copy(), wle[:n_out.value].copy(), n_found.value
+
+    def stats(self):
+        st = QcsFitStats()
+        check(lib().qcs_fit_stats_get(self.h, C.byref(st)), "qcs_fit_stats_get")
+        return {k: getattr(st, k) for k, _ in QcsFitStats._fields_}
+
+    # ---- device-pointer API (torch tensors on cuda) -----------------------------------------
+    @staticmethod
+    def _dp(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else None
+
+    def fit_dev(self, d_rois, d_wle, n, d_loc, stream=None):
+        rows = C.c_int()
+        cap = d_loc.shape[0]
+        check(lib().qcs_fit_dev(self.h, self._dp(d_rois), self._dp(d_wle), n, self._dp(d_loc), cap, C.byref(rows),
+                                stream), "qcs_fit_dev")
+        return rows.value
+
+    def wle_dev(self, d_rois, n, d_wle, stream=None):
+        check(lib().qcs_wle_dev(self.h, self._dp(d_rois), n, self._dp(d_wle), stream), "qcs_wle_dev")
+
+    def crlb_dev(self, d_loc, n, stream=None):
+        check(lib().qcs_crlb_dev(self.h, self._dp(d_loc), n, stream), "qcs_crlb_dev")
+
+    def extract_dev(self, d_frames, nframes, start_frame, d_rois, d_wle, cap, stream=None):
+        n_out, n_found = C.c_int(), C.c_int()
+        rc = lib().qcs_extract_dev(self.h, self._dp(d_frames), nframes, start_frame, self._dp(d_rois),
+                                   self._dp(d_wle), cap, C.byref(n_out), C.byref(n_found), stream)
+        if rc not in (QCS_OK, QCS_ERR_CAPACITY):
+            check(rc, "qcs_extract_dev")
+        return n_out.value, n_found.value
+
+    def compact_sorted_dev(self, d_loc, n, first, end, d_out, stream=None):
+        m = C.c_int()
+        check(lib().qcs_compact_sorted_dev(self.h, self._dp(d_loc), n, first, end, self._dp(d_out), C.byref(m),
+                                           stream), "qcs_compact_sorted_dev")
+        return m.value
+
+    def compact_dev(self, d_loc, n, d_out, stream=None):
+        m = C.c_int()
+        check(lib().qcs_compact_dev(self.h, self._dp(d_loc), n, self._dp(d_out), C.byref(m), stream),
+              "qcs_compact_dev")
+        return m.value
+
+    def render_dev(self, d_loc, n, d_img, d_zimg=None, stream=None):
+        check(lib().qcs_render_dev(self.h, self._dp(d_loc), n, self._dp(d_img), self._dp(d_zimg), stream),
+              "qcs_render_dev")
