@@ -9,17 +9,19 @@
  *   exp(x): Cody-Waite range reduction x = n ln2 + r with ln2 split in a high and a low part,
  *           |r| <= ln2/2, then the degree-7 Taylor polynomial of e^r (truncation error
  *           r^8/8! < 5.1e-9 relative), scaled by 2^n with ldexp.
- *   log(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1 exact (Sterbenz),
- *           log(1+f) = f - f^2/2 + f^3 g(f) with g a degree-7 polynomial fitted by least
- *           squares on Chebyshev nodes of [sqrt(1/2)-1, sqrt(2)-1] (max error < 1 ulp),
+ *   log(x): x = m 2^e with m in [sqrt(1/2), sqrt(2)) taken from the IEEE bit pattern
+ *           (subnormals pre-scaled by 2^23), f = m - 1 exact (Sterbenz),
+ *           log(1+f) = f + f^2 P(f) with P a degree-8 polynomial fitted by least squares on
+ *           Chebyshev nodes of [sqrt(1/2)-1, sqrt(2)-1] (max error 0.82 ulp of log1p),
  *           plus e ln2 (split constant).
  *
- * Every step is an IEEE-754 single operation (fmaf, rintf, frexpf, ldexpf), so any IEEE
+ * Every step is an IEEE-754 single operation or an integer operation on the bit pattern, so any IEEE
  * implementation evaluates them bit-identically.  The tests pin both routines against
  * double-precision exp/log over their whole argument range.
  */
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 #include "orc.h"
 
 static int g_tmode = ORC_TMODE_DET;
@@ -57,22 +59,28 @@ float orc_qlogf(float x)
     if (x < 0.0f) return NAN;
     if (x == 0.0f) return -INFINITY;
     if (x == INFINITY) return x;
-    int e;
-    float m = frexpf(x, &e);                  /* m in [0.5, 1) */
-    if (m < 0.707106781186547524f) { m = m + m; e = e - 1; }
-    float f = m - 1.0f;                       /* exact */
-    float g = -0.0810227021574974f;
-    g = fmaf(g, f, 0.1271267831325531f);
-    g = fmaf(g, f, -0.12972144782543182f);
-    g = fmaf(g, f, 0.14202114939689636f);
-    g = fmaf(g, f, -0.166422501206398f);
-    g = fmaf(g, f, 0.20001471042633057f);
-    g = fmaf(g, f, -0.2500029504299164f);
-    g = fmaf(g, f, 0.333333283662796f);
-    float f2 = f * f;
-    float r = fmaf(g, f2 * f, -0.5f * f2);
-    r = r + f;
-    float ef = (float)e;
+    int esh = 0;
+    if (x < 1.17549435082228751e-38f) { x = x * 8388608.0f; esh = -23; }   /* subnormal: scale by 2^23 */
+    uint32_t ix;
+    memcpy(&ix, &x, sizeof ix);
+    ix = ix - 0x3f3504f3u;                   /* 0x3f3504f3 = sqrt(1/2) rounded to float */
+    const int e = ((int32_t)ix) >> 23;
+    uint32_t mb = (ix & 0x007fffffu) + 0x3f3504f3u;
+    float m;
+    memcpy(&m, &mb, sizeof m);               /* m in [sqrt(1/2), sqrt(2)) */
+    const float f = m - 1.0f;                /* exact */
+    float p = -0.07866668701171875f;
+    p = fmaf(p, f, 0.12821750342845917f);
+    p = fmaf(p, f, -0.131016343832016f);
+    p = fmaf(p, f, 0.1418900042772293f);
+    p = fmaf(p, f, -0.1662759929895401f);
+    p = fmaf(p, f, 0.2000196874141693f);
+    p = fmaf(p, f, -0.2500077188014984f);
+    p = fmaf(p, f, 0.33333322405815125f);
+    p = fmaf(p, f, -0.4999999701976776f);
+    const float f2 = f * f;
+    const float r = fmaf(p, f2, f);
+    const float ef = (float)(e + esh);
     return fmaf(ef, LN2_HI_F, fmaf(ef, LN2_LO_F, r));
 }
 

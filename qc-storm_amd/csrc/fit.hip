@@ -42,10 +42,13 @@ FitConst make_fit_const(const qcs_params &p)
 
 // ---------------------------------------------------------------------------------------------
 // Weighted Poisson deviance of the one-emitter model (separable evaluation).  Clamps p in place
-// as the reference's loss does with its argument array.
-template <int MODEL, int R, bool FAST>
+// as the reference's loss does with its argument array.  SAFE = false uses the positive-normal
+// fast path of the logarithm and reports in `bad` whether any model value was zero, negative,
+// subnormal or not finite; the caller then re-evaluates with SAFE = true (full special-value
+// handling), which yields the same value the CPU oracle computes.
+template <int MODEL, int R, bool FAST, bool SAFE>
 __device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                         const float (&wy)[R], float (&p)[Traits<MODEL>::P])
+                                         const float (&wy)[R], float (&p)[Traits<MODEL>::P], bool &bad)
 {
     constexpr int P = Traits<MODEL>::P;
 #pragma unroll
@@ -58,31 +61,51 @@ __device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&
 #pragma unroll
     for (int c = 0; c < R; c++) {
         const float dx = ((float)c + 0.5f) - p[1];
-        ex[c] = vexp<FAST>((-(dx * dx)) * sx);
+        ex[c] = vexp_nonpos<FAST>((-(dx * dx)) * sx);
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const float dy = ((float)r + 0.5f) - p[2];
-        ay[r] = a * vexp<FAST>((-(dy * dy)) * sy);
+        ay[r] = a * vexp_nonpos<FAST>((-(dy * dy)) * sy);
     }
     float total = 0.0f;
+    bool nonnormal = false;
 #pragma unroll
     for (int r = 0; r < R; r++) {
 #pragma unroll
         for (int c = 0; c < R; c++) {
             const float I = __builtin_fmaf(ay[r], ex[c], b);
+            float LI;
+            if (SAFE || FAST) {
+                LI = vlog<FAST>(I);
+            } else {
+                LI = qlogf_normal(I);
+                nonnormal = nonnormal || !is_pos_normal(I);
+            }
             const float qq = q[r * R + c];
-            const float t = __builtin_fmaf(-qq, vlog<FAST>(I) - lq[r * R + c], I - qq);
+            const float t = __builtin_fmaf(-qq, LI - lq[r * R + c], I - qq);
             total = __builtin_fmaf(wy[r] * wx[c], t, total);
         }
     }
+    bad = nonnormal;
     return total;
 }
 
-template <int P>
-__device__ __forceinline__ void bfgs_update(float (&D)[P * P], const float (&s)[P], const float (&y)[P])
+template <int MODEL, int R, bool FAST>
+__device__ __forceinline__ float loss_eval(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
+                                          const float (&wy)[R], float (&p)[Traits<MODEL>::P])
 {
-    // ConstructD0 (bfgsMLE_core.h:87-182)
+    bool bad;
+    float L = loss_1e<MODEL, R, FAST, false>(q, lq, wx, wy, p, bad);
+    if (!FAST && bad) L = loss_1e<MODEL, R, FAST, true>(q, lq, wx, wy, p, bad);
+    return L;
+}
+
+// ConstructD0 (bfgsMLE_core.h:87-182).  D is the lane's inverse-Hessian approximation, stored in
+// LDS with element k at D[k * 64] (lane-interleaved: conflict-free, off the VGPR budget).
+template <int P>
+__device__ __forceinline__ void bfgs_update(float *D, const float (&s)[P], const float (&y)[P])
+{
     float sy = 0.0f;
 #pragma unroll
     for (int k = 0; k < P; k++) sy = __builtin_fmaf(s[k], y[k], sy);
@@ -96,14 +119,18 @@ __device__ __forceinline__ void bfgs_update(float (&D)[P * P], const float (&s)[
 #pragma unroll
         for (int c = 0; c < P; c++) A[r * P + c] = __builtin_fmaf(-s[r], ty[c], (r == c) ? 1.0f : 0.0f);
 #pragma unroll
-    for (int r = 0; r < P; r++)
+    for (int c = 0; c < P; c++) {
+        float Dc[P];
 #pragma unroll
-        for (int c = 0; c < P; c++) {
+        for (int k = 0; k < P; k++) Dc[k] = D[(k * P + c) * 64];
+#pragma unroll
+        for (int r = 0; r < P; r++) {
             float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < P; k++) acc = __builtin_fmaf(A[r * P + k], D[k * P + c], acc);
+            for (int k = 0; k < P; k++) acc = __builtin_fmaf(A[r * P + k], Dc[k], acc);
             T[r * P + c] = acc;
         }
+    }
 #pragma unroll
     for (int r = 0; r < P; r++)
 #pragma unroll
@@ -111,12 +138,8 @@ __device__ __forceinline__ void bfgs_update(float (&D)[P * P], const float (&s)[
             float acc = 0.0f;
 #pragma unroll
             for (int k = 0; k < P; k++) acc = __builtin_fmaf(T[r * P + k], A[c * P + k], acc);
-            D[r * P + c] = acc;
+            D[(r * P + c) * 64] = acc + s[r] * ts[c];
         }
-#pragma unroll
-    for (int r = 0; r < P; r++)
-#pragma unroll
-        for (int c = 0; c < P; c++) D[r * P + c] = D[r * P + c] + s[r] * ts[c];
 }
 
 template <int P>
@@ -139,17 +162,19 @@ enum { PH_G0 = 0, PH_G1 = 1, PH_L0 = 2, PH_L1 = 3, PH_BS = 4 };
 // last iteration -- gradient, BFGS update and new direction.
 template <int MODEL, int R, bool FAST>
 __device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                            const float (&wy)[R], float (&x)[Traits<MODEL>::P])
+                                            const float (&wy)[R], float (&x)[Traits<MODEL>::P], float *cold)
 {
     constexpr int P = Traits<MODEL>::P, IT = Traits<MODEL>::ITERS;
     constexpr int NEVAL = 2 * P + IT * 12 + (IT - 1) * 2 * P;
-    float D[P * P], g[P], gprev[P], s[P], d[P], t[P];
+    // cold state in LDS (element k at cold[k * 64]): D (P*P), previous gradient (P), step (P)
+    float *D = cold, *gprev_l = cold + P * P * 64, *s_l = cold + (P * P + P) * 64;
+    float g[P], d[P], t[P];
 #pragma unroll
     for (int r = 0; r < P; r++)
 #pragma unroll
-        for (int c = 0; c < P; c++) D[r * P + c] = (r == c) ? 1.0f : 0.0f;
+        for (int c = 0; c < P; c++) D[(r * P + c) * 64] = (r == c) ? 1.0f : 0.0f;
 #pragma unroll
-    for (int k = 0; k < P; k++) { t[k] = x[k]; g[k] = 0.0f; gprev[k] = 0.0f; s[k] = 0.0f; d[k] = 0.0f; }
+    for (int k = 0; k < P; k++) { t[k] = x[k]; g[k] = 0.0f; d[k] = 0.0f; }
     float L0 = 0.0f, L1 = 0.0f, lo = 0.0f, hi = 0.0f, ln = 0.0f;
     bool sel = false;
     int phase = PH_G0, gi = 0, bsn = 0;
@@ -171,7 +196,7 @@ __device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float
             for (int k = 0; k < P; k++) pr[k] = __builtin_fmaf(d[k], step, x[k]);
         }
 
-        const float L = loss_1e<MODEL, R, FAST>(q, lq, wx, wy, pr);
+        const float L = loss_eval<MODEL, R, FAST>(q, lq, wx, wy, pr);
 
         if (phase == PH_G0) {
 #pragma unroll
@@ -195,15 +220,15 @@ __device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float
 #pragma unroll
                     for (int k = 0; k < P; k++) d[k] = -g[k];
                 } else {
-                    float y[P];
+                    float y[P], s[P];
 #pragma unroll
-                    for (int k = 0; k < P; k++) y[k] = g[k] - gprev[k];
+                    for (int k = 0; k < P; k++) { y[k] = g[k] - gprev_l[k * 64]; s[k] = s_l[k * 64]; }
                     bfgs_update<P>(D, s, y);
 #pragma unroll
                     for (int r = 0; r < P; r++) {     // MatMulVector (bfgsMLE_core.h:184-204)
                         float acc = 0.0f;
 #pragma unroll
-                        for (int c = 0; c < P; c++) acc = __builtin_fmaf(-D[r * P + c], g[c], acc);
+                        for (int c = 0; c < P; c++) acc = __builtin_fmaf(-D[(r * P + c) * 64], g[c], acc);
                         d[r] = acc;
                     }
                 }
@@ -231,13 +256,14 @@ __device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float
                 const float scale = (lo + hi) * 0.5f;
 #pragma unroll
                 for (int k = 0; k < P; k++) {
-                    s[k] = d[k] * scale;
-                    x[k] = x[k] + s[k];
+                    const float sk = d[k] * scale;
+                    s_l[k * 64] = sk;
+                    x[k] = x[k] + sk;
                 }
 #pragma unroll
                 for (int k = 0; k < P; k++) {
                     x[k] = (x[k] < 0.01f) ? 0.01f : x[k];   // IninfConstrain (bfgsMLE_core.h:76-85)
-                    gprev[k] = g[k];
+                    gprev_l[k * 64] = g[k];
                     t[k] = x[k];
                 }
                 gi = 0;
@@ -265,7 +291,7 @@ __device__ __forceinline__ void load_roi(const uint16_t *__restrict__ rec, const
         const float adu = (float)((i & 1) ? (word >> 16) : (word & 0xffffu));
         const float e = (adu - fc.Offset) * fc.KAdc;     // bfgsMLE_GS2D_core.h:144
         q[i] = (e > 1.0f) ? e : 1.0f;
-        lq[i] = qlogf(q[i]);
+        lq[i] = qlogf_normal(q[i]);   // q >= 1
     }
     auto word_at = [&](int idx) -> int {
         const uint32_t word = v[idx >> 1];
@@ -380,7 +406,8 @@ __global__ void __launch_bounds__(64) fit_1e_kernel(const uint16_t *__restrict__
         wle_weights<R>(wsx, wsy, mlefit, wx, wy);
     }
 
-    bfgs_fit_1e<MODEL, R, FAST>(q, lq, wx, wy, x);
+    __shared__ float cold[(P * P + 2 * P) * 64];
+    bfgs_fit_1e<MODEL, R, FAST>(q, lq, wx, wy, x, cold + threadIdx.x);
 
     float row[QCS_REC_FLOATS];
     bool ok;
