@@ -188,7 +188,7 @@ int qcs_ctx_destroy(qcs_ctx *c)
 int qcs_crlb_dev(qcs_ctx *c, float *d_loc, int n, void *stream)
 {
     if (!c || (!d_loc && n > 0) || n < 0) return fail(QCS_ERR_ARG, "invalid argument");
-    return crlb_launch(c->p, d_loc, n, (hipStream_t)stream);
+    return crlb_launch(c->p, d_loc, n, nullptr, (hipStream_t)stream);
 }
 
 int qcs_fit_stats_get(qcs_ctx *c, qcs_fit_stats *out)

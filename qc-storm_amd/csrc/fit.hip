@@ -18,16 +18,60 @@
 
 namespace qcs {
 
-enum { MODEL_GS2D = 0, MODEL_AS3D = 1 };
+enum { MODEL_GS2D = 0, MODEL_AS3D = 1, MODEL_2E = 2, MODEL_3E = 3, MODEL_AS3D2E = 4 };
 
+// Parameter layout of each model: amplitude / x / y / sigma_x / sigma_y index of emitter k,
+// background index, iteration count, loss clamp, WLE weighting.
 template <int MODEL> struct Traits;
 template <> struct Traits<MODEL_GS2D> {   // bfgsMLE_GS2D_core.h:25-37
-    static constexpr int P = 5, ITERS = 8;
+    static constexpr int P = 5, ITERS = 8, K = 1, IB = 4;
+    static constexpr bool WEIGHTED = true;
     static constexpr float CLAMP = 0.0f;
+    __device__ static constexpr int ia(int) { return 0; }
+    __device__ static constexpr int ix(int) { return 1; }
+    __device__ static constexpr int iy(int) { return 2; }
+    __device__ static constexpr int isx(int) { return 3; }
+    __device__ static constexpr int isy(int) { return 3; }
 };
 template <> struct Traits<MODEL_AS3D> {   // bfgsMLE_AS3D_core.h:25-38, loss clamp :501-505
-    static constexpr int P = 6, ITERS = 11;
+    static constexpr int P = 6, ITERS = 11, K = 1, IB = 5;
+    static constexpr bool WEIGHTED = true;
     static constexpr float CLAMP = 0.01f;
+    __device__ static constexpr int ia(int) { return 0; }
+    __device__ static constexpr int ix(int) { return 1; }
+    __device__ static constexpr int iy(int) { return 2; }
+    __device__ static constexpr int isx(int) { return 3; }
+    __device__ static constexpr int isy(int) { return 4; }
+};
+template <> struct Traits<MODEL_2E> {     // bfgsMLE_GS2D_2Emitter_core.h:24-51
+    static constexpr int P = 8, ITERS = 13, K = 2, IB = 7;
+    static constexpr bool WEIGHTED = true;
+    static constexpr float CLAMP = 0.0f;
+    __device__ static constexpr int ia(int k) { return 3 * k; }
+    __device__ static constexpr int ix(int k) { return 3 * k + 1; }
+    __device__ static constexpr int iy(int k) { return 3 * k + 2; }
+    __device__ static constexpr int isx(int) { return 6; }
+    __device__ static constexpr int isy(int) { return 6; }
+};
+template <> struct Traits<MODEL_3E> {     // bfgsMLE_GS2D_3Emitter_core.h:24-54 (unweighted)
+    static constexpr int P = 11, ITERS = 16, K = 3, IB = 10;
+    static constexpr bool WEIGHTED = false;
+    static constexpr float CLAMP = 0.0f;
+    __device__ static constexpr int ia(int k) { return 3 * k; }
+    __device__ static constexpr int ix(int k) { return 3 * k + 1; }
+    __device__ static constexpr int iy(int k) { return 3 * k + 2; }
+    __device__ static constexpr int isx(int) { return 9; }
+    __device__ static constexpr int isy(int) { return 9; }
+};
+template <> struct Traits<MODEL_AS3D2E> { // bfgsMLE_AS3D_2Emitter_core.h:24-54 (unweighted), clamp :590-594
+    static constexpr int P = 11, ITERS = 16, K = 2, IB = 10;
+    static constexpr bool WEIGHTED = false;
+    static constexpr float CLAMP = 0.01f;
+    __device__ static constexpr int ia(int k) { return 5 * k; }
+    __device__ static constexpr int ix(int k) { return 5 * k + 1; }
+    __device__ static constexpr int iy(int k) { return 5 * k + 2; }
+    __device__ static constexpr int isx(int k) { return 5 * k + 3; }
+    __device__ static constexpr int isy(int k) { return 5 * k + 4; }
 };
 
 FitConst make_fit_const(const qcs_params &p)
@@ -41,32 +85,37 @@ FitConst make_fit_const(const qcs_params &p)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Weighted Poisson deviance of the one-emitter model (separable evaluation).  Clamps p in place
-// as the reference's loss does with its argument array.  SAFE = false uses the positive-normal
-// fast path of the logarithm and reports in `bad` whether any model value was zero, negative,
-// subnormal or not finite; the caller then re-evaluates with SAFE = true (full special-value
-// handling), which yields the same value the CPU oracle computes.
+// Weighted Poisson deviance, model I = sum_k A_k exp(-(sx_k dx^2 + sy_k dy^2)) + B evaluated
+// separably (one exp per emitter and ROI row / column).  Clamps p in place as the reference's
+// loss does with its argument array.  SAFE = false uses the positive-normal fast path of the
+// logarithm and reports in `bad` whether any model value was zero, negative, subnormal or not
+// finite; the caller then re-evaluates with SAFE = true (full special-value handling), which
+// yields the value the CPU oracle computes.
 template <int MODEL, int R, bool FAST, bool SAFE>
-__device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                         const float (&wy)[R], float (&p)[Traits<MODEL>::P], bool &bad)
+__device__ __forceinline__ float loss_model(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
+                                            const float (&wy)[R], float (&p)[Traits<MODEL>::P], bool &bad)
 {
-    constexpr int P = Traits<MODEL>::P;
+    using T = Traits<MODEL>;
+    constexpr int P = T::P, K = T::K;
 #pragma unroll
-    for (int i = 0; i < P; i++) p[i] = (p[i] < Traits<MODEL>::CLAMP) ? Traits<MODEL>::CLAMP : p[i];
-    const float a = p[0] * 128.0f;
-    const float sx = p[3] * 0.1f;
-    const float sy = (MODEL == MODEL_AS3D) ? p[4] * 0.1f : sx;
-    const float b = p[P - 1] * 64.0f;
-    float ex[R], ay[R];
+    for (int i = 0; i < P; i++) p[i] = (p[i] < T::CLAMP) ? T::CLAMP : p[i];
+    const float b = p[T::IB] * 64.0f;
+    float ex[K][R], ay[K][R];
 #pragma unroll
-    for (int c = 0; c < R; c++) {
-        const float dx = ((float)c + 0.5f) - p[1];
-        ex[c] = vexp_nonpos<FAST>((-(dx * dx)) * sx);
-    }
+    for (int k = 0; k < K; k++) {
+        const float a = p[T::ia(k)] * 128.0f;
+        const float sx = p[T::isx(k)] * 0.1f;
+        const float sy = p[T::isy(k)] * 0.1f;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const float dy = ((float)r + 0.5f) - p[2];
-        ay[r] = a * vexp_nonpos<FAST>((-(dy * dy)) * sy);
+        for (int c = 0; c < R; c++) {
+            const float dx = ((float)c + 0.5f) - p[T::ix(k)];
+            ex[k][c] = vexp_nonpos<FAST>((-(dx * dx)) * sx);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float dy = ((float)r + 0.5f) - p[T::iy(k)];
+            ay[k][r] = a * vexp_nonpos<FAST>((-(dy * dy)) * sy);
+        }
     }
     float total = 0.0f;
     bool nonnormal = false;
@@ -74,7 +123,15 @@ __device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&
     for (int r = 0; r < R; r++) {
 #pragma unroll
         for (int c = 0; c < R; c++) {
-            const float I = __builtin_fmaf(ay[r], ex[c], b);
+            float I;
+            if (K == 1) {
+                I = __builtin_fmaf(ay[0][r], ex[0][c], b);
+            } else {
+                I = ay[0][r] * ex[0][c];
+#pragma unroll
+                for (int k = 1; k < K; k++) I = __builtin_fmaf(ay[k][r], ex[k][c], I);
+                I = I + b;
+            }
             float LI;
             if (SAFE || FAST) {
                 LI = vlog<FAST>(I);
@@ -84,7 +141,8 @@ __device__ __forceinline__ float loss_1e(const float (&q)[R * R], const float (&
             }
             const float qq = q[r * R + c];
             const float t = __builtin_fmaf(-qq, LI - lq[r * R + c], I - qq);
-            total = __builtin_fmaf(wy[r] * wx[c], t, total);
+            if (T::WEIGHTED) total = __builtin_fmaf(wy[r] * wx[c], t, total);
+            else total = total + t;
         }
     }
     bad = nonnormal;
@@ -96,8 +154,8 @@ __device__ __forceinline__ float loss_eval(const float (&q)[R * R], const float 
                                           const float (&wy)[R], float (&p)[Traits<MODEL>::P])
 {
     bool bad;
-    float L = loss_1e<MODEL, R, FAST, false>(q, lq, wx, wy, p, bad);
-    if (!FAST && bad) L = loss_1e<MODEL, R, FAST, true>(q, lq, wx, wy, p, bad);
+    float L = loss_model<MODEL, R, FAST, false>(q, lq, wx, wy, p, bad);
+    if (!FAST && bad) L = loss_model<MODEL, R, FAST, true>(q, lq, wx, wy, p, bad);
     return L;
 }
 
@@ -161,7 +219,7 @@ enum { PH_G0 = 0, PH_G1 = 1, PH_L0 = 2, PH_L1 = 3, PH_BS = 4 };
 // evaluations (eleven halvings of [1e-5, 1]), step, clamp >= 0.01, and -- except after the
 // last iteration -- gradient, BFGS update and new direction.
 template <int MODEL, int R, bool FAST>
-__device__ __forceinline__ void bfgs_fit_1e(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
+__device__ __forceinline__ void bfgs_fit(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
                                             const float (&wy)[R], float (&x)[Traits<MODEL>::P], float *cold)
 {
     constexpr int P = Traits<MODEL>::P, IT = Traits<MODEL>::ITERS;
@@ -351,12 +409,13 @@ __device__ __forceinline__ float zpoly(const float (&pc)[5], float d, float corr
 // two-emitter stage (valid background and SNR > 6, bfgsMLE_GS2D_core.h:279-297).
 template <int MODEL, int R, bool FAST>
 __global__ void __launch_bounds__(64) fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
-                                                    const int *__restrict__ idx, int n, float *__restrict__ loc,
-                                                    uint8_t *__restrict__ flag, FitConst fc)
+                                                    const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
+                                                    float *__restrict__ loc, int *__restrict__ flag, FitConst fc)
 {
     constexpr int P = Traits<MODEL>::P;
     constexpr int HALF = R / 2;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     const int roi = idx ? idx[i] : i;
 
@@ -407,7 +466,7 @@ __global__ void __launch_bounds__(64) fit_1e_kernel(const uint16_t *__restrict__
     }
 
     __shared__ float cold[(P * P + 2 * P) * 64];
-    bfgs_fit_1e<MODEL, R, FAST>(q, lq, wx, wy, x, cold + threadIdx.x);
+    bfgs_fit<MODEL, R, FAST>(q, lq, wx, wy, x, cold + threadIdx.x);
 
     float row[QCS_REC_FLOATS];
     bool ok;
@@ -467,6 +526,8 @@ __global__ void __launch_bounds__(64) fit_1e_kernel(const uint16_t *__restrict__
     if (flag) flag[roi] = (!ok && valid && snr > 6.0f) ? 1 : 0;
 }
 
+#include "fit_multi.inc"
+
 // ---------------------------------------------------------------------------------------------
 // Diagonal-Fisher CRLB of x and y (GaussianCRLB_Calc_top, LocPrecCalc_GaussianCRLB.h:64-128):
 // numeric second derivatives of the model and its log with dh = 0.02 on a G x G grid.
@@ -479,9 +540,11 @@ __device__ __forceinline__ float crlb_psf(int r, int c, float peak, float ix, fl
 }
 
 template <int G>
-__global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n, float QE, float RN, float PS)
+__global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n, const int *__restrict__ n_dev, float QE,
+                                                  float RN, float PS)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     float *row = loc + (size_t)i * QCS_REC_FLOATS;
     float te = row[6] * QE, be = row[7] * QE;
@@ -529,20 +592,22 @@ __global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n
 // ---------------------------------------------------------------------------------------------
 template <int MODEL, bool FAST>
 static int launch_1e_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, const float *wle, const int *idx, int n,
-                       float *loc, uint8_t *flag, const FitConst &fc)
+                       const int *n_dev, float *loc, int *flag, const FitConst &fc)
 {
     switch (R) {
-    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
-    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
-    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, loc, flag, fc); break;
-    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the one-emitter fitter", R);
+    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 11: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 11, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 13: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 13, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the fitter (5..13)", R);
     }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
 
-int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n, float *d_loc,
-                  uint8_t *d_flag, hipStream_t s)
+int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n,
+                  const int *d_n, float *d_loc, int *d_flag, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
     const FitConst fc = make_fit_const(p);
@@ -550,21 +615,223 @@ int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wl
     const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
     const bool fast = (p.FitMath == QCS_MATH_FAST);
     if (as3d)
-        return fast ? launch_1e_r<MODEL_AS3D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc)
-                    : launch_1e_r<MODEL_AS3D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc);
-    return fast ? launch_1e_r<MODEL_GS2D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc)
-                : launch_1e_r<MODEL_GS2D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_loc, d_flag, fc);
+        return fast ? launch_1e_r<MODEL_AS3D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
+                    : launch_1e_r<MODEL_AS3D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
+    return fast ? launch_1e_r<MODEL_GS2D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
+                : launch_1e_r<MODEL_GS2D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
 }
 
-int crlb_launch(const qcs_params &p, float *d_loc, int n, hipStream_t s)
+template <int MODEL, bool FAST>
+static int launch_me_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, const float *wle, const int *list,
+                       const int *n_dev, float *loc, float *second, int *fs, int *ff, const FitConst &fc)
+{
+    switch (R) {
+    case 5: hipLaunchKernelGGL((fit_me_kernel<MODEL, 5, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
+    case 7: hipLaunchKernelGGL((fit_me_kernel<MODEL, 7, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
+    case 9: hipLaunchKernelGGL((fit_me_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
+    case 11: hipLaunchKernelGGL((fit_me_kernel<MODEL, 11, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
+    case 13: hipLaunchKernelGGL((fit_me_kernel<MODEL, 13, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
+    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the multi-emitter fitter (5..13)", R);
+    }
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+// multi-emitter stage over list[j], j < *n_dev (grid sized for n_max)
+int fit_launch_me(const qcs_params &p, int model, const uint16_t *d_rois, const float *d_wle, const int *d_list, int n_max,
+                  const int *d_n, float *d_loc, float *d_second, int *d_fs, int *d_ff, hipStream_t s)
+{
+    if (n_max <= 0) return QCS_OK;
+    const FitConst fc = make_fit_const(p);
+    const dim3 grid((n_max + 63) / 64);
+    const bool fast = (p.FitMath == QCS_MATH_FAST);
+    const int R = p.ROISize;
+    switch (model) {
+    case MODEL_2E:
+        return fast ? launch_me_r<MODEL_2E, true>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc)
+                    : launch_me_r<MODEL_2E, false>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc);
+    case MODEL_3E:
+        return fast ? launch_me_r<MODEL_3E, true>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc)
+                    : launch_me_r<MODEL_3E, false>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc);
+    default:
+        return fast ? launch_me_r<MODEL_AS3D2E, true>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc)
+                    : launch_me_r<MODEL_AS3D2E, false>(R, grid, s, d_rois, d_wle, d_list, d_n, d_loc, d_second, d_fs, d_ff, fc);
+    }
+}
+
+int crlb_launch(const qcs_params &p, float *d_loc, int n, const int *d_n, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
     const dim3 grid((n + 63) / 64);
     if (p.LocType == QCS_LOCTYPE_AS3D)
-        hipLaunchKernelGGL(crlb_kernel<15>, grid, dim3(64), 0, s, d_loc, n, p.QE, p.ReadNoise_e, p.PixelSize);
+        hipLaunchKernelGGL(crlb_kernel<15>, grid, dim3(64), 0, s, d_loc, n, d_n, p.QE, p.ReadNoise_e, p.PixelSize);
     else
-        hipLaunchKernelGGL(crlb_kernel<9>, grid, dim3(64), 0, s, d_loc, n, p.QE, p.ReadNoise_e, p.PixelSize);
+        hipLaunchKernelGGL(crlb_kernel<9>, grid, dim3(64), 0, s, d_loc, n, d_n, p.QE, p.ReadNoise_e, p.PixelSize);
     QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multi-emitter cascade (LDLocData_TypeDef::BFGS_MLELocalization, bfgs_top.cu:26-204, with
+// gpuMoleculeFitClasify, bfgs_top_MoleculeClasify.cu:26-92).  The reference builds its lists
+// with atomic counters (arbitrary order); here every list is a stable compaction (exclusive
+// scan of flags), so list order and the rows appended for second emitters are deterministic:
+//   1E list  = ROIs with WLE type <= WLEFit, in index order
+//   2E list  = ROIs with WLE type MultiFit, in index order, then rejected 1E fits with SNR > 6
+//   3E list  = rejected first emitters of the 2E stage, in 2E-list order (2D only)
+//   appended = accepted second emitters of the 2E stage, then of the 3E stage, in list order
+// All counts stay on the device; one read-back at the end.
+
+__global__ void cascade_classify_kernel(const float *__restrict__ wle, int n, int *__restrict__ f1, int *__restrict__ f2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int mt = (int)wle[(size_t)i * QCS_WLE_FLOATS + QCS_WLE_FLUOTYPE];
+    f1[i] = (mt <= QCS_MOLTYPE_WLEFIT) ? 1 : 0;
+    f2[i] = (mt <= QCS_MOLTYPE_WLEFIT) ? 0 : 1;
+}
+
+// cnt layout: [0] n1, [1] n2 classified, [2] n2 total, [3] n3, [4] seconds from 2E, [5] seconds
+// from 3E, [6] rejected, [7] rows, [8] loc_cap
+__global__ void cascade_count_kernel(const int *__restrict__ tile_sums, int ntiles, int *__restrict__ cnt, int slot, int mode)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int total = tile_sums[ntiles];
+    if (mode == 0) cnt[slot] = total;
+    else if (mode == 1) cnt[slot] += total;   // accumulate (2E list = classified + re-routed)
+}
+
+__global__ void cascade_rows_kernel(int n, int *__restrict__ cnt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int extra = cnt[4] + cnt[5];
+    const int room = cnt[8] - n;
+    extra = extra < room ? extra : room;
+    cnt[7] = n + extra;
+}
+
+// append second-emitter rows: row = n + base + offs[j] for flagged j (bounded by the capacity)
+__global__ void __launch_bounds__(256) append_kernel(const int *__restrict__ flags, const int *__restrict__ offs,
+                                                     const int *__restrict__ n_list, const float *__restrict__ second,
+                                                     const int *__restrict__ cnt, int base_slot, int n, int loc_cap,
+                                                     float *__restrict__ loc)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= *n_list || !flags[j]) return;
+    const int base = base_slot >= 0 ? cnt[base_slot] : 0;
+    const int row = n + base + offs[j];
+    if (row >= loc_cap) return;
+    const float4 *src = reinterpret_cast<const float4 *>(second + (size_t)j * QCS_REC_FLOATS);
+    float4 *dst = reinterpret_cast<float4 *>(loc + (size_t)row * QCS_REC_FLOATS);
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+}
+
+__global__ void cascade_set_kernel(int *p, int v) { *p = v; }
+
+// flags of entries j >= *n_list are cleared so a scan over n entries counts only the list
+__global__ void clear_tail_kernel(int *__restrict__ f, int n, const int *__restrict__ n_list)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && j >= *n_list) f[j] = 0;
+}
+
+int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc, int loc_cap, int *n_rows,
+                hipStream_t s)
+{
+    const qcs_params &p = c->p;
+    const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
+    const int ntiles = scan_tile_count((size_t)n);
+    int rc;
+    // scratch: lists (3 x n), flags (6 x n), offsets (n), tile sums, seconds (2 x n rows), counters
+    if ((rc = c->fit_idx1.ensure(sizeof(int) * (size_t)n)) || (rc = c->fit_idx2.ensure(sizeof(int) * (size_t)n)) ||
+        (rc = c->fit_idx3.ensure(sizeof(int) * (size_t)n)) ||
+        (rc = c->fit_flag.ensure(sizeof(int) * ((size_t)n * 7 + ntiles + 1))) ||
+        (rc = c->fit_rows.ensure(sizeof(float) * QCS_REC_FLOATS * 2 * (size_t)n)) || (rc = c->fit_cnt.ensure(64)) ||
+        (rc = c->pin_cnt.ensure(64)))
+        return rc;
+    int *list1 = c->fit_idx1.as<int>(), *list2 = c->fit_idx2.as<int>(), *list3 = c->fit_idx3.as<int>();
+    int *f1 = c->fit_flag.as<int>(), *f2 = f1 + n, *rr = f2 + n, *fs2 = rr + n, *ff2 = fs2 + n, *fs3 = ff2 + n;
+    int *offs = fs3 + n, *tsum = offs + n;
+    int *ff3 = f1;   // f1 is free again once the 1E list is built
+    float *sec2 = c->fit_rows.as<float>(), *sec3 = sec2 + (size_t)QCS_REC_FLOATS * n;
+    int *cnt = c->fit_cnt.as<int>();
+    const dim3 g((n + 255) / 256);
+
+    fill_async(cnt, 16, 0, s);
+    hipLaunchKernelGGL(cascade_set_kernel, dim3(1), dim3(1), 0, s, cnt + 8, loc_cap);
+    fill_async(rr, n, 0, s);
+    fill_async(fs2, n, 0, s);
+    fill_async(ff2, n, 0, s);
+    fill_async(fs3, n, 0, s);
+    hipLaunchKernelGGL(cascade_classify_kernel, g, dim3(256), 0, s, d_wle, n, f1, f2);
+    // 1E list and classified part of the 2E list
+    if ((rc = scan_exclusive(f1, n, offs, tsum, s)) || (rc = compact_scatter(f1, offs, n, nullptr, nullptr, 0, list1, s)))
+        return rc;
+    hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 0, 0);
+    if ((rc = scan_exclusive(f2, n, offs, tsum, s)) || (rc = compact_scatter(f2, offs, n, nullptr, nullptr, 0, list2, s)))
+        return rc;
+    hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 1, 0);
+    hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 0);
+    // one-emitter stage; rejected fits with SNR > 6 flag themselves in rr
+    if ((rc = fit_launch_1e(p, d_rois, d_wle, list1, n, cnt + 0, d_loc, rr, s))) return rc;
+    // re-routed ROIs join the 2E list after the classified ones, in index order
+    if ((rc = scan_exclusive(rr, n, offs, tsum, s)) || (rc = compact_scatter(rr, offs, n, nullptr, cnt + 1, 0, list2, s)))
+        return rc;
+    hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 1);
+    // two-emitter stage
+    if ((rc = fit_launch_me(p, as3d ? MODEL_AS3D2E : MODEL_2E, d_rois, d_wle, list2, n, cnt + 2, d_loc, sec2, fs2, ff2, s)))
+        return rc;
+    hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs2, n, cnt + 2);
+    hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff2, n, cnt + 2);
+    if ((rc = scan_exclusive(fs2, n, offs, tsum, s))) return rc;
+    hipLaunchKernelGGL(append_kernel, g, dim3(256), 0, s, fs2, offs, cnt + 2, sec2, cnt, -1, n, loc_cap, d_loc);
+    hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 4, 0);
+    if ((rc = scan_exclusive(ff2, n, offs, tsum, s))) return rc;
+    if (as3d) {
+        hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 6, 0);
+    } else {
+        if ((rc = compact_scatter(ff2, offs, n, list2, nullptr, 0, list3, s))) return rc;
+        hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 3, 0);
+        fill_async(ff3, n, 0, s);
+        if ((rc = fit_launch_me(p, MODEL_3E, d_rois, d_wle, list3, n, cnt + 3, d_loc, sec3, fs3, ff3, s))) return rc;
+        hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs3, n, cnt + 3);
+        hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff3, n, cnt + 3);
+        if ((rc = scan_exclusive(fs3, n, offs, tsum, s))) return rc;
+        hipLaunchKernelGGL(append_kernel, g, dim3(256), 0, s, fs3, offs, cnt + 3, sec3, cnt, 4, n, loc_cap, d_loc);
+        hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 5, 0);
+        if ((rc = scan_exclusive(ff3, n, offs, tsum, s))) return rc;
+        hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 6, 0);
+    }
+    hipLaunchKernelGGL(cascade_rows_kernel, dim3(1), dim3(1), 0, s, n, cnt);
+    QCS_HIP(hipGetLastError());
+    if ((rc = crlb_launch(p, d_loc, loc_cap, cnt + 7, s))) return rc;
+    int *h = c->pin_cnt.as<int>();
+    QCS_HIP(hipMemcpyAsync(h, cnt, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipStreamSynchronize(s));
+    qcs_fit_stats &st = c->stats;
+    st.n_in = n;
+    st.n_single = h[0];
+    st.n_2e = h[2];
+    st.n_3e = h[3];
+    st.n_rejected = h[6];
+    st.n_rows = h[7];
+    // FitRatio_* as bfgs_top.cu:64-173
+    const float nd = (float)(n < 1 ? 1 : n);
+    st.FitRatio_1E = (float)h[0] / nd;
+    st.FitRatio_2E = (float)h[2] / nd;
+    st.FitRatio_3E = (float)h[3] / nd;
+    st.FitRatio_Final_1E = (float)(h[0] - (h[2] - h[1])) / nd;
+    if (!as3d) {
+        st.FitRatio_Final_2E = (float)(h[2] - h[3]) / nd;
+        st.FitRatio_Final_3E = (float)(h[3] - h[6]) / nd;
+    } else {
+        st.FitRatio_Final_2E = (float)(h[2] - h[6]) / nd;
+        st.FitRatio_Final_3E = 0.0f;
+    }
+    st.FitRatio_Final_4E = (float)h[6] / nd;
+    if (n_rows) *n_rows = h[7];
     return QCS_OK;
 }
 

@@ -24,9 +24,9 @@ int qcs_fit_dev(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     if (!d_wle) return fail(QCS_ERR_ARG, "WLE records are required (qcs_wle_dev computes them)");
     hipStream_t s = (hipStream_t)stream;
     if (!c->p.MultiEmitterFitEn) {
-        int rc = fit_launch_1e(c->p, d_rois, d_wle, nullptr, n, d_loc, nullptr, s);
+        int rc = fit_launch_1e(c->p, d_rois, d_wle, nullptr, n, nullptr, d_loc, nullptr, s);
         if (rc) return rc;
-        rc = crlb_launch(c->p, d_loc, n, s);
+        rc = crlb_launch(c->p, d_loc, n, nullptr, s);
         if (rc) return rc;
         c->stats.n_single = n;
         c->stats.n_rows = n;
@@ -63,10 +63,3 @@ int qcs_fit_rois(qcs_ctx *c, const uint16_t *rois, const float *wle, int n, floa
 
 }  // extern "C"
 
-namespace qcs {
-// Multi-emitter cascade (bfgs_top.cu:92-193): implemented in fit_multi.hip.
-__attribute__((weak)) int fit_cascade(qcs_ctx *, const uint16_t *, const float *, int, float *, int, int *, hipStream_t)
-{
-    return fail(QCS_ERR_ARG, "multi-emitter fitting is not available in this build");
-}
-}  // namespace qcs

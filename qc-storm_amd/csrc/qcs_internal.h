@@ -47,10 +47,21 @@ struct HostBuf {
 
 int validate_params(const qcs_params &p);
 
-// fit stage internals (fit.hip)
+// fit stage internals (fit.hip).  n is the grid bound; when d_n is set the kernels process
+// *d_n items (a count that lives on the device).
 int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n,
-                  float *d_loc, uint8_t *d_flag, hipStream_t s);
-int crlb_launch(const qcs_params &p, float *d_loc, int n, hipStream_t s);
+                  const int *d_n, float *d_loc, int *d_flag, hipStream_t s);
+int fit_launch_me(const qcs_params &p, int model, const uint16_t *d_rois, const float *d_wle, const int *d_list,
+                  int n_max, const int *d_n, float *d_loc, float *d_second, int *d_fs, int *d_ff, hipStream_t s);
+int crlb_launch(const qcs_params &p, float *d_loc, int n, const int *d_n, hipStream_t s);
+
+// device utilities (extract.hip): exclusive scan of n ints (total at tile_sums[ntiles]),
+// constant fill, stable compaction out[base + k] = src ? src[j] : j for flagged j.
+int scan_tile_count(size_t n);
+int scan_exclusive(const int *in, int n, int *out, int *tile_sums, hipStream_t s);
+void fill_async(int *p, size_t n, int v, hipStream_t s);
+int compact_scatter(const int *flags, const int *offs, int n, const int *src, const int *base_dev, int base_const,
+                    int *out, hipStream_t s);
 
 }  // namespace qcs
 

@@ -133,9 +133,11 @@ __device__ __forceinline__ void two_sided_sigma(const float (&mean)[R], float &s
 }
 
 template <int R>
-__global__ void __launch_bounds__(64) wle_psf_kernel(const uint16_t *__restrict__ rois, int n, float *__restrict__ wle)
+__global__ void __launch_bounds__(64) wle_psf_kernel(const uint16_t *__restrict__ rois, int n, const int *n_dev,
+                                                     float *__restrict__ wle)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     const uint16_t *roi = rois + (size_t)i * (R * (R + 1));
     float mx[R], my[R];
@@ -189,10 +191,11 @@ __device__ __forceinline__ float nn_term(const RoiKey &a, const RoiKey &b, bool 
 }
 
 // all-pairs nearest neighbour (gpu_CalculateNearestNeighborDistance)
-__global__ void __launch_bounds__(256) wle_nn_allpairs_kernel(const uint16_t *__restrict__ rois, int n, int R,
-                                                              float *__restrict__ wle)
+__global__ void __launch_bounds__(256) wle_nn_allpairs_kernel(const uint16_t *__restrict__ rois, int n, const int *n_dev,
+                                                              int R, float *__restrict__ wle)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     const int W = R * (R + 1), P = R * R;
     const RoiKey a = roi_key(rois, W, P, i);
@@ -204,10 +207,11 @@ __global__ void __launch_bounds__(256) wle_nn_allpairs_kernel(const uint16_t *__
 // (frame, y)-sorted list: sweep outward through the frame's segment.  Terms from other frames and
 // from the ROI itself are >= 1e8 and never beat the scan's initial value 1e8, so only same-frame
 // neighbours matter; the sweep stops once the row distance alone exceeds the best distance.
-__global__ void __launch_bounds__(256) wle_nn_sorted_kernel(const uint16_t *__restrict__ rois, int n, int R,
-                                                            float *__restrict__ wle)
+__global__ void __launch_bounds__(256) wle_nn_sorted_kernel(const uint16_t *__restrict__ rois, int n, const int *n_dev,
+                                                            int R, float *__restrict__ wle)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     const int W = R * (R + 1), P = R * R;
     const RoiKey a = roi_key(rois, W, P, i);
@@ -227,6 +231,8 @@ __global__ void __launch_bounds__(256) wle_nn_sorted_kernel(const uint16_t *__re
     wle[(size_t)i * QCS_WLE_FLOATS + QCS_WLE_NEARDIST] = sqrtf(best);
 }
 
+__global__ void fill_flag_kernel(int *flag) { *flag = 0; }
+
 // 1 in *flag when the list is not sorted by (frame, y)
 __global__ void wle_sorted_check_kernel(const uint16_t *__restrict__ rois, int n, int R, int *flag)
 {
@@ -237,10 +243,11 @@ __global__ void wle_sorted_check_kernel(const uint16_t *__restrict__ rois, int n
     if (b.f < a.f || (b.f == a.f && b.y < a.y)) atomicOr(flag, 1);
 }
 
-__global__ void __launch_bounds__(256) wle_classify_kernel(const uint16_t *__restrict__ rois, int n, int R, int loc_type,
-                                                           float *__restrict__ wle)
+__global__ void __launch_bounds__(256) wle_classify_kernel(const uint16_t *__restrict__ rois, int n, const int *n_dev, int R,
+                                                           int loc_type, float *__restrict__ wle)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = *n_dev;
     if (i >= n) return;
     const int W = R * (R + 1), P = R * R;
     const float th = (loc_type == QCS_LOCTYPE_GS2D) ? ((float)R / 2.1f) / 2.35f : ((float)R / 1.9f) / 2.35f;
@@ -258,40 +265,52 @@ __global__ void __launch_bounds__(256) wle_classify_kernel(const uint16_t *__res
     o[QCS_WLE_FLUOTYPE] = (float)mt;
 }
 
-int wle_launch(const qcs_params &p, const uint16_t *d_rois, int n, float *d_wle, int sorted, int *d_flag, hipStream_t s)
+// n ROIs (host count) or, when n_dev is set, *n_dev ROIs with grids sized for n.
+// sorted: 1 = list sorted by (frame, y); 0 = unknown order (all-pairs scan).
+static int wle_launch_impl(const qcs_params &p, const uint16_t *d_rois, int n, const int *n_dev, float *d_wle, int sorted,
+                           hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
     const int R = p.ROISize;
     const dim3 g64((n + 63) / 64), g256((n + 255) / 256);
     switch (R) {
-    case 5: hipLaunchKernelGGL(wle_psf_kernel<5>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 7: hipLaunchKernelGGL(wle_psf_kernel<7>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 9: hipLaunchKernelGGL(wle_psf_kernel<9>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 11: hipLaunchKernelGGL(wle_psf_kernel<11>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 13: hipLaunchKernelGGL(wle_psf_kernel<13>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 15: hipLaunchKernelGGL(wle_psf_kernel<15>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
-    case 17: hipLaunchKernelGGL(wle_psf_kernel<17>, g64, dim3(64), 0, s, d_rois, n, d_wle); break;
+    case 5: hipLaunchKernelGGL(wle_psf_kernel<5>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 7: hipLaunchKernelGGL(wle_psf_kernel<7>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 9: hipLaunchKernelGGL(wle_psf_kernel<9>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 11: hipLaunchKernelGGL(wle_psf_kernel<11>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 13: hipLaunchKernelGGL(wle_psf_kernel<13>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 15: hipLaunchKernelGGL(wle_psf_kernel<15>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
+    case 17: hipLaunchKernelGGL(wle_psf_kernel<17>, g64, dim3(64), 0, s, d_rois, n, n_dev, d_wle); break;
     default: return fail(QCS_ERR_ARG, "ROISize %d not supported", R);
     }
     QCS_HIP(hipGetLastError());
-    if (sorted < 0) {
-        // unknown order: check on the device
-        QCS_HIP(hipMemsetAsync(d_flag, 0, sizeof(int), s));
-        hipLaunchKernelGGL(wle_sorted_check_kernel, g256, dim3(256), 0, s, d_rois, n, R, d_flag);
-        QCS_HIP(hipGetLastError());
-        int h = 0;
-        QCS_HIP(hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        QCS_HIP(hipStreamSynchronize(s));
-        sorted = (h == 0);
-    }
     if (sorted)
-        hipLaunchKernelGGL(wle_nn_sorted_kernel, g256, dim3(256), 0, s, d_rois, n, R, d_wle);
+        hipLaunchKernelGGL(wle_nn_sorted_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, d_wle);
     else
-        hipLaunchKernelGGL(wle_nn_allpairs_kernel, g256, dim3(256), 0, s, d_rois, n, R, d_wle);
+        hipLaunchKernelGGL(wle_nn_allpairs_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, d_wle);
     QCS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(wle_classify_kernel, g256, dim3(256), 0, s, d_rois, n, R, p.LocType, d_wle);
+    hipLaunchKernelGGL(wle_classify_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, p.LocType, d_wle);
     QCS_HIP(hipGetLastError());
     return QCS_OK;
+}
+
+// extraction output: raster order, count on the device
+int wle_launch_devn(const qcs_params &p, const uint16_t *d_rois, const int *d_n, int cap, float *d_wle, hipStream_t s)
+{
+    return wle_launch_impl(p, d_rois, cap, d_n, d_wle, 1, s);
+}
+
+// arbitrary ROI array: check the order on the device first
+int wle_launch(const qcs_params &p, const uint16_t *d_rois, int n, float *d_wle, int *d_flag, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    hipLaunchKernelGGL(fill_flag_kernel, dim3(1), dim3(1), 0, s, d_flag);
+    hipLaunchKernelGGL(wle_sorted_check_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_rois, n, p.ROISize, d_flag);
+    QCS_HIP(hipGetLastError());
+    int h = 0;
+    QCS_HIP(hipMemcpyAsync(&h, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipStreamSynchronize(s));
+    return wle_launch_impl(p, d_rois, n, nullptr, d_wle, h == 0, s);
 }
 
 }  // namespace qcs
@@ -303,5 +322,5 @@ extern "C" int qcs_wle_dev(qcs_ctx *c, const uint16_t *d_rois, int n, float *d_w
     if (n == 0) return QCS_OK;
     int rc = c->fit_cnt.ensure(64);
     if (rc) return rc;
-    return wle_launch(c->p, d_rois, n, d_wle, -1, c->fit_cnt.as<int>(), (hipStream_t)stream);
+    return wle_launch(c->p, d_rois, n, d_wle, c->fit_cnt.as<int>(), (hipStream_t)stream);
 }
