@@ -193,6 +193,17 @@ typedef struct qcs_fit_stats {
 } qcs_fit_stats;
 int qcs_fit_stats_get(qcs_ctx *ctx, qcs_fit_stats *out);
 
+/* Stage timing with HIP events recorded on the launch stream (the reference prints clock()
+ * timers per stage, IJ/OnlineLocalizationLD.cpp:97-106,345-354).  Enabling resets the totals;
+ * qcs_timing_get waits for the recorded events.  fit_1e covers the one-emitter kernel launches
+ * (fit_1e_launches of them, fit_1e_rois ROIs), fit_me the multi-emitter ones. */
+typedef struct qcs_timing {
+    double extract_ms, fit_1e_ms, fit_me_ms, crlb_ms, compact_ms, render_ms;
+    long long fit_1e_launches, fit_1e_rois, fit_me_launches;
+} qcs_timing;
+int qcs_timing_enable(qcs_ctx *ctx, int on);
+int qcs_timing_get(qcs_ctx *ctx, qcs_timing *out);
+
 /* CRLB only (LocPrecCalc_GaussianCRLB.cu:23-40): fills CRBX/CRBY of rows [0, n). */
 int qcs_crlb_dev(qcs_ctx *ctx, float *d_loc, int n, void *stream);
 

@@ -82,6 +82,51 @@ int validate_params(const qcs_params &p)
     return QCS_OK;
 }
 
+int timer_begin(Timer &t, hipStream_t s, hipEvent_t *ev)
+{
+    *ev = nullptr;
+    if (!t.on) return QCS_OK;
+    QCS_HIP(hipEventCreate(ev));
+    QCS_HIP(hipEventRecord(*ev, s));
+    return QCS_OK;
+}
+
+int timer_end(Timer &t, int cat, hipStream_t s, hipEvent_t ev, long long items)
+{
+    if (!t.on || !ev) return QCS_OK;
+    hipEvent_t b;
+    QCS_HIP(hipEventCreate(&b));
+    QCS_HIP(hipEventRecord(b, s));
+    if (t.npend == t.cap) {
+        int nc = t.cap ? 2 * t.cap : 64;
+        Timer::Pending *np = new (std::nothrow) Timer::Pending[nc];
+        if (!np) return fail(QCS_ERR_NOMEM, "timer allocation failed");
+        for (int i = 0; i < t.npend; i++) np[i] = t.pend[i];
+        delete[] t.pend;
+        t.pend = np;
+        t.cap = nc;
+    }
+    t.pend[t.npend++] = {cat, ev, b, items};
+    return QCS_OK;
+}
+
+static int timer_resolve(Timer &t)
+{
+    for (int i = 0; i < t.npend; i++) {
+        Timer::Pending &p = t.pend[i];
+        QCS_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        QCS_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        t.ms[p.cat] += ms;
+        t.launches[p.cat] += 1;
+        t.items[p.cat] += p.items;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    t.npend = 0;
+    return QCS_OK;
+}
+
 }  // namespace qcs
 
 using namespace qcs;
@@ -181,6 +226,8 @@ int qcs_ctx_destroy(qcs_ctx *c)
                            &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev};
     for (auto *b : bufs) b->release();
     c->pin_cnt.release();
+    (void)timer_resolve(c->timer);
+    delete[] c->timer.pend;
     delete c;
     return QCS_OK;
 }
@@ -189,6 +236,34 @@ int qcs_crlb_dev(qcs_ctx *c, float *d_loc, int n, void *stream)
 {
     if (!c || (!d_loc && n > 0) || n < 0) return fail(QCS_ERR_ARG, "invalid argument");
     return crlb_launch(c->p, d_loc, n, nullptr, (hipStream_t)stream);
+}
+
+int qcs_timing_enable(qcs_ctx *c, int on)
+{
+    if (!c) return fail(QCS_ERR_ARG, "null context");
+    int rc = timer_resolve(c->timer);
+    if (rc) return rc;
+    for (int i = 0; i < TM_COUNT; i++) { c->timer.ms[i] = 0; c->timer.launches[i] = 0; c->timer.items[i] = 0; }
+    c->timer.on = on ? 1 : 0;
+    return QCS_OK;
+}
+
+int qcs_timing_get(qcs_ctx *c, qcs_timing *out)
+{
+    if (!c || !out) return fail(QCS_ERR_ARG, "null argument");
+    int rc = timer_resolve(c->timer);
+    if (rc) return rc;
+    const Timer &t = c->timer;
+    out->extract_ms = t.ms[TM_EXTRACT];
+    out->fit_1e_ms = t.ms[TM_FIT1E];
+    out->fit_me_ms = t.ms[TM_FITME];
+    out->crlb_ms = t.ms[TM_CRLB];
+    out->compact_ms = t.ms[TM_COMPACT];
+    out->render_ms = t.ms[TM_RENDER];
+    out->fit_1e_launches = t.launches[TM_FIT1E];
+    out->fit_1e_rois = t.items[TM_FIT1E];
+    out->fit_me_launches = t.launches[TM_FITME];
+    return QCS_OK;
 }
 
 int qcs_fit_stats_get(qcs_ctx *c, qcs_fit_stats *out)

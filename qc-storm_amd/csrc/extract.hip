@@ -581,6 +581,8 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
     int *info = ctx->ex_tmp.as<int>();
     int3 *xyb = ctx->ex_xy.as<int3>();
 
+    hipEvent_t ev;
+    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
     fill_async(reinterpret_cast<int *>(var_sums), 4 * (size_t)c.nbatch, 0, s);
     fill_async(row_count, nrows, 0, s);
     hipLaunchKernelGGL(var_kernel, dim3(64, c.nbatch), dim3(256), 0, s, d_frames, c, var_sums);
@@ -602,6 +604,7 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
         rc = wle_launch_devn(p, d_rois, info, cap, d_wle, s);
         if (rc) return rc;
     }
+    if ((rc = timer_end(ctx->timer, TM_EXTRACT, s, ev, nframes))) return rc;
     int *h = ctx->pin_cnt.as<int>();
     QCS_HIP(hipMemcpyAsync(h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     QCS_HIP(hipStreamSynchronize(s));

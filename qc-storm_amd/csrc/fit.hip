@@ -775,14 +775,19 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 1, 0);
     hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 0);
     // one-emitter stage; rejected fits with SNR > 6 flag themselves in rr
+    hipEvent_t ev;
+    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
     if ((rc = fit_launch_1e(p, d_rois, d_wle, list1, n, cnt + 0, d_loc, rr, s))) return rc;
+    if ((rc = timer_end(c->timer, TM_FIT1E, s, ev, n))) return rc;
     // re-routed ROIs join the 2E list after the classified ones, in index order
     if ((rc = scan_exclusive(rr, n, offs, tsum, s)) || (rc = compact_scatter(rr, offs, n, nullptr, cnt + 1, 0, list2, s)))
         return rc;
     hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 1);
     // two-emitter stage
+    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
     if ((rc = fit_launch_me(p, as3d ? MODEL_AS3D2E : MODEL_2E, d_rois, d_wle, list2, n, cnt + 2, d_loc, sec2, fs2, ff2, s)))
         return rc;
+    if ((rc = timer_end(c->timer, TM_FITME, s, ev, n))) return rc;
     hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs2, n, cnt + 2);
     hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff2, n, cnt + 2);
     if ((rc = scan_exclusive(fs2, n, offs, tsum, s))) return rc;
@@ -795,7 +800,9 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
         if ((rc = compact_scatter(ff2, offs, n, list2, nullptr, 0, list3, s))) return rc;
         hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 3, 0);
         fill_async(ff3, n, 0, s);
+        if ((rc = timer_begin(c->timer, s, &ev))) return rc;
         if ((rc = fit_launch_me(p, MODEL_3E, d_rois, d_wle, list3, n, cnt + 3, d_loc, sec3, fs3, ff3, s))) return rc;
+        if ((rc = timer_end(c->timer, TM_FITME, s, ev, n))) return rc;
         hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs3, n, cnt + 3);
         hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff3, n, cnt + 3);
         if ((rc = scan_exclusive(fs3, n, offs, tsum, s))) return rc;
@@ -806,7 +813,9 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     }
     hipLaunchKernelGGL(cascade_rows_kernel, dim3(1), dim3(1), 0, s, n, cnt);
     QCS_HIP(hipGetLastError());
+    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
     if ((rc = crlb_launch(p, d_loc, loc_cap, cnt + 7, s))) return rc;
+    if ((rc = timer_end(c->timer, TM_CRLB, s, ev, n))) return rc;
     int *h = c->pin_cnt.as<int>();
     QCS_HIP(hipMemcpyAsync(h, cnt, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
     QCS_HIP(hipStreamSynchronize(s));

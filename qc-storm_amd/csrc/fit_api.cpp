@@ -24,10 +24,15 @@ int qcs_fit_dev(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     if (!d_wle) return fail(QCS_ERR_ARG, "WLE records are required (qcs_wle_dev computes them)");
     hipStream_t s = (hipStream_t)stream;
     if (!c->p.MultiEmitterFitEn) {
-        int rc = fit_launch_1e(c->p, d_rois, d_wle, nullptr, n, nullptr, d_loc, nullptr, s);
+        hipEvent_t ev;
+        int rc = timer_begin(c->timer, s, &ev);
         if (rc) return rc;
+        rc = fit_launch_1e(c->p, d_rois, d_wle, nullptr, n, nullptr, d_loc, nullptr, s);
+        if (rc) return rc;
+        if ((rc = timer_end(c->timer, TM_FIT1E, s, ev, n)) || (rc = timer_begin(c->timer, s, &ev))) return rc;
         rc = crlb_launch(c->p, d_loc, n, nullptr, s);
         if (rc) return rc;
+        if ((rc = timer_end(c->timer, TM_CRLB, s, ev, n))) return rc;
         c->stats.n_single = n;
         c->stats.n_rows = n;
         c->stats.FitRatio_1E = 1.0f;

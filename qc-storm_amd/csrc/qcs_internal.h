@@ -47,6 +47,24 @@ struct HostBuf {
 
 int validate_params(const qcs_params &p);
 
+// stage timing (api.cpp): a recorded interval per launch group, resolved on demand
+enum { TM_EXTRACT = 0, TM_FIT1E, TM_FITME, TM_CRLB, TM_COMPACT, TM_RENDER, TM_COUNT };
+struct Timer {
+    int on = 0;
+    struct Pending {
+        int cat;
+        hipEvent_t a, b;
+        long long items;
+    };
+    Pending *pend = nullptr;
+    int npend = 0, cap = 0;
+    double ms[TM_COUNT] = {0};
+    long long launches[TM_COUNT] = {0}, items[TM_COUNT] = {0};
+};
+// begin/end an interval on stream s (no-ops when timing is off); end returns the status
+int timer_begin(Timer &t, hipStream_t s, hipEvent_t *ev);
+int timer_end(Timer &t, int cat, hipStream_t s, hipEvent_t ev, long long items);
+
 // fit stage internals (fit.hip).  n is the grid bound; when d_n is set the kernels process
 // *d_n items (a count that lives on the device).
 int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wle, const int *d_idx, int n,
@@ -68,6 +86,7 @@ int compact_scatter(const int *flags, const int *offs, int n, const int *src, co
 struct qcs_ctx {
     qcs_params p;
     int device = 0;
+    qcs::Timer timer;
     qcs_fit_stats stats{};
     // extraction scratch
     qcs::DevBuf ex_bkg, ex_tmp, ex_det, ex_mark, ex_cnt, ex_xy, ex_var, ex_frames;

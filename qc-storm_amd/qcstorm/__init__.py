@@ -66,6 +66,14 @@ class QcsFitStats(C.Structure):
     ]
 
 
+class QcsTiming(C.Structure):
+    _fields_ = [
+        ("extract_ms", C.c_double), ("fit_1e_ms", C.c_double), ("fit_me_ms", C.c_double), ("crlb_ms", C.c_double),
+        ("compact_ms", C.c_double), ("render_ms", C.c_double),
+        ("fit_1e_launches", C.c_longlong), ("fit_1e_rois", C.c_longlong), ("fit_me_launches", C.c_longlong),
+    ]
+
+
 _P = C.POINTER
 _vp, _i, _f, _u64, _ll = C.c_void_p, C.c_int, C.c_float, C.c_uint64, C.c_longlong
 
@@ -84,6 +92,8 @@ PROTOTYPES = {
     "qcs_fit_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i), _vp]),
     "qcs_fit_stats_get": (_i, [_vp, _P(QcsFitStats)]),
     "qcs_crlb_dev": (_i, [_vp, _vp, _i, _vp]),
+    "qcs_timing_enable": (_i, [_vp, _i]),
+    "qcs_timing_get": (_i, [_vp, _P(QcsTiming)]),
     "qcs_compact_sorted_dev": (_i, [_vp, _vp, _i, _i, _i, _vp, _P(_i), _vp]),
     "qcs_compact_dev": (_i, [_vp, _vp, _i, _vp, _P(_i), _vp]),
     "qcs_render_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
@@ -240,6 +250,14 @@ class Context:
         if rc not in (QCS_OK, QCS_ERR_CAPACITY):
             check(rc, "qcs_extract")
         return rois[:n_out.value].copy(), wle[:n_out.value].copy(), n_found.value
+
+    def timing_enable(self, on=True):
+        check(lib().qcs_timing_enable(self.h, int(on)), "qcs_timing_enable")
+
+    def timing(self):
+        t = QcsTiming()
+        check(lib().qcs_timing_get(self.h, C.byref(t)), "qcs_timing_get")
+        return {k: getattr(t, k) for k, _ in QcsTiming._fields_}
 
     def stats(self):
         st = QcsFitStats()

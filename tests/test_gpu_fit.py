@@ -45,3 +45,45 @@ def test_gs2d_fast_math_statistics(gpu):
     # median deviation well inside the FP32 noise band of the schedule (FP32 vs FP64 median ~2e-4 px)
     assert np.mean((ref[:, 0] > 0) == (got[:, 0] > 0)) > 0.99
     assert np.median(d) < 2e-3
+
+
+def _multi_rois(R, n, seed, astig=False):
+    sp = Q.default_sim(width=R, height=R, n_per_unit_max=3, min_sep=1.0, max_sep=3.0, astig=int(astig),
+                       photons_min=800, photons_max=6000)
+    rois, truth = Q.sim_rois_host(sp, seed, n)
+    rng = np.random.default_rng(seed)
+    rois[:, R * R + 4] = (rng.random(n) < 0.3).astype(np.uint16)    # detector "multi" flag on some ROIs
+    return rois, truth
+
+
+@pytest.mark.parametrize("R,lt", [(7, 0), (9, 0), (11, 0), (7, 1), (9, 1)])
+def test_multi_emitter_cascade_bit_exact(gpu, R, lt):
+    rois, _ = _multi_rois(R, 3000, 500 + R + 10 * lt, astig=(lt == 1))
+    params = Q.default_params(QE=1.0, ROISize=R, LocType=lt, MultiEmitterFitEn=1)
+    w = O.wle(rois, R, lt)
+    ref, st = O.fit(params, rois, w)
+    ctx = Q.Context(params)
+    got = ctx.fit_rois(rois, w)
+    gst = ctx.stats()
+    print("R", R, "lt", lt, "oracle", st, "gpu", {k: gst[k] for k in ("n_single", "n_2e", "n_3e", "n_rejected", "n_rows")})
+    assert (gst["n_single"], gst["n_2e"], gst["n_3e"], gst["n_rejected"], gst["n_rows"]) == \
+        (st["n_single"], st["n_2e"], st["n_3e"], st["n_rejected"], st["n_rows"])
+    assert got.shape == ref.shape
+    bad = np.argwhere(got.view(np.uint32) != ref.view(np.uint32))
+    if len(bad):
+        r = bad[0][0]
+        print("first mismatch row", r, "\n gpu", got[r], "\n orc", ref[r])
+    assert len(bad) == 0
+
+
+@pytest.mark.parametrize("R", [7, 9])
+def test_as3d_single_bit_exact(gpu, R):
+    sp = Q.default_sim(width=R, height=R, astig=1, photons_min=1000, photons_max=8000, bg_min=20, bg_max=80)
+    rois, _ = Q.sim_rois_host(sp, 900 + R, 4096)
+    params = Q.default_params(QE=1.0, ROISize=R, LocType=1, p1_XGY=-111.11, p1_XLY=-111.11)
+    w = O.wle(rois, R, 1)
+    ref, _ = O.fit(params, rois, w)
+    got = Q.Context(params).fit_rois(rois, w)
+    bad = np.argwhere(got.view(np.uint32) != ref.view(np.uint32))
+    print("R", R, "accepted", int((ref[:, 0] > 0).sum()), "mismatch", len(bad))
+    assert len(bad) == 0
