@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X localization path on the full-frame 2D workload.
+
+Workload: 200 uint16 frames of 2048 x 2048 px (100 nm px), emitters uniform at 0.3 /um^2
+(0.003 per px, ~12.6k per frame), PSF sigma U(1.0, 1.6) px, photons log-U(500, 5000), background
+U(10, 100) photons/px, Poisson noise, ADU = e/0.45 + 100, QE = 1 (simulator seed 1001).  The frames
+are simulated on the GPU and stay resident in HBM.  One step is one pass of the hot path over the
+200 frames, all on one HIP stream:
+
+    ROI extraction (qcs_extract_dev, batches of 1 frame, 7x7 ROIs + WLE pre-estimation)
+    -> single-emitter 2D BFGS MLE fit + CRLB (qcs_fit_dev)
+    -> frame-sorted compaction (qcs_compact_sorted_dev)
+    -> 10 nm-pixel rendering into a 20480 x 20480 float image (qcs_render_dev, image cleared first)
+
+`value` = fitted ROIs per second over whole steps (every extracted ROI is fitted).  The roofline
+entry is for the dominant kernel (the 7x7 one-emitter BFGS fit): algorithmic FLOPs per launch
+(120,736 FLOP per fit = 176 loss evaluations x 49 pixels x 14 FLOP, FMA = 2) divided by the
+average launch duration, measured with HIP events recorded on the launch stream by the library.
+The CPU baseline is the oracle's OpenMP fit + CRLB (same schedule, bit-identical output) on a
+bounded sample of the same ROIs, on the host cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--math exact|fast] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "qc-storm_amd"))
+
+import numpy as np  # noqa: E402
+
+import qcstorm as Q  # noqa: E402
+
+FLOP_PER_FIT_7X7 = 176 * 49 * 14          # loss evaluations x pixels x FLOP per pixel-evaluation
+PEAK_FP32_TFLOPS = 157.3                  # MI355X FP32 vector peak (dense)
+METRIC = "ROI fits/s (7×7 2D & multi-emitter) + warm-start speedup at equal RMSE"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fit_gs2d_r7_fullframe_traffic.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_per_fit():
+    """HBM bytes per fitted ROI of the fit kernel, from the committed PMC summary (or None)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("bytes_per_fit:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(params, rois, wle, seconds):
+    """Oracle fit + CRLB (OpenMP over ROIs) on a bounded sample; returns (dict, sample rows)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O  # the oracle is used here only as the CPU baseline and checker
+    threads = max(1, min(16, os.cpu_count() or 1))
+    pilot = min(rois.shape[0], 4096 * threads)
+    t0 = time.perf_counter()
+    O.fit_omp(params, rois[:pilot], wle[:pilot], threads)
+    rate = pilot / max(time.perf_counter() - t0, 1e-6)
+    n = int(min(rois.shape[0], max(pilot, rate * seconds)))
+    t0 = time.perf_counter()
+    loc = O.fit_omp(params, rois[:n], wle[:n], threads)
+    dt = time.perf_counter() - t0
+    return ({"value": n / dt, "unit": "fits/s", "cores": threads, "kind": "port",
+             "sample": "fit+CRLB of the first %d extracted 7x7 ROIs of the workload (%.1f s, %d OpenMP threads)"
+                       % (n, dt, threads)}, loc)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--math", choices=("exact", "fast"), default="exact")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the localization path runs on a single MI355X "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU (torch.cuda.is_available() is False)")
+    Q.lib()  # raises when libqcstorm.so is missing: there is no fallback
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream(device=dev)
+    sh = C.c_void_p(stream.cuda_stream)
+
+    W = H = 2048
+    F = args.frames
+    R = 7
+    params = Q.default_params(ImageWidth=W, ImageHigh=H, ROISize=R, LocType=Q.LOCTYPE_GS2D, MultiEmitterFitEn=0,
+                              WLEEn=1, PixelSize=100.0, PixelZoom=10.0, QE=1.0, TotalFrameNum=F, BatchedImageNum=1,
+                              FitMath=Q.MATH_FAST if args.math == "fast" else Q.MATH_EXACT)
+    Q.lib().qcs_params_update_sr_size(C.byref(params))
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.003, qe=1.0)
+
+    log("bench: simulating %d frames %dx%d on the GPU" % (F, W, H))
+    frames = torch.empty((F, H, W), dtype=torch.int16, device=dev)
+    with torch.cuda.stream(stream):
+        chunk = 25
+        for f0 in range(0, F, chunk):
+            nf = min(chunk, F - f0)
+            Q.check(Q.lib().qcs_sim_frames_dev(C.byref(sp), 1001, 1 + f0, nf,
+                                               C.c_void_p(frames[f0].data_ptr()), sh), "qcs_sim_frames_dev")
+    stream.synchronize()
+
+    cap = 20000 * F
+    rois = torch.empty((cap, Q.roi_words(R)), dtype=torch.int16, device=dev)
+    wle = torch.empty((cap, Q.WLE_FLOATS), dtype=torch.float32, device=dev)
+    loc = torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev)
+    out = torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev)
+    img = torch.empty((params.SRImageHigh, params.SRImageWidth), dtype=torch.float32, device=dev)
+    ctx = Q.Context(params)
+
+    def step():
+        n, found = ctx.extract_dev(frames, F, 1, rois, wle, cap, sh)
+        if found != n:
+            raise RuntimeError("ROI buffer too small: %d found, %d kept" % (found, n))
+        rows = ctx.fit_dev(rois, wle, n, loc, sh)
+        m = ctx.compact_sorted_dev(loc, rows, 1, F, out, sh)
+        with torch.cuda.stream(stream):
+            img.zero_()
+        ctx.render_dev(out, m, img, None, sh)
+        return n, m
+
+    log("bench: warmup %d step(s)" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    log("bench: timing %d step(s)" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fits = kept = 0
+    for _ in range(args.steps):
+        n, m = step()
+        fits += n
+        kept += m
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = ctx.timing()
+    log("bench: stage timing %s" % json.dumps(tm))
+
+    value = fits / dt
+    fit_launches = max(tm["fit_1e_launches"], 1)
+    fit_avg_ms = tm["fit_1e_ms"] / fit_launches
+    fits_per_launch = tm["fit_1e_rois"] / fit_launches
+    achieved = FLOP_PER_FIT_7X7 * fits_per_launch / (fit_avg_ms * 1e-3) / 1e12
+    tpf = traffic_per_fit()
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS,
+                "traffic": (tpf * fits_per_launch) if tpf is not None else None,
+                "kernel": "fit_1e_kernel<GS2D,7>", "flop_per_launch": FLOP_PER_FIT_7X7 * fits_per_launch,
+                "avg_launch_ms": fit_avg_ms}
+
+    n_last = fits // max(args.steps, 1)
+    stages = {"frames_per_s": F * args.steps / dt,
+              "rois_per_step": n_last, "localizations_per_step": kept // max(args.steps, 1),
+              "extract_ms_per_step": tm["extract_ms"] / args.steps,
+              "fit_ms_per_step": tm["fit_1e_ms"] / args.steps,
+              "crlb_ms_per_step": tm["crlb_ms"] / args.steps,
+              "compact_ms_per_step": tm["compact_ms"] / args.steps,
+              "render_ms_per_step": tm["render_ms"] / args.steps,
+              "fit_kernel_fits_per_s": tm["fit_1e_rois"] / (tm["fit_1e_ms"] * 1e-3),
+              "extract_frame_GBps": F * W * H * 2 * args.steps / (tm["extract_ms"] * 1e-3) / 1e9}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle, bounded sample)")
+        h_rois = rois[:n_last].cpu().numpy().view(np.uint16)
+        h_wle = wle[:n_last].cpu().numpy()
+        cpu, ref = cpu_baseline(params, h_rois, h_wle, args.cpu_seconds)
+        if args.math == "exact":
+            got = loc[:ref.shape[0]].cpu().numpy()
+            same = int(np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1).sum())
+            parity = {"rows": int(ref.shape[0]), "bit_exact_rows": same}
+
+    line = {"metric": METRIC, "value": value, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "full-frame 2D pipeline: %d frames 2048x2048 u16, 0.3 emitters/um^2, "
+                                   "extraction + 7x7 single-emitter BFGS fit + CRLB + sorted compaction + "
+                                   "10 nm rendering" % F,
+                       "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
+                       "rois_per_step": n_last},
+            "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity}
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,6 +83,8 @@ typedef struct orc_fit_stats {
 /* full cascade + CRLB; returns rows written (oValidFluoNum) */
 int   orc_fit(const qcs_params *p, const uint16_t *rois, const float *wle, int n, float *loc,
               int loc_cap, orc_fit_stats *st);
+/* single-emitter mode, OpenMP over ROIs (CPU baseline); -1 in multi-emitter mode */
+int   orc_fit_omp(const qcs_params *p, const uint16_t *rois, const float *wle, int n, float *loc, int threads);
 /* same schedule in FP64 (transcendentals from libm, double); loc in double */
 int   orc_fit_f64(const qcs_params *p, const uint16_t *rois, const float *wle, int n, double *loc,
                   int loc_cap, orc_fit_stats *st);

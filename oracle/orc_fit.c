@@ -34,6 +34,8 @@ int   orc_model_separable(void);
 #undef MAXP
 #undef MAXR
 #undef MAXPIX
+#define M_GS2D_F 0
+#define M_AS3D_F 1
 #undef M_GS2D
 #undef M_AS3D
 #undef M_2E
@@ -70,6 +72,24 @@ int orc_fit(const qcs_params *p, const uint16_t *rois, const float *wle, int n, 
     int r = fit_all_f32(p, rois, w, n, loc, loc_cap, st);
     free(owned);
     return r;
+}
+
+/* Single-emitter mode over ROIs in parallel (OpenMP, `threads` threads): the same per-ROI code as
+ * orc_fit, used as the CPU baseline; returns rows written (= n) or -1 in multi-emitter mode. */
+int orc_fit_omp(const qcs_params *p, const uint16_t *rois, const float *wle, int n, float *loc, int threads)
+{
+    if (p->MultiEmitterFitEn) return -1;
+    const int R = p->ROISize, W = QCS_ROI_WORDS(R);
+    const int model = (p->LocType == QCS_LOCTYPE_AS3D) ? M_AS3D_F : M_GS2D_F;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+    for (int i = 0; i < n; i++) {
+        float snr;
+        int valid;
+        fit_1e_f32(p, model, rois + (long)i * W, wle + (long)i * QCS_WLE_FLOATS, loc + (long)i * QCS_REC_FLOATS, &snr, &valid);
+    }
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int i = 0; i < n; i++) crlb_rows_f32(p, loc + (long)i * QCS_REC_FLOATS, 1);
+    return n;
 }
 
 int orc_fit_f64(const qcs_params *p, const uint16_t *rois, const float *wle, int n, double *loc,

@@ -73,6 +73,11 @@ int fit_launch_me(const qcs_params &p, int model, const uint16_t *d_rois, const 
                   int n_max, const int *d_n, float *d_loc, float *d_second, int *d_fs, int *d_ff, hipStream_t s);
 int crlb_launch(const qcs_params &p, float *d_loc, int n, const int *d_n, hipStream_t s);
 
+// rendering helpers (post.hip): splat rows into the SR image(s); max of (int) image value into
+// *d_out (which must start at 0)
+int render_launch(const qcs_params &p, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);
+int max_disp_launch(const float *d_img, size_t npx, int *d_out, hipStream_t s);
+
 // device utilities (extract.hip): exclusive scan of n ints (total at tile_sums[ntiles]),
 // constant fill, stable compaction out[base + k] = src ? src[j] : j for flagged j.
 int scan_tile_count(size_t n);

@@ -52,6 +52,7 @@ PROTOS = {
     "orc_wle": (None, [_vp, _i, _i, _i, _vp]),
     "orc_fit": (_i, [_PP, _vp, _vp, _i, _vp, _i, _P(OrcStats)]),
     "orc_fit_f64": (_i, [_PP, _vp, _vp, _i, _vp, _i, _P(OrcStats)]),
+    "orc_fit_omp": (_i, [_PP, _vp, _vp, _i, _vp, _i]),
     "orc_fit_params": (_i, [_PP, _i, _vp, _vp, _i, _vp, _i]),
     "orc_fit_params_f64": (_i, [_PP, _i, _vp, _vp, _i, _vp, _i]),
     "orc_loss": (_f, [_PP, _i, _vp, _vp, _vp]),
@@ -120,6 +121,14 @@ def fit(params, rois, wle_rec, cap=None):
     rows = lib().orc_fit(C.byref(params), p(np.ascontiguousarray(rois)), p(np.ascontiguousarray(wle_rec)), n,
                          p(loc), cap, C.byref(st))
     return loc[:rows].copy(), {k: getattr(st, k) for k, _ in OrcStats._fields_}
+
+
+def fit_omp(params, rois, wle_rec, threads):
+    n = rois.shape[0]
+    loc = np.zeros((n, Q.REC_FLOATS), np.float32)
+    lib().orc_fit_omp(C.byref(params), p(np.ascontiguousarray(rois)), p(np.ascontiguousarray(wle_rec)), n, p(loc),
+                      threads)
+    return loc
 
 
 def fit_f64(params, rois, wle_rec, cap=None):
