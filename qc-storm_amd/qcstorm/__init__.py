@@ -305,3 +305,75 @@ class Context:
     def render_dev(self, d_loc, n, d_img, d_zimg=None, stream=None):
         check(lib().qcs_render_dev(self.h, self._dp(d_loc), n, self._dp(d_img), self._dp(d_zimg), stream),
               "qcs_render_dev")
+
+    def loc_batch_sync(self, frames, start_frame=1, cap=None):
+        """lm_LocBatchedImg: extract + fit + frame-sorted compaction of a group of frames (host arrays)."""
+        frames = np.ascontiguousarray(frames, np.uint16)
+        cap = cap or 2 * MAX_POINT_NUM
+        out = np.zeros((cap, REC_FLOATS), np.float32)
+        n = lib().qcs_loc_batch_sync(self.h, _ptr(frames), frames.shape[0], start_frame, _ptr(out), cap)
+        if n < 0:
+            check(int(-n), "qcs_loc_batch_sync")
+        return out[:min(n, cap)].copy()
+
+
+class Session:
+    """Online localization session (the lm_* flow): feed frames, finish, fetch rows and the SR image."""
+
+    def __init__(self, params):
+        self.params = params
+        h = C.c_void_p()
+        check(lib().qcs_session_create(C.byref(params), C.byref(h)), "qcs_session_create")
+        self.h = h
+        # the library may fill SRImageWidth/SRImageHigh
+        if params.SRImageWidth <= 16 or params.SRImageHigh <= 16:
+            lib().qcs_params_update_sr_size(C.byref(self.params))
+
+    def feed(self, frames):
+        frames = np.ascontiguousarray(frames, np.uint16)
+        check(lib().qcs_session_feed(self.h, _ptr(frames), frames.shape[0]), "qcs_session_feed")
+
+    def finish(self):
+        check(lib().qcs_session_finish(self.h), "qcs_session_finish")
+
+    def is_finished(self):
+        return bool(lib().qcs_session_is_finished(self.h))
+
+    def wait(self, timeout_s=120.0, poll_s=0.005):
+        import time
+        t0 = time.time()
+        while not self.is_finished():
+            if time.time() - t0 > timeout_s:
+                raise TimeoutError("session did not finish within %.0f s" % timeout_s)
+            time.sleep(poll_s)
+
+    def locs(self):
+        n = lib().qcs_session_fetch_locs(self.h, None, 0)
+        if n < 0:
+            check(int(-n), "qcs_session_fetch_locs")
+        out = np.zeros((n, REC_FLOATS), np.float32)
+        m = lib().qcs_session_fetch_locs(self.h, _ptr(out), n)
+        if m < 0:
+            check(int(-m), "qcs_session_fetch_locs")
+        return out
+
+    def sr_image(self):
+        img = np.zeros((self.params.SRImageHigh, self.params.SRImageWidth), np.float32)
+        check(lib().qcs_session_get_sr_image(self.h, _ptr(img)), "qcs_session_get_sr_image")
+        return img
+
+    def max_disp_val(self):
+        v = C.c_int()
+        check(lib().qcs_session_get_max_disp_val(self.h, C.byref(v)), "qcs_session_get_max_disp_val")
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().qcs_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

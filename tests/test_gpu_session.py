@@ -1,0 +1,75 @@
+"""Online session (lm_* flow) and the synchronous batch call vs the oracle pipeline.
+
+The oracle pipeline restates the reference's online loop (IJ/OnlineLocalizationLD.cpp:128-300):
+each fed group of frames is one batched image for extraction, ROIs accumulate until PointNumTh
+(or the end), then fit + CRLB, frame-sorted removal of rejected rows over [first, end] and
+rendering.  Rows are compared bit-exactly; the SR image (float atomics) within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import qcstorm as Q
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_pipeline(params, groups, first_frame=1):
+    rois, wles = [], []
+    f = first_frame
+    for g in groups:
+        p = Q.default_params(**{k: getattr(params, k) for k, _ in Q.QcsParams._fields_})
+        p.BatchedImageNum = g.shape[0]
+        r, w, _, _ = O.extract(p, g, start_frame=f, cap=200000)
+        rois.append(r)
+        wles.append(w)
+        f += g.shape[0]
+    rois, wles = np.concatenate(rois), np.concatenate(wles)
+    loc, _ = O.fit(params, rois, wles)
+    return O.compact_sorted(loc, first_frame, f - 1)
+
+
+@pytest.mark.parametrize("lt,mu,R,group", [(0, 0, 7, 1), (0, 1, 7, 2), (1, 0, 9, 3)])
+def test_session_matches_oracle_pipeline(gpu, lt, mu, R, group):
+    W, H, nf = 256, 192, 6
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.006 if mu else 0.003, astig=int(lt == 1))
+    frames = Q.sim_frames_host(sp, 500 + R + group, 1, nf)
+    params = Q.default_params(QE=1.0, ImageWidth=W, ImageHigh=H, LocType=lt, MultiEmitterFitEn=mu, ROISize=R,
+                              PixelZoom=5.0, p1_XGY=-111.11, p1_XLY=-111.11, ZDepthCorrFactor=1.0)
+    Q.lib().qcs_params_update_sr_size(Q.C.byref(params))
+    groups = [frames[i:i + group] for i in range(0, nf, group)]
+    s = Q.Session(params)
+    for g in groups:
+        s.feed(g)
+    s.finish()
+    s.wait()
+    got = s.locs()
+    img = s.sr_image()
+    mx = s.max_disp_val()
+    s.close()
+    ref = _oracle_pipeline(params, groups)
+    print("case", lt, mu, R, group, "rows", got.shape[0], ref.shape[0])
+    assert got.shape == ref.shape and ref.shape[0] > 50
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    rimg, _ = O.render(params, ref, with_z=(lt == 1))
+    np.testing.assert_allclose(img, rimg, rtol=1e-5, atol=1e-5 * max(float(rimg.max()), 1.0))
+    assert mx == max(int(img.max()), 1)
+
+
+def test_loc_batch_sync_matches_oracle(gpu):
+    W, H = 256, 256
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.003)
+    frames = Q.sim_frames_host(sp, 71, 5, 3)
+    params = Q.default_params(QE=1.0, ImageWidth=W, ImageHigh=H)
+    ctx = Q.Context(params)
+    got = ctx.loc_batch_sync(frames, start_frame=5)
+    ref = _oracle_pipeline(params, [frames], first_frame=5)
+    assert got.shape == ref.shape and ref.shape[0] > 50
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_session_rejects_unsupported_modes(gpu):
+    with pytest.raises(Q.QcsError):
+        Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, ConsecFitEn=1))
+    with pytest.raises(Q.QcsError):
+        Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, LocType=Q.LOCTYPE_DH3D))
