@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--math", choices=("exact", "fast"), default="exact")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the localization path runs on a single MI355X "
@@ -93,9 +95,19 @@ def main():
     import torch
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (torch.cuda.is_available() is False)")
-    Q.lib()  # raises when libqcstorm.so is missing: there is no fallback
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        # 1 GiB read + 1 GiB written per launch: the reference point for FETCH_SIZE / WRITE_SIZE
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(max(args.steps, 1)):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read_per_launch": src.numel() * 4, "copy_bytes_written_per_launch": dst.numel() * 4,
+                          "launches": max(args.steps, 1)}), flush=True)
+        return
+    Q.lib()  # raises when libqcstorm.so is missing: there is no fallback
     stream = torch.cuda.Stream(device=dev)
     sh = C.c_void_p(stream.cuda_stream)
 

@@ -13,23 +13,27 @@
 //
 // Pipeline for one call (any number of frames, batches of BatchedImageNum frames; no host
 // synchronisation until the final count read):
-//   1. var_kernel     exact integer sums of the background-removed variance samples per batch
-//   2. detect_kernel  one 64 x 16 output tile per work-group: raw tile + halo -> LDS, background
-//                     removal, 5-tap horizontal then vertical smoothing (uint16 truncation after
-//                     each pass, as the reference stores them), 3x3 local-maximum test against
-//                     the batch's thresholds.  Candidates and marks leave as one 64-bit ballot
-//                     mask per row segment; per-row candidate counts via integer atomics.
-//   3. scan           exclusive prefix sum of the row counts -> raster-order offsets
-//   4. batch_kernel   per-batch found / kept (capacity) / output offsets
-//   5. emit_kernel    candidate (x, y) list in raster order
-//   6. cut_kernel     detector classification (multi mode), raw R x R cut, header
-//   7. WLE            PSF width, nearest neighbour (sorted sweep), type
+//   1. walk_kernel<0>     background-removed values over each batch's variance region
+//   2. var_reduce_kernel  exact integer sums of the variance samples per batch
+//   3. walk_kernel<1>     one wave per 56-column strip walks down the rows: raw rows enter a
+//                         9-row LDS ring, background removal, 5-tap H and V smoothing (uint16
+//                         truncation after each pass, as the reference stores them), 3x3
+//                         local-maximum test against the batch's thresholds.  Candidates and
+//                         marks leave as one 56-bit ballot mask per row and strip; per-row
+//                         candidate counts via integer atomics.
+//   4. scan               exclusive prefix sum of the row counts -> raster-order offsets
+//   5. batch_kernel       per-batch found / kept (capacity) / output offsets
+//   6. emit_kernel        candidate (x, y) list in raster order
+//   7. cut_kernel         detector classification (multi mode), raw R x R cut, header
+//   8. WLE                PSF width, nearest neighbour (sorted sweep over the list), type
 #include <string.h>
+#include <type_traits>
 #include "qcs_internal.h"
 
 namespace qcs {
 
-int wle_launch_devn(const qcs_params &p, const uint16_t *d_rois, const int *d_n, int cap, float *d_wle, hipStream_t s);
+int wle_launch_extract(const qcs_params &p, const uint16_t *d_rois, const int3 *d_xyb, int B, int H, const int *d_n, int cap,
+                       float *d_wle, hipStream_t s);
 void filter_taps_host(int R, int loc_type, int multi, float taps[5]);
 
 // FilterInit (SE/LDROIExtraction_ImageFiltering.cu:22-81): host FP32 without contraction; the
@@ -69,7 +73,8 @@ struct ExtractConst {
     int batch_cap;         // candidates kept per batch
     int cap;               // output capacity
     int start_frame;
-    int nseg;              // 64-pixel segments per row
+    int nseg;              // SEGW-column segments per row (one walking wave each)
+    int walk_rows;         // image rows per walking wave
     float taps[5];
 };
 
@@ -83,86 +88,24 @@ __device__ __forceinline__ int batch_rows(const ExtractConst &c, int b)
     return nb * c.H;
 }
 
-// sum over 8 positions of the minimum of four border samples of the 9 x 9 window (exact in
-// integers; the reference's float sum of uint16 minima is exact too), from global memory
-__device__ __forceinline__ int bkg_sum_global(const uint16_t *img, int W, int BH, int x, int y)
+// variance region of batch b: S x S samples at (cx + i, cy + j), each the minimum of the
+// background-removed image at (X, Y), (X - S, Y), (X, Y - S), (X - S, Y - S)
+// (ImageVarianceCalc, SE/LDROIExtraction.cu:140-177; gpuImageVarianceCalcOnSelectRegion,
+// SE/LDROIExtraction_ImageFiltering.cu:503-533)
+__device__ __forceinline__ int var_side(const ExtractConst &c, int BH)
 {
-    const int yt = clampi(y - 4, 0, BH - 1), yb = clampi(y + 4, 0, BH - 1);
-    const int xl = clampi(x - 4, 0, W - 1), xr = clampi(x + 4, 0, W - 1);
-    int s = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int l1 = img[(size_t)yt * W + clampi(x - 4 + i, 0, W - 1)];
-        const int l2 = img[(size_t)clampi(y - 4 + i, 0, BH - 1) * W + xr];
-        const int l3 = img[(size_t)yb * W + clampi(x - 3 + (7 - i), 0, W - 1)];
-        const int l4 = img[(size_t)clampi(y - 3 + (7 - i), 0, BH - 1) * W + xl];
-        const int a = l1 < l2 ? l1 : l2, b = l3 < l4 ? l3 : l4;
-        s += a < b ? a : b;
-    }
-    return s;
-}
-
-// background-removed value (uint16 truncation) of in-image pixel (x, y), from global memory
-__device__ __forceinline__ int bkg_removed_global(const uint16_t *img, int W, int BH, int x, int y)
-{
-    const int g0 = x & ~31;
-    const int xl = (x - 1 < g0) ? g0 : x - 1, xr = (x + 1 > g0 + 31) ? g0 + 31 : x + 1;
-    const float b0 = (float)bkg_sum_global(img, W, BH, xl, y) / 8.0f;
-    const float b1 = (float)bkg_sum_global(img, W, BH, x, y) / 8.0f;
-    const float b2 = (float)bkg_sum_global(img, W, BH, xr, y) / 8.0f;
-    const float mb = ((b0 + b1) + b2) / 3.0f;
-    float v = (float)img[(size_t)y * W + x] - mb;
-    v = fmaxf(v, 0.0f);
-    return (int)v;
-}
-
-// ---- 1. variance sums per batch -------------------------------------------------------------
-__global__ void __launch_bounds__(256) var_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
-                                                  unsigned long long *__restrict__ sums)
-{
-    const int b = blockIdx.y;
-    const int BH = batch_rows(c, b);
-    const uint16_t *img = frames + (size_t)b * c.B * c.H * c.W;
-    const int cx = c.W / 2, cy = BH / 2;
     int sw = c.W / 2 - 5, sh = BH / 2 - 5;
     sw = sw > 256 ? 256 : sw;
     sh = sh > 256 ? 256 : sh;
     sw = sw < 1 ? 1 : sw;
     sh = sh < 1 ? 1 : sh;
-    const int S = sw < sh ? sw : sh;
-    unsigned long long s1 = 0, s2 = 0;
-    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < S * S; g += gridDim.x * blockDim.x) {
-        const int X = cx + g % S, Y = cy + g / S;
-        int d1 = bkg_removed_global(img, c.W, BH, X, Y);
-        const int d2 = bkg_removed_global(img, c.W, BH, X - S, Y);
-        int d3 = bkg_removed_global(img, c.W, BH, X, Y - S);
-        const int d4 = bkg_removed_global(img, c.W, BH, X - S, Y - S);
-        d1 = d1 < d2 ? d1 : d2;
-        d3 = d3 < d4 ? d3 : d4;
-        d1 = d1 < d3 ? d1 : d3;
-        s1 += (unsigned long long)d1;
-        s2 += (unsigned long long)d1 * (unsigned long long)d1;
-    }
-    // wave reduction then one atomic per wave (integer: order-independent)
-    for (int off = 32; off > 0; off >>= 1) {
-        s1 += __shfl_down(s1, off, 64);
-        s2 += __shfl_down(s2, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&sums[2 * b + 0], s1);
-        atomicAdd(&sums[2 * b + 1], s2);
-    }
+    return sw < sh ? sw : sh;
 }
 
 __device__ __forceinline__ void batch_thresholds(const ExtractConst &c, const unsigned long long *sums, int b, int BH,
                                                  float &th, float &th1)
 {
-    int sw = c.W / 2 - 5, sh = BH / 2 - 5;
-    sw = sw > 256 ? 256 : sw;
-    sh = sh > 256 ? 256 : sh;
-    sw = sw < 1 ? 1 : sw;
-    sh = sh < 1 ? 1 : sh;
-    const int S = sw < sh ? sw : sh;
+    const int S = var_side(c, BH);
     const double n = (double)S * (double)S;
     const double m1 = (double)sums[2 * b] / n, m2 = (double)sums[2 * b + 1] / n;
     float var = (float)(m2 - m1 * m1);
@@ -177,144 +120,312 @@ __device__ __forceinline__ void batch_thresholds(const ExtractConst &c, const un
     }
 }
 
-// ---- 2. fused filter + detect ----------------------------------------------------------------
-constexpr int TW = 64, TH = 16;                    // output tile
-constexpr int RAW_W = TW + 16, RAW_H = TH + 14;   // raw halo: cols x0-8.., rows y0-7..
-constexpr int BK_W = TW + 8, BK_H = TH + 6;       // background sums: cols x0-4.., rows y0-3..
-constexpr int T_W = TW + 6;                       // background-removed: cols x0-3.., rows y0-3..
-constexpr int HC_W = TW + 2;                      // H-smoothed: cols x0-1.., rows y0-3..
-constexpr int S_H = TH + 2;                       // V-smoothed: cols x0-1.., rows y0-1..
+// ---- 1+2. row-walking filter wave ----------------------------------------------------------
+// One wave (64 lanes, one column each) walks down a strip of the batched image.  Lane l owns
+// column xc = x0 - 4 + l; the background sum needs raw columns xc-4..xc+4, so the raw rows in
+// LDS are 72 wide (8 halo columns loaded by lanes 0..7).  Each stage narrows the valid lanes by
+// its stencil reach: background sums on lanes 0..63, background-removed values on 1..62,
+// H-smoothed values on 3..60, detection on 4..59 -- SEGW = 56 output columns per wave.
+//
+// A step handles 4 image rows r..r+3: the raw rows r+4..r+7 (loaded from HBM during the previous
+// step) enter a 12-row LDS ring, then for each row r:
+//   bk(r, x) = sum_{i<8} min(R(r-4, x-4+i), R(r-4+i, x+4), R(r+4, x+4-i), R(r+4-i, x-4))
+//              with R the raw image, row/column indices clamped to the batch
+//              (gpuRemoveBackgroundIntensity, SE/LDROIExtraction_ImageFiltering.cu:125-267);
+//   tv(r, x) = uint16 trunc(max(raw - ((bk[xl] / 8 + bk[x] / 8) + bk[xr] / 8) / 3, 0)), xl/xr the
+//              neighbours clamped to x's 32-column group.  The float expression equals the
+//              integer max(raw - ceil((bk[xl] + bk[x] + bk[xr]) / 24), 0) exactly: the three
+//              eighths sum exactly, s/3 rounds to within 2^-9 of S/24, and S/24 is either an
+//              integer or at least 1/24 from one, so the truncation never changes;
+//   hc(r, x) = uint16(min(fma-chain of tv(r, clamp(x-2+t)) * tap[t], 65535))      (H pass)
+//   sv(y, x) = uint16(min(fma-chain of hc(clamp(y-2+t), x) * tap[t], 65535))      (V pass,
+//              gpuImageConv_Seperable_H/V, SE/LDROIExtraction_ImageFiltering.cu:286-470);
+//   detection of rows r-3..r once sv(r+1) exists (gpuROIFindingLD,
+//   SE/LDROIExtraction_ROIExtraction.cu:25-164).
+// Cross-lane values (background sums, tv, sv) go through LDS with one barrier per phase; hc stays
+// lane-local.  Four barriers per four rows keep the wave's LDS round trips off the critical path.
+// MODE 0 computes tv only, over the variance region, into tvbuf; MODE 1 is the full pass.
+constexpr int SEGW = 56;             // output columns per wave
+constexpr int RAWW = 72;             // raw row width in LDS
+constexpr int VAR_MAX = 512;         // region side bound (2 * 256)
 
-__global__ void __launch_bounds__(256) detect_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
-                                                     const unsigned long long *__restrict__ var_sums,
-                                                     unsigned long long *__restrict__ cand_mask,
-                                                     unsigned long long *__restrict__ mark_mask,
-                                                     int *__restrict__ row_count)
+template <int MODE, bool SMOOTH>
+__global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
+                                                  const unsigned long long *__restrict__ var_sums,
+                                                  uint16_t *__restrict__ tvbuf,
+                                                  unsigned long long *__restrict__ cand_mask,
+                                                  unsigned long long *__restrict__ mark_mask,
+                                                  int *__restrict__ row_count)
 {
-    __shared__ uint16_t raw[RAW_H][RAW_W];
-    __shared__ int bk[BK_H][BK_W];
-    __shared__ uint16_t tv[BK_H][T_W];
-    __shared__ uint16_t hc[BK_H][HC_W];
-    __shared__ uint16_t sv[S_H][HC_W];
+    constexpr int RS = 4;    // image rows per step
+    constexpr int NR = 16;   // raw ring rows (a step needs rows r-4 .. r+RS+3); power of two
+    __shared__ uint16_t raw[NR][RAWW];
+    __shared__ int bks[RS][64];
+    __shared__ float tvs[RS][64];
+    __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
+    __shared__ int svs[8][64];
 
     const int b = blockIdx.z;
     const int BH = batch_rows(c, b);
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    if (y0 >= BH) return;
     const int W = c.W;
     const uint16_t *img = frames + (size_t)b * c.B * c.H * W;
-    const int tid = threadIdx.x;
+    const int l = threadIdx.x;
 
-    // raw tile with clamped coordinates: raw[i][j] = img(clamp(y0-7+i), clamp(x0-8+j))
-    for (int k = tid; k < RAW_H * RAW_W; k += 256) {
-        const int i = k / RAW_W, j = k % RAW_W;
-        raw[i][j] = img[(size_t)clampi(y0 - 7 + i, 0, BH - 1) * W + clampi(x0 - 8 + j, 0, W - 1)];
+    int x0, ys, ye, rb, rlast;
+    int S = 0, rx0 = 0, ry0 = 0;
+    if (MODE == 0) {
+        S = var_side(c, BH);
+        rx0 = W / 2 - S;
+        ry0 = BH / 2 - S;
+        x0 = rx0 + blockIdx.x * SEGW;
+        if (x0 >= rx0 + 2 * S) return;
+        ys = ry0 + blockIdx.y * c.walk_rows;
+        if (ys >= ry0 + 2 * S) return;
+        ye = min(ys + c.walk_rows, ry0 + 2 * S);
+        rb = ys;          // steps compute tv rows r .. r+3
+        rlast = ye - 1;
+    } else {
+        x0 = blockIdx.x * SEGW;
+        ys = blockIdx.y * c.walk_rows;
+        if (ys >= BH) return;
+        ye = min(ys + c.walk_rows, BH);
+        rb = ys - 4;      // step r: hc rows r..r+3, sv rows r-2..r+1, detection rows r-3..r
+        rlast = ye + 2;
     }
-    __syncthreads();
-    // background sums at rows y0-3+i, cols x0-4+j (columns past the right edge keep the
-    // reference's clamped loads; rows outside the batch are never read)
-    for (int k = tid; k < BK_H * BK_W; k += 256) {
-        const int i = k / BK_W, j = k % BK_W;
-        const int ri = i + 4, cj = j + 4;   // position in raw[]
-        int s = 0;
+    // ring slots are relative to rb; a step starts at rb + 4 * (4n + PH), so in interior steps the
+    // slots are compile-time functions of the phase PH
+    auto slot = [&](int v) { return (v - rb + 2 * NR) & (NR - 1); };
+    auto ring8 = [&](int v) { return (v - rb + 16) & 7; };
+    const int xc = x0 - 4 + l;
+    const int xcl = clampi(xc, 0, W - 1);
+    // halo columns: lanes 0..3 -> x0-8+l (slot l), lanes 4..7 -> x0+60+(l-4) (slot 68+l-4)
+    const int hx = (l < 4) ? clampi(x0 - 8 + l, 0, W - 1) : clampi(x0 + 56 + l, 0, W - 1);
+    const int hslot = (l < 4) ? l : 64 + l;
+    auto ldp = [&](const uint16_t *src, uint16_t &main, uint16_t &halo) {
+        main = src[xcl];
+        halo = (l < 8) ? src[hx] : (uint16_t)0;
+    };
+    auto ld = [&](int v, uint16_t &main, uint16_t &halo) { ldp(img + (unsigned)(clampi(v, 0, BH - 1) * W), main, halo); };
+
+    for (int v = rb - 4; v < rb + 4; v++) {
+        uint16_t m, h;
+        ld(v, m, h);
+        raw[slot(v)][l + 4] = m;
+        if (l < 8) raw[slot(v)][hslot] = h;
+    }
+    uint16_t pf[RS], pfh[RS];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int l1 = raw[ri - 4][cj - 4 + t];
-            const int l2 = raw[ri - 4 + t][cj + 4];
-            const int l3 = raw[ri + 4][cj - 3 + (7 - t)];
-            const int l4 = raw[ri - 3 + (7 - t)][cj - 4];
-            const int a = l1 < l2 ? l1 : l2, bb = l3 < l4 ? l3 : l4;
-            s += a < bb ? a : bb;
+    for (int k = 0; k < RS; k++) ld(rb + 4 + k, pf[k], pfh[k]);
+
+    float th = 0.0f, th1 = 0.0f;
+    float tap[5] = {0, 0, 0, 0, 0};
+    if (MODE == 1) {
+        batch_thresholds(c, var_sums, b, BH, th, th1);
+#pragma unroll
+        for (int t = 0; t < 5; t++) tap[t] = c.taps[t];
+    }
+    const int g0 = xc & ~31;
+    const int li = max(min(((xc - 1 < g0) ? g0 : xc - 1) - (x0 - 4), 63), 0);
+    const int ri = max(min(((xc + 1 > g0 + 31) ? g0 + 31 : xc + 1) - (x0 - 4), 63), 0);
+    int hci[5];   // H-pass source lanes (columns clamped to the image)
+#pragma unroll
+    for (int t = 0; t < 5; t++) hci[t] = max(min(clampi(xc - 2 + t, 0, W - 1) - (x0 - 4), 63), 0);
+    const int half = c.R / 2;
+    const bool out_lane = (l >= 4 && l < 4 + SEGW);
+    const bool det_col = out_lane && xc < W && xc >= half && xc < W - half;
+    const int ylo = max(ys - 1, 0), yhi = min(ye, BH - 1);   // sv rows this wave needs
+    unsigned long long *cmask = cand_mask + (size_t)b * c.B * c.H * c.nseg + blockIdx.x;
+    unsigned long long *mmask = mark_mask + (size_t)b * c.B * c.H * c.nseg + blockIdx.x;
+    int *rcount = row_count + (size_t)b * c.B * c.H;
+    const int nseg = c.nseg, H = c.H;
+    int ryd = ((rb - 3) % H + H) % H;   // frame row of detection row r-3 for the current step
+
+    // One step of 4 rows.  EDGE: rows near the batch or walk ends (bounds checks, clamped rows,
+    // dynamic ring slots); otherwise every row is in range and PH fixes the ring slots.
+    auto step = [&](int r, auto edge_tag, auto ph_tag) {
+        constexpr bool EDGE = decltype(edge_tag)::value;
+        constexpr int PH = decltype(ph_tag)::value;
+        auto rs = [&](int off) { if constexpr (EDGE) return slot(r + off); else return (4 * PH + off + 2 * NR) & (NR - 1); };
+        auto h8 = [&](int off) { if constexpr (EDGE) return ring8(r + off); else return (4 * PH + off + 16) & 7; };
+        // stage rows r+4 .. r+7 (loaded one step ago), then prefetch rows r+8 .. r+11
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            raw[rs(4 + k)][l + 4] = pf[k];
+            if (l < 8) raw[rs(4 + k)][hslot] = pfh[k];
         }
-        bk[i][j] = s;
-    }
-    __syncthreads();
-    // background-removed values at in-image rows y0-3+i, cols x0-3+j
-    for (int k = tid; k < BK_H * T_W; k += 256) {
-        const int i = k / T_W, j = k % T_W;
-        const int y = y0 - 3 + i, x = x0 - 3 + j;
-        if (y < 0 || y >= BH || x < 0 || x >= W) continue;
-        const int g0 = x & ~31;
-        const int xl = (x - 1 < g0) ? g0 : x - 1, xr = (x + 1 > g0 + 31) ? g0 + 31 : x + 1;
-        const float b0 = (float)bk[i][xl - (x0 - 4)] / 8.0f;
-        const float b1 = (float)bk[i][x - (x0 - 4)] / 8.0f;
-        const float b2 = (float)bk[i][xr - (x0 - 4)] / 8.0f;
-        const float mb = ((b0 + b1) + b2) / 3.0f;
-        float v = (float)raw[i + 4][j + 5] - mb;
-        v = fmaxf(v, 0.0f);
-        tv[i][j] = (uint16_t)v;
-    }
-    __syncthreads();
-    if (c.smooth) {
-        // horizontal pass at rows y0-3+i, cols x0-1+j (clamp-to-edge reads of the removed image)
-        for (int k = tid; k < BK_H * HC_W; k += 256) {
-            const int i = k / HC_W, j = k % HC_W;
-            const int y = y0 - 3 + i, x = x0 - 1 + j;
-            if (y < 0 || y >= BH || x < 0 || x >= W) continue;
-            float acc = 0.0f;
+        if constexpr (EDGE) {
 #pragma unroll
-            for (int t = 0; t < 5; t++)
-                acc = __builtin_fmaf((float)tv[i][clampi(x - 2 + t, 0, W - 1) - (x0 - 3)], c.taps[t], acc);
-            acc = acc > 65535.0f ? 65535.0f : acc;
-            hc[i][j] = (uint16_t)acc;
+            for (int k = 0; k < RS; k++) ld(r + 8 + k, pf[k], pfh[k]);
+        } else {
+            const uint16_t *p8 = img + (unsigned)((r + 8) * W);
+#pragma unroll
+            for (int k = 0; k < RS; k++) ldp(p8 + k * W, pf[k], pfh[k]);
         }
         __syncthreads();
-        // vertical pass at rows y0-1+i, cols x0-1+j
-        for (int k = tid; k < S_H * HC_W; k += 256) {
-            const int i = k / HC_W, j = k % HC_W;
-            const int y = y0 - 1 + i, x = x0 - 1 + j;
-            if (y < 0 || y >= BH || x < 0 || x >= W) continue;
-            float acc = 0.0f;
+        // background sums of rows r .. r+3
+        int colR[RS + 7], colL[RS + 7];   // column xc+4 rows r-4.., column xc-4 rows r-3..
 #pragma unroll
-            for (int t = 0; t < 5; t++)
-                acc = __builtin_fmaf((float)hc[clampi(y - 2 + t, 0, BH - 1) - (y0 - 3)][j], c.taps[t], acc);
-            acc = acc > 65535.0f ? 65535.0f : acc;
-            sv[i][j] = (uint16_t)acc;
+        for (int i = 0; i < RS + 7; i++) {
+            colR[i] = raw[rs(i - 4)][l + 8];
+            colL[i] = raw[rs(i - 3)][l];
         }
-    } else {
-        for (int k = tid; k < S_H * HC_W; k += 256) {
-            const int i = k / HC_W, j = k % HC_W;
-            const int y = y0 - 1 + i, x = x0 - 1 + j;
-            if (y < 0 || y >= BH || x < 0 || x >= W) continue;
-            sv[i][j] = tv[i + 2][j + 2];
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            const uint16_t *rA = raw[rs(k - 4)], *rB = raw[rs(k + 4)];
+            int bk = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int l1 = rA[l + i], l2 = colR[k + i];
+                const int l3 = rB[l + 8 - i], l4 = colL[k + 7 - i];
+                const int m1 = l1 < l2 ? l1 : l2, m2 = l3 < l4 ? l3 : l4;
+                bk += m1 < m2 ? m1 : m2;
+            }
+            bks[k][l] = bk;
         }
-    }
-    __syncthreads();
-
-    // detection: wave w handles rows y0 + 4w .. +3, lane = column
-    float th, th1;
-    batch_thresholds(c, var_sums, b, BH, th, th1);
-    const int lane = tid & 63, wv = tid >> 6;
-    const int half = c.R / 2;
-    const int x = x0 + lane;
-    for (int rr = 0; rr < 4; rr++) {
-        const int y = y0 + wv * 4 + rr;
-        if (y >= BH) break;   // uniform across the wave
-        const int ry = y % c.H;
-        bool cand = false, mark = false;
-        if (x < W && x >= half && ry >= half && x < W - half && ry < c.H - half) {
-            const int si = y - (y0 - 1), sj = x - (x0 - 1);
-            const int cv = sv[si][sj];
-            if ((float)cv >= th) {
-                const int p00 = sv[si - 1][sj - 1], p01 = sv[si - 1][sj], p02 = sv[si - 1][sj + 1];
-                const int p10 = sv[si][sj - 1], p12 = sv[si][sj + 1];
-                const int p20 = sv[si + 1][sj - 1], p21 = sv[si + 1][sj], p22 = sv[si + 1][sj + 1];
-                const int s8 = ((float)p00 > th1) + ((float)p01 > th1) + ((float)p02 > th1) + ((float)p10 > th1) +
-                               ((float)p12 > th1) + ((float)p20 > th1) + ((float)p21 > th1) + ((float)p22 > th1);
-                const int sin = (cv > p01) + (cv >= p12) + (cv >= p21) + (cv > p10);
-                const int sout = (cv > p00) + (cv > p02) + (cv > p20) + (cv > p22);
-                mark = (s8 >= 7 && sin == 4 && sout >= 3);
-                cand = mark && (sout == 4);
+        __syncthreads();
+        int tvk[RS];
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            const int rawc = raw[rs(k)][l + 4];
+            const int t = rawc - (bks[k][li] + bks[k][l] + bks[k][ri] + 23) / 24;
+            tvk[k] = t > 0 ? t : 0;
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int k = 0; k < RS; k++) {
+                const int v = r + k;
+                if (out_lane && v < ye && xc < rx0 + 2 * S)
+                    tvbuf[(size_t)b * VAR_MAX * VAR_MAX + (size_t)(v - ry0) * VAR_MAX + (xc - rx0)] = (uint16_t)tvk[k];
+            }
+            return;
+        }
+        if constexpr (SMOOTH) {
+#pragma unroll
+            for (int k = 0; k < RS; k++) tvs[k][l] = (float)tvk[k];
+            __syncthreads();
+        }
+        // H pass into the lane's ring (rows inside the batch only)
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            if (EDGE && (r + k < 0 || r + k >= BH)) continue;   // uniform
+            float hc;
+            if constexpr (SMOOTH) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 5; t++) acc = __builtin_fmaf(tvs[k][hci[t]], tap[t], acc);
+                acc = acc > 65535.0f ? 65535.0f : acc;
+                hc = (float)(int)acc;
+            } else {
+                hc = (float)tvk[k];
+            }
+            hcs[h8(k)][l] = hc;
+        }
+        // V pass: sv rows r-2 .. r+1 (hc rows clamped to the batch)
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            const int y = r - 2 + k;
+            if (EDGE && (y < ylo || y > yhi)) continue;   // uniform
+            int sv;
+            if constexpr (SMOOTH) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 5; t++) {
+                    float hv;
+                    if constexpr (EDGE) hv = hcs[ring8(clampi(y - 2 + t, 0, BH - 1))][l];
+                    else hv = hcs[h8(k - 4 + t)][l];
+                    acc = __builtin_fmaf(hv, tap[t], acc);
+                }
+                acc = acc > 65535.0f ? 65535.0f : acc;
+                sv = (int)acc;
+            } else {
+                sv = (int)hcs[h8(k - 2)][l];
+            }
+            svs[h8(k - 2)][l] = sv;
+        }
+        __syncthreads();
+        // detection rows r-3 .. r
+#pragma unroll
+        for (int k = 0; k < RS; k++) {
+            const int d = r - 3 + k;
+            int ry = ryd + k;
+            ry = ry >= H ? ry - H : ry;
+            if (EDGE && (d < ys || d >= ye)) continue;   // uniform
+            bool cand = false, mark = false;
+            if (det_col && ry >= half && ry < H - half) {
+                const int *md = svs[h8(k - 3)];
+                const int cv = md[l];
+                if ((float)cv >= th) {
+                    const int *up = svs[h8(k - 4)], *dn = svs[h8(k - 2)];
+                    const int p00 = up[l - 1], p01 = up[l], p02 = up[l + 1];
+                    const int p10 = md[l - 1], p12 = md[l + 1];
+                    const int p20 = dn[l - 1], p21 = dn[l], p22 = dn[l + 1];
+                    const int s8 = ((float)p00 > th1) + ((float)p01 > th1) + ((float)p02 > th1) + ((float)p10 > th1) +
+                                   ((float)p12 > th1) + ((float)p20 > th1) + ((float)p21 > th1) + ((float)p22 > th1);
+                    const int sin = (cv > p01) + (cv >= p12) + (cv >= p21) + (cv > p10);
+                    const int sout = (cv > p00) + (cv > p02) + (cv > p20) + (cv > p22);
+                    mark = (s8 >= 7 && sin == 4 && sout >= 3);
+                    cand = mark && (sout == 4);
+                }
+            }
+            const unsigned long long cm = (__ballot(cand) >> 4) & ((1ULL << SEGW) - 1);
+            const unsigned long long mm = (__ballot(mark) >> 4) & ((1ULL << SEGW) - 1);
+            if (l == 0) {
+                cmask[(size_t)d * nseg] = cm;
+                mmask[(size_t)d * nseg] = mm;
+                if (cm) atomicAdd(&rcount[d], __popcll(cm));
             }
         }
-        const unsigned long long cm = __ballot(cand);
-        const unsigned long long mm = __ballot(mark);
-        if (lane == 0) {
-            const size_t grow = (size_t)b * c.B * c.H + y;   // stacked row index
-            cand_mask[grow * c.nseg + blockIdx.x] = cm;
-            mark_mask[grow * c.nseg + blockIdx.x] = mm;
-            if (cm) atomicAdd(&row_count[grow], __popcll(cm));
+    };
+
+    using E = std::true_type;
+    using I = std::false_type;
+    for (int r = rb; r <= rlast; r += RS) {
+        bool edge;
+        if (MODE == 0) edge = (r - 4 < 0) || (r + 11 > BH - 1) || (r + RS - 1 >= ye);
+        else edge = (r - 4 < 0) || (r + 11 > BH - 1) || (r - 2 < ylo) || (r + 1 > yhi) || (r - 3 < ys) || (r >= ye);
+        if (edge) {
+            step(r, E{}, std::integral_constant<int, 0>{});
+        } else {
+            switch (((r - rb) >> 2) & 3) {
+            case 0: step(r, I{}, std::integral_constant<int, 0>{}); break;
+            case 1: step(r, I{}, std::integral_constant<int, 1>{}); break;
+            case 2: step(r, I{}, std::integral_constant<int, 2>{}); break;
+            default: step(r, I{}, std::integral_constant<int, 3>{}); break;
+            }
         }
+        ryd += RS;
+        ryd = ryd >= H ? ryd - H : ryd;
+    }
+}
+
+// integer sums of the variance samples of each batch from the region's background-removed values
+__global__ void __launch_bounds__(256) var_reduce_kernel(ExtractConst c, const uint16_t *__restrict__ tvbuf,
+                                                         unsigned long long *__restrict__ sums)
+{
+    const int b = blockIdx.y;
+    const int BH = batch_rows(c, b);
+    const int S = var_side(c, BH);
+    const uint16_t *t = tvbuf + (size_t)b * VAR_MAX * VAR_MAX;
+    unsigned long long s1 = 0, s2 = 0;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < S * S; g += gridDim.x * blockDim.x) {
+        const int i = g % S, j = g / S;   // sample (X, Y) = (cx + i, cy + j), region origin (cx - S, cy - S)
+        int d1 = t[(size_t)(S + j) * VAR_MAX + S + i];
+        const int d2 = t[(size_t)(S + j) * VAR_MAX + i];
+        int d3 = t[(size_t)j * VAR_MAX + S + i];
+        const int d4 = t[(size_t)j * VAR_MAX + i];
+        d1 = d1 < d2 ? d1 : d2;
+        d3 = d3 < d4 ? d3 : d4;
+        d1 = d1 < d3 ? d1 : d3;
+        s1 += (unsigned long long)d1;
+        s2 += (unsigned long long)d1 * (unsigned long long)d1;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_down(s1, off, 64);
+        s2 += __shfl_down(s2, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&sums[2 * b + 0], s1);
+        atomicAdd(&sums[2 * b + 1], s2);
     }
 }
 
@@ -428,7 +539,7 @@ __global__ void __launch_bounds__(256) emit_kernel(ExtractConst c, const unsigne
         while (m) {
             const int bit = __ffsll((long long)m) - 1;
             m &= m - 1;
-            if (local < kept) xyb[out0 + local] = make_int3(sgi * 64 + bit, y, b);
+            if (local < kept) xyb[out0 + local] = make_int3(sgi * SEGW + bit, y, b);
             local++;
         }
     }
@@ -437,47 +548,64 @@ __global__ void __launch_bounds__(256) emit_kernel(ExtractConst c, const unsigne
 // ---- 6. classification + cut -----------------------------------------------------------------
 __device__ __forceinline__ int mark_count(const unsigned long long *mark, int nseg, size_t row, int xa, int xb)
 {
-    // number of marks in [xa, xb] of one stacked row
+    // number of marks in [xa, xb] of one stacked row (segments of SEGW columns)
     int cnt = 0;
-    for (int sgi = xa >> 6; sgi <= (xb >> 6); sgi++) {
+    for (int sgi = xa / SEGW; sgi <= xb / SEGW; sgi++) {
         unsigned long long m = mark[row * nseg + sgi];
-        const int lo = sgi * 64;
+        const int lo = sgi * SEGW;
         if (xa > lo) m &= ~0ULL << (xa - lo);
-        if (xb < lo + 63) m &= ~0ULL >> (63 - (xb - lo));
+        if (xb < lo + SEGW - 1) m &= ~0ULL >> (63 - (xb - lo));
         cnt += __popcll(m);
     }
     return cnt;
 }
 
-__global__ void __launch_bounds__(256) cut_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
-                                                  const unsigned long long *__restrict__ mark_mask,
-                                                  const int *__restrict__ info, const int3 *__restrict__ xyb,
-                                                  uint16_t *__restrict__ rois)
+// one wave per ROI record (grid-stride over the kept count): lanes write consecutive uint16
+// words of the record, so the stores are contiguous
+__global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
+                                                 const unsigned long long *__restrict__ mark_mask,
+                                                 const int *__restrict__ info, const int3 *__restrict__ xyb,
+                                                 uint16_t *__restrict__ rois)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= info[0]) return;
-    const int R = c.R, half = R / 2, WR = R * (R + 1);
-    const int3 e = xyb[i];
-    const int x = e.x, y = e.y, b = e.z;
-    const size_t rbase = (size_t)b * c.B * c.H;   // stacked row of the batch's first row
-    int ft = QCS_ROITYPE_SINGLE;
-    if (c.multi) {
-        int all = 0, inner = 0;
-        for (int r = 0; r < R; r++) all += mark_count(mark_mask, c.nseg, rbase + y - half + r, x - half, x + half);
-        for (int r = 0; r < 3; r++) inner += mark_count(mark_mask, c.nseg, rbase + y - 1 + r, x - 1, x + 1);
-        ft = (all - inner > 0) ? QCS_ROITYPE_MULTI : QCS_ROITYPE_SINGLE;
+    const int n = info[0];
+    const int R = c.R, half = R / 2, WR = R * (R + 1), RR = R * R;
+    const int l = threadIdx.x;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int3 e = xyb[i];
+        const int x = e.x, y = e.y, b = e.z;
+        const size_t rbase = (size_t)b * c.B * c.H;   // stacked row of the batch's first row
+        int ft = QCS_ROITYPE_SINGLE;
+        if (c.multi) {
+            // lanes 0..R-1 count the marks of one window row each
+            int all = 0, inner = 0;
+            if (l < R) {
+                const size_t row = rbase + y - half + l;
+                all = mark_count(mark_mask, c.nseg, row, x - half, x + half);
+                if (l >= half - 1 && l <= half + 1) inner = mark_count(mark_mask, c.nseg, row, x - 1, x + 1);
+            }
+            int d = all - inner;
+            for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+            ft = (d > 0) ? QCS_ROITYPE_MULTI : QCS_ROITYPE_SINGLE;
+        }
+        const uint16_t *img = frames + rbase * c.W;
+        uint16_t *o = rois + (size_t)i * WR;
+        const int fr = c.start_frame + b * c.B + y / c.H;
+        for (int k = l; k < WR; k += 64) {
+            uint16_t v;
+            if (k < RR) {
+                const int r = k / R, cc = k - r * R;
+                v = img[(size_t)(y - half + r) * c.W + (x - half + cc)];
+            } else {
+                const int h = k - RR;
+                v = h == 0 ? (uint16_t)x
+                  : h == 1 ? (uint16_t)(y % c.H)
+                  : h == 2 ? (uint16_t)(fr % 65536)
+                  : h == 3 ? (uint16_t)(fr / 65536)
+                  : h == 4 ? (uint16_t)ft : (uint16_t)0;
+            }
+            o[k] = v;
+        }
     }
-    const uint16_t *img = frames + rbase * c.W;
-    uint16_t *o = rois + (size_t)i * WR;
-    for (int r = 0; r < R; r++)
-        for (int cc = 0; cc < R; cc++) o[r * R + cc] = img[(size_t)(y - half + r) * c.W + (x - half + cc)];
-    const int fr = c.start_frame + b * c.B + y / c.H;
-    o[R * R + 0] = (uint16_t)x;
-    o[R * R + 1] = (uint16_t)(y % c.H);
-    o[R * R + 2] = (uint16_t)(fr % 65536);
-    o[R * R + 3] = (uint16_t)(fr / 65536);
-    o[R * R + 4] = (uint16_t)ft;
-    for (int k = 5; k < R; k++) o[R * R + k] = 0;
 }
 
 __global__ void fill_kernel(int *__restrict__ p, size_t n, int v)
@@ -558,14 +686,23 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
     c.batch_cap = p.MaxROIPerBatch > 0 ? p.MaxROIPerBatch : QCS_MAX_POINT_NUM;
     c.cap = cap;
     c.start_frame = start_frame;
-    c.nseg = (c.W + 63) / 64;
+    c.nseg = (c.W + SEGW - 1) / SEGW;
     filter_taps_host(R, p.LocType, p.MultiEmitterFitEn, c.taps);
+    // rows per walking wave: long walks amortise the 14-row pipeline fill; shorter ones keep
+    // small calls spread over the chip
+    const int rows_per_batch = c.B * c.H;
+    c.walk_rows = 128;
+    while (c.walk_rows > 16 &&
+           (long long)c.nseg * ((rows_per_batch + c.walk_rows - 1) / c.walk_rows) * c.nbatch < 8192)
+        c.walk_rows /= 2;
 
     const size_t nrows = (size_t)nframes * c.H;
     if (nrows > (size_t)SCAN_TILE * SCAN_TILE) return fail(QCS_ERR_ARG, "too many rows in one extraction call");
+    if ((size_t)rows_per_batch * c.W >= ((size_t)1 << 31)) return fail(QCS_ERR_ARG, "batched image larger than 2^31 pixels");
     const int ntiles = (int)((nrows + SCAN_TILE - 1) / SCAN_TILE);
     int rc;
     if ((rc = ctx->ex_var.ensure(sizeof(unsigned long long) * 2 * c.nbatch)) ||
+        (rc = ctx->ex_det.ensure(sizeof(uint16_t) * VAR_MAX * VAR_MAX * (size_t)c.nbatch)) ||
         (rc = ctx->ex_mark.ensure(sizeof(unsigned long long) * nrows * c.nseg * 2)) ||
         (rc = ctx->ex_cnt.ensure(sizeof(int) * (nrows * 2 + ntiles + 1))) ||
         (rc = ctx->ex_xy.ensure(sizeof(int3) * (size_t)(cap > 0 ? cap : 1))) ||
@@ -573,6 +710,7 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
         (rc = ctx->pin_cnt.ensure(64)))
         return rc;
     unsigned long long *var_sums = ctx->ex_var.as<unsigned long long>();
+    uint16_t *tvbuf = ctx->ex_det.as<uint16_t>();
     unsigned long long *cand = ctx->ex_mark.as<unsigned long long>();
     unsigned long long *mark = cand + nrows * c.nseg;
     int *row_count = ctx->ex_cnt.as<int>();
@@ -581,15 +719,28 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
     int *info = ctx->ex_tmp.as<int>();
     int3 *xyb = ctx->ex_xy.as<int3>();
 
+    // variance region extent (largest over the batches: the first batch is a full one)
+    int Smax;
+    {
+        int sw = c.W / 2 - 5, sh = rows_per_batch / 2 - 5;
+        sw = sw > 256 ? 256 : (sw < 1 ? 1 : sw);
+        sh = sh > 256 ? 256 : (sh < 1 ? 1 : sh);
+        Smax = sw < sh ? sw : sh;
+    }
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
     fill_async(reinterpret_cast<int *>(var_sums), 4 * (size_t)c.nbatch, 0, s);
     fill_async(row_count, nrows, 0, s);
-    hipLaunchKernelGGL(var_kernel, dim3(64, c.nbatch), dim3(256), 0, s, d_frames, c, var_sums);
+    hipLaunchKernelGGL((walk_kernel<0, false>), dim3((2 * Smax + SEGW - 1) / SEGW, (2 * Smax + c.walk_rows - 1) / c.walk_rows, c.nbatch),
+                       dim3(64), 0, s, d_frames, c, var_sums, tvbuf, cand, mark, row_count);
     QCS_HIP(hipGetLastError());
-    const int rows_per_batch = c.B * c.H;
-    hipLaunchKernelGGL(detect_kernel, dim3(c.nseg, (rows_per_batch + TH - 1) / TH, c.nbatch), dim3(256), 0, s, d_frames, c,
-                       var_sums, cand, mark, row_count);
+    hipLaunchKernelGGL(var_reduce_kernel, dim3(16, c.nbatch), dim3(256), 0, s, c, tvbuf, var_sums);
+    QCS_HIP(hipGetLastError());
+    const dim3 wgrid(c.nseg, (rows_per_batch + c.walk_rows - 1) / c.walk_rows, c.nbatch);
+    if (c.smooth)
+        hipLaunchKernelGGL((walk_kernel<1, true>), wgrid, dim3(64), 0, s, d_frames, c, var_sums, tvbuf, cand, mark, row_count);
+    else
+        hipLaunchKernelGGL((walk_kernel<1, false>), wgrid, dim3(64), 0, s, d_frames, c, var_sums, tvbuf, cand, mark, row_count);
     QCS_HIP(hipGetLastError());
     hipLaunchKernelGGL(scan_tiles_kernel, dim3(ntiles), dim3(SCAN_BLOCK), 0, s, row_count, (int)nrows, row_off, tile_sums);
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(SCAN_BLOCK), 0, s, tile_sums, ntiles);
@@ -599,9 +750,10 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
                        (int)nrows, info, xyb);
     QCS_HIP(hipGetLastError());
     if (cap > 0) {
-        hipLaunchKernelGGL(cut_kernel, dim3((cap + 255) / 256), dim3(256), 0, s, d_frames, c, mark, info, xyb, d_rois);
+        const int cut_blocks = cap < 16384 ? cap : 16384;
+        hipLaunchKernelGGL(cut_kernel, dim3(cut_blocks), dim3(64), 0, s, d_frames, c, mark, info, xyb, d_rois);
         QCS_HIP(hipGetLastError());
-        rc = wle_launch_devn(p, d_rois, info, cap, d_wle, s);
+        rc = wle_launch_extract(p, d_rois, xyb, c.B, c.H, info, cap, d_wle, s);
         if (rc) return rc;
     }
     if ((rc = timer_end(ctx->timer, TM_EXTRACT, s, ev, nframes))) return rc;

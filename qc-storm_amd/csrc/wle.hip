@@ -231,6 +231,36 @@ __global__ void __launch_bounds__(256) wle_nn_sorted_kernel(const uint16_t *__re
     wle[(size_t)i * QCS_WLE_FLOATS + QCS_WLE_NEARDIST] = sqrtf(best);
 }
 
+// Same sweep over the extraction's (x, y-in-batch, batch) list, which is in raster order: the
+// frame of an entry is batch * B + y / H, and two entries share a frame when they share the
+// batch and the frame's row range.
+__global__ void __launch_bounds__(256) wle_nn_xyb_kernel(const int3 *__restrict__ xyb, const int *__restrict__ n_dev,
+                                                         int B, int H, float *__restrict__ wle)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = *n_dev;
+    if (i >= n) return;
+    const int3 a = xyb[i];
+    const int fy0 = a.y - a.y % H;   // first batch row of a's frame
+    float best = 100000000.0f;
+    for (int j = i + 1; j < n; j++) {
+        const int3 q = xyb[j];
+        const int dy = q.y - a.y;
+        if (q.z != a.z || q.y - fy0 >= H || (float)(dy * dy) > best) break;
+        const int dx = q.x - a.x;
+        best = fminf(best, (float)(dx * dx + dy * dy));
+    }
+    for (int j = i - 1; j >= 0; j--) {
+        const int3 q = xyb[j];
+        const int dy = a.y - q.y;
+        if (q.z != a.z || q.y < fy0 || (float)(dy * dy) > best) break;
+        const int dx = q.x - a.x;
+        best = fminf(best, (float)(dx * dx + dy * dy));
+    }
+    wle[(size_t)i * QCS_WLE_FLOATS + QCS_WLE_NEARDIST] = sqrtf(best);
+    (void)B;
+}
+
 __global__ void fill_flag_kernel(int *flag) { *flag = 0; }
 
 // 1 in *flag when the list is not sorted by (frame, y)
@@ -266,9 +296,10 @@ __global__ void __launch_bounds__(256) wle_classify_kernel(const uint16_t *__res
 }
 
 // n ROIs (host count) or, when n_dev is set, *n_dev ROIs with grids sized for n.
-// sorted: 1 = list sorted by (frame, y); 0 = unknown order (all-pairs scan).
+// sorted: 2 = extraction list (raster order, (x, y, batch) entries in d_xyb, count in n_dev);
+// 1 = list sorted by (frame, y); 0 = unknown order (all-pairs scan).
 static int wle_launch_impl(const qcs_params &p, const uint16_t *d_rois, int n, const int *n_dev, float *d_wle, int sorted,
-                           hipStream_t s)
+                           hipStream_t s, const int3 *d_xyb = nullptr, int B = 1, int H = 1)
 {
     if (n <= 0) return QCS_OK;
     const int R = p.ROISize;
@@ -284,7 +315,9 @@ static int wle_launch_impl(const qcs_params &p, const uint16_t *d_rois, int n, c
     default: return fail(QCS_ERR_ARG, "ROISize %d not supported", R);
     }
     QCS_HIP(hipGetLastError());
-    if (sorted)
+    if (sorted == 2)
+        hipLaunchKernelGGL(wle_nn_xyb_kernel, g256, dim3(256), 0, s, d_xyb, n_dev, B, H, d_wle);
+    else if (sorted)
         hipLaunchKernelGGL(wle_nn_sorted_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, d_wle);
     else
         hipLaunchKernelGGL(wle_nn_allpairs_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, d_wle);
@@ -294,10 +327,11 @@ static int wle_launch_impl(const qcs_params &p, const uint16_t *d_rois, int n, c
     return QCS_OK;
 }
 
-// extraction output: raster order, count on the device
-int wle_launch_devn(const qcs_params &p, const uint16_t *d_rois, const int *d_n, int cap, float *d_wle, hipStream_t s)
+// extraction output: raster order, count on the device, (x, y, batch) list alongside
+int wle_launch_extract(const qcs_params &p, const uint16_t *d_rois, const int3 *d_xyb, int B, int H, const int *d_n, int cap,
+                       float *d_wle, hipStream_t s)
 {
-    return wle_launch_impl(p, d_rois, cap, d_n, d_wle, 1, s);
+    return wle_launch_impl(p, d_rois, cap, d_n, d_wle, 2, s, d_xyb, B, H);
 }
 
 // arbitrary ROI array: check the order on the device first

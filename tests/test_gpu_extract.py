@@ -15,6 +15,8 @@ CASES = [
     (200, 150, 3, 1, 1, 0, 9, 0.003, 100000),    # AS3D, width not a multiple of 32/64
     (320, 256, 2, 1, 0, 1, 11, 0.03, 100000),
     (256, 256, 2, 1, 0, 0, 7, 0.01, 50),         # capacity overflow: first 50 in raster order
+    (2048, 2048, 16, 2, 0, 0, 7, 0.003, 400000), # full-size frames: 128-row walks, 37 strips, seams
+    (1000, 700, 3, 3, 0, 1, 9, 0.01, 100000),    # W not a multiple of 56 or 32, 3-frame batch
 ]
 
 

@@ -98,3 +98,33 @@ def test_extract_finds_simulated_emitters():
     d = np.hypot(cx[:, None] - tr[None, :, 1], cy[:, None] - tr[None, :, 2]).min(1)
     # every detection sits on a simulated emitter, and most bright emitters are detected
     assert found == len(rois) and np.all(d < 1.5) and len(rois) >= 0.8 * inside.sum()
+
+
+def test_integer_background_form_is_exact():
+    """The GPU walker evaluates the background-removed value as max(raw - ceil(S / 24), 0), S the
+    integer sum of the three background sums; it must equal the reference's float expression
+    trunc(max(raw - ((b0/8 + b1/8) + b2/8) / 3, 0)) for every S and raw (checked exhaustively over
+    S at the raw values where the result changes, and on random triples)."""
+    f32 = np.float32
+    S = np.arange(0, 3 * 8 * 65535 + 1, dtype=np.int64)
+    b0 = S // 3
+    b1 = S // 3
+    b2 = S - 2 * (S // 3)
+
+    def float_form(raw, b0, b1, b2):
+        s = (b0.astype(f32) / f32(8) + b1.astype(f32) / f32(8)) + b2.astype(f32) / f32(8)
+        mb = s / f32(3)
+        v = np.maximum(raw.astype(f32) - mb, f32(0))
+        return v.astype(np.int64)
+
+    for d in (-1, 0, 1, 2):
+        raw = np.clip(S // 24 + d, 0, 65535)
+        ref = float_form(raw, b0, b1, b2)
+        got = np.maximum(raw - (b0 + b1 + b2 + 23) // 24, 0)
+        assert np.array_equal(ref, got)
+    rng = np.random.default_rng(5)
+    bb = rng.integers(0, 8 * 65536, (3, 2_000_000))
+    raw = rng.integers(0, 65536, 2_000_000)
+    ref = float_form(raw, bb[0], bb[1], bb[2])
+    got = np.maximum(raw - (bb.sum(0) + 23) // 24, 0)
+    assert np.array_equal(ref, got)
