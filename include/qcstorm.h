@@ -200,6 +200,8 @@ int qcs_fit_stats_get(qcs_ctx *ctx, qcs_fit_stats *out);
 typedef struct qcs_timing {
     double extract_ms, fit_1e_ms, fit_me_ms, crlb_ms, compact_ms, render_ms;
     long long fit_1e_launches, fit_1e_rois, fit_me_launches;
+    double cfit_ms, nn_ms;               /* converged fitting (qcs_cfit_dev, qcs_fit_mle_dev, warm path), network */
+    long long cfit_launches, cfit_rois, nn_launches;
 } qcs_timing;
 int qcs_timing_enable(qcs_ctx *ctx, int on);
 int qcs_timing_get(qcs_ctx *ctx, qcs_timing *out);
@@ -220,6 +222,30 @@ int qcs_compact_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_out, int *
  * d_zimg (3D z-weighted plane, may be NULL for 2D) -- FluoRenderWithPrec_2D/3D
  * (ImageRender_2d.cu:55-155, ImageRender_3d.cu:60-147). */
 int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream);
+
+/* ========================================================================================
+ * Converged multi-emitter fitting (no counterpart in the reference, whose fitters run a fixed
+ * iteration schedule): the 2D model mu = sum_k N_k/(2 pi s^2) exp(-d_k^2/(2 s^2)) + b with K = 1..3
+ * emitters sharing s, Poisson MLE by BFGS with analytic gradient run to convergence (step below
+ * 1e-4 in scaled units, at most 100 iterations).  ROISize 7, 9 or 11, LocType 0.
+ * ====================================================================================== */
+#define QCS_CFIT_INIT_HEUR  0   /* centroid / edge-background heuristic */
+#define QCS_CFIT_INIT_GIVEN 1   /* d_init holds n x (3K+2) starting vectors */
+#define QCS_CFIT_INIT_ADD   2   /* d_init holds the (K-1)-emitter fits; add one at the largest residual */
+#define QCS_CFIT_CONVERGED  1   /* info[1] flags */
+#define QCS_CFIT_MAXITER    2
+#define QCS_CFIT_STALLED    4
+/* Fit model K to every ROI.  theta (n x (3K+2)): [N_k photons, x_k, y_k px (pixel edges)]_k, s px,
+ * b photons/px.  info (n x 4, may be NULL): iterations, flags, loss evaluations, 0.  ll (n, may be
+ * NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
+int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, int init_mode, const float *d_init,
+                 float *d_theta, int *d_info, double *d_ll, void *stream);
+/* The plain multi-emitter path: fits K = 1, 2, 3 in sequence (each started from the previous
+ * one plus an emitter at the largest residual), picks the emitter count by the FP64
+ * log-likelihood ratio (chi^2(3) at p = 1e-3) among valid models (every emitter >= 50 photons
+ * inside the ROI), and writes 3 rows per ROI (rows 3j..3j+2, unused rows zero) with the FP64
+ * full-Fisher CRLB in CRLBx/CRLBy.  d_sel (n, may be NULL) receives the chosen count. */
+int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, void *stream);
 
 /* ========================================================================================
  * Host-pointer stage API.  Same semantics as the *_dev calls, with the H2D / D2H copies the

@@ -223,7 +223,8 @@ int qcs_ctx_destroy(qcs_ctx *c)
     if (!c) return QCS_OK;
     qcs::DevBuf *bufs[] = {&c->ex_bkg, &c->ex_tmp, &c->ex_det, &c->ex_mark, &c->ex_cnt, &c->ex_xy, &c->ex_var,
                            &c->ex_frames, &c->fit_idx1, &c->fit_idx2, &c->fit_idx3, &c->fit_flag, &c->fit_cnt,
-                           &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev};
+                           &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev, &c->cf_theta, &c->cf_ll,
+                           &c->cf_aux};
     for (auto *b : bufs) b->release();
     c->pin_cnt.release();
     (void)timer_resolve(c->timer);
@@ -263,6 +264,11 @@ int qcs_timing_get(qcs_ctx *c, qcs_timing *out)
     out->fit_1e_launches = t.launches[TM_FIT1E];
     out->fit_1e_rois = t.items[TM_FIT1E];
     out->fit_me_launches = t.launches[TM_FITME];
+    out->cfit_ms = t.ms[TM_CFIT];
+    out->nn_ms = t.ms[TM_NN];
+    out->cfit_launches = t.launches[TM_CFIT];
+    out->cfit_rois = t.items[TM_CFIT];
+    out->nn_launches = t.launches[TM_NN];
     return QCS_OK;
 }
 

@@ -1,0 +1,766 @@
+// cfit.hip -- converged multi-emitter Poisson-MLE fitting with FP64 model selection and FP64
+// full-Fisher CRLB on gfx950.
+//
+// This is the "plain" path of the batched multi-emitter mode and the labeller of the learned
+// warm start; it has no counterpart in the reference (whose fitters run a fixed iteration
+// schedule, bfgsMLE_GS2D_core.h:313-426).  Model, pixel (r, c) centred at (c + 0.5, r + 0.5):
+//
+//   mu = sum_k N_k / (2 pi s^2) exp(-((c + .5 - x_k)^2 + (r + .5 - y_k)^2) / (2 s^2)) + b
+//
+// with K = 1..3 emitters sharing the PSF width s.  theta = [N_k, x_k, y_k]_k, s, b in photons and
+// pixels; the optimiser works on u = theta / scale (N / 1000, b / 100) so the identity is a
+// sensible first inverse Hessian.  Loss: Poisson negative log-likelihood sum(mu - q ln mu) with
+// q = max((ADU - Offset) * KAdc, 1) / QE, analytic gradient, BFGS with Armijo backtracking and
+// box projection, run to convergence (max step < tol) or max_iter.
+//
+// One ROI per lane; the observed image sits in LDS (lane-interleaved), as does the packed
+// symmetric inverse Hessian, leaving VGPRs for the separable model.  The starting inverse Hessian
+// is the inverse Fisher information at the initial point, which cuts the iterations about 3x
+// against the identity and makes the quasi-Newton step a reliable distance to the optimum: a fit
+// has converged when that full step is below tol in every scaled coordinate (or below 20 tol
+// where the FP32 loss can no longer resolve it -- the step is then taken and the fit ends); a
+// line search that finds no decrease in 16 backtracks ends the fit as stalled.
+#include <math.h>
+#include <string.h>
+#include "qcs_internal.h"
+
+namespace qcs {
+
+struct CFitConst {
+    float offset, kadc, inv_qe;
+    int max_iter;
+    float tol;          // convergence: max |step| in scaled units
+    int init_mode;      // CF_INIT_*
+    float sigma0;       // heuristic initial PSF width (px)
+};
+enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
+enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4 };
+
+template <int K> struct CP { static constexpr int P = 3 * K + 2, NS = P * (P + 1) / 2; };
+
+__device__ __forceinline__ float cf_scale(int i, int P)
+{
+    // N -> kilo-photons, b -> hundreds of photons, positions and width in pixels
+    return (i == P - 1) ? 100.0f : ((i < P - 2 && i % 3 == 0) ? 1000.0f : 1.0f);
+}
+
+// packed upper-triangular index of (i, j), i <= j
+__device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
+
+// loss sum((mu - q) - q ln(mu / q)) (the Poisson negative log-likelihood up to a constant) and,
+// if G, its gradient with respect to theta (physical units)
+template <int K, int R, bool G>
+__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<K>::P], float (&g)[CP<K>::P])
+{
+    constexpr int P = CP<K>::P;
+    const float s = th[3 * K], b = th[3 * K + 1];
+    const float is2 = 1.0f / (s * s);
+    const float h = -0.5f * is2;
+    const float nrm = 0.15915494309189535f * is2;   // 1 / (2 pi s^2)
+    float ex[K][R], dx[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            dx[k][c] = ((float)c + 0.5f) - th[3 * k + 1];
+            ex[k][c] = __expf(h * dx[k][c] * dx[k][c]);
+        }
+    float L = 0.0f;
+    float gN[K], gx[K], gy[K], gd2 = 0.0f, gw = 0.0f, gb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        float ay[K], dy[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
+            ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
+        }
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            float gk[K];
+            float mu = b;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                gk[k] = ay[k] * ex[k][c];
+                mu += gk[k];
+            }
+            mu = fmaxf(mu, 1e-6f);
+            const float q = qs[(r * R + c) * 64];
+            // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
+            L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
+            if (G) {
+                const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
+                gb += w;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const float wg = w * gk[k];
+                    gN[k] += wg;
+                    gx[k] = fmaf(wg, dx[k][c], gx[k]);
+                    gy[k] = fmaf(wg, dy[k], gy[k]);
+                    gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
+                    gw += wg;
+                }
+            }
+        }
+    }
+    if (G) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            g[3 * k] = gN[k] / fmaxf(th[3 * k], 1e-3f);
+            g[3 * k + 1] = gx[k] * is2;
+            g[3 * k + 2] = gy[k] * is2;
+        }
+        g[3 * K] = (gd2 * is2 - 2.0f * gw) / s;
+        g[3 * K + 1] = gb;
+        (void)P;
+    }
+    return L;
+}
+
+template <int K, int R>
+__device__ __forceinline__ void cf_project(float (&th)[CP<K>::P])
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        th[3 * k] = fmaxf(th[3 * k], 1.0f);
+        th[3 * k + 1] = fminf(fmaxf(th[3 * k + 1], -0.5f), (float)R + 0.5f);
+        th[3 * k + 2] = fminf(fmaxf(th[3 * k + 2], -0.5f), (float)R + 0.5f);
+    }
+    th[3 * K] = fminf(fmaxf(th[3 * K], 0.4f), 0.5f * (float)R);
+    th[3 * K + 1] = fmaxf(th[3 * K + 1], 0.01f);
+}
+
+// Inverse of the Fisher information sum grad(mu) grad(mu)^T / mu at theta, in scaled units,
+// into the packed symmetric D (the BFGS starting inverse Hessian).  Returns false when the
+// Cholesky factorisation fails (D untouched).
+template <int K, int R>
+__device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<K>::P], float *D)
+{
+    constexpr int P = CP<K>::P, NS = CP<K>::NS;
+    float F[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) F[i] = 0.0f;
+    const float s = th[3 * K], is2 = 1.0f / (s * s), h = -0.5f * is2, nrm = 0.15915494309189535f * is2;
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            float J[P];
+            float mu = th[3 * K + 1], jw = 0.0f;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const float dx = ((float)c + 0.5f) - th[3 * k + 1], dy = ((float)r + 0.5f) - th[3 * k + 2];
+                const float d2 = fmaf(dx, dx, dy * dy);
+                const float mk = th[3 * k] * nrm * __expf(h * d2);
+                mu += mk;
+                J[3 * k] = mk / fmaxf(th[3 * k], 1e-3f) * cf_scale(3 * k, P);
+                J[3 * k + 1] = mk * dx * is2;
+                J[3 * k + 2] = mk * dy * is2;
+                jw = fmaf(mk, fmaf(d2, is2, -2.0f), jw);
+            }
+            J[3 * K] = jw / s;
+            J[3 * K + 1] = cf_scale(3 * K + 1, P);
+            const float im = __builtin_amdgcn_rcpf(fmaxf(mu, 1e-6f));
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                const float ja = J[a] * im;
+#pragma unroll
+                for (int b2 = a; b2 < P; b2++) F[tri(a, b2, P)] = fmaf(ja, J[b2], F[tri(a, b2, P)]);
+            }
+        }
+    }
+    // F = U^T U (U upper, in place)
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        float d = F[tri(j, j, P)];
+#pragma unroll
+        for (int k = 0; k < j; k++) d = fmaf(-F[tri(k, j, P)], F[tri(k, j, P)], d);
+        if (!(d > 0.0f)) return false;
+        const float u = sqrtf(d), iu = 1.0f / u;
+        F[tri(j, j, P)] = u;
+#pragma unroll
+        for (int i = j + 1; i < P; i++) {
+            float v = F[tri(j, i, P)];
+#pragma unroll
+            for (int k = 0; k < j; k++) v = fmaf(-F[tri(k, j, P)], F[tri(k, i, P)], v);
+            F[tri(j, i, P)] = v * iu;
+        }
+    }
+    // U^-1 in place, rows bottom-up, each row right to left
+#pragma unroll
+    for (int i = P - 1; i >= 0; i--) {
+        const float iu = 1.0f / F[tri(i, i, P)];
+#pragma unroll
+        for (int j = P - 1; j > i; j--) {
+            float v = 0.0f;
+#pragma unroll
+            for (int k = i + 1; k <= j; k++) v = fmaf(F[tri(i, k, P)], F[tri(k, j, P)], v);
+            F[tri(i, j, P)] = -v * iu;
+        }
+        F[tri(i, i, P)] = iu;
+    }
+    // F^-1 = U^-1 U^-T, in place row by row (entries (i, j), j >= i, use row i and row j from k >= j)
+#pragma unroll
+    for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int j = i; j < P; j++) {
+            float v = 0.0f;
+#pragma unroll
+            for (int k = j; k < P; k++) v = fmaf(F[tri(i, k, P)], F[tri(j, k, P)], v);
+            F[tri(i, j, P)] = v;
+        }
+#pragma unroll
+    for (int i = 0; i < NS; i++) D[i * 64] = F[i];
+    return true;
+}
+
+// FP64 log-likelihood sum(q ln mu - mu) (without the ln q! term, common to every model)
+template <int K, int R>
+__device__ double cf_loglik64(const float *qs, const float (&th)[CP<K>::P])
+{
+    const double s = th[3 * K], b = th[3 * K + 1];
+    const double h = -0.5 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+    double ll = 0.0;
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            double mu = b;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
+                mu += (double)th[3 * k] * nrm * exp(h * (dx * dx + dy * dy));
+            }
+            const double q = qs[(r * R + c) * 64];
+            ll += q * log(mu) - mu;
+        }
+    return ll;
+}
+
+// initial vector: heuristic (K = 1), given, or the previous model plus an emitter at the largest
+// positive residual (K >= 2; init holds the K-1 model's theta)
+template <int K, int R>
+__device__ __forceinline__ void cf_init(const float *qs, const float *init, int mode, float sigma0, float (&th)[CP<K>::P])
+{
+    constexpr int P = CP<K>::P;
+    if (mode == CF_INIT_GIVEN) {
+#pragma unroll
+        for (int i = 0; i < P; i++) th[i] = init[i];
+        return;
+    }
+    if constexpr (K > 1) {
+      if (mode == CF_INIT_ADD) {
+        constexpr int P0 = CP<K - 1>::P;
+        float t0[P0];
+#pragma unroll
+        for (int i = 0; i < P0; i++) t0[i] = init[i];
+        const float s = t0[3 * (K - 1)], b = t0[3 * (K - 1) + 1];
+        const float h = -0.5f / (s * s), nrm = 0.15915494309189535f / (s * s);
+        float best = -1e30f;
+        int bi = (R / 2) * R + R / 2;
+        for (int r = 0; r < R; r++)
+            for (int c = 0; c < R; c++) {
+                float mu = b;
+#pragma unroll
+                for (int k = 0; k < K - 1; k++) {
+                    const float dx = ((float)c + 0.5f) - t0[3 * k + 1], dy = ((float)r + 0.5f) - t0[3 * k + 2];
+                    mu += t0[3 * k] * nrm * __expf(h * (dx * dx + dy * dy));
+                }
+                const float res = qs[(r * R + c) * 64] - mu;
+                if (res > best) { best = res; bi = r * R + c; }
+            }
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) {
+            th[3 * k] = t0[3 * k];
+            th[3 * k + 1] = t0[3 * k + 1];
+            th[3 * k + 2] = t0[3 * k + 2];
+        }
+        th[3 * (K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * s * s;
+        th[3 * (K - 1) + 1] = (float)(bi % R) + 0.5f;
+        th[3 * (K - 1) + 2] = (float)(bi / R) + 0.5f;
+        th[3 * K] = s;
+        th[3 * K + 1] = b;
+        return;
+      }
+    }
+    // heuristic: background from the smaller of the opposite edge means, photons from the excess,
+    // position at the excess-weighted centroid (single emitter; extra emitters share it)
+    float e[4] = {0, 0, 0, 0};
+    for (int i = 0; i < R; i++) {
+        e[0] += qs[(0 * R + i) * 64];
+        e[1] += qs[((R - 1) * R + i) * 64];
+        e[2] += qs[(i * R + 0) * 64];
+        e[3] += qs[(i * R + R - 1) * 64];
+    }
+    const float b = 0.5f * (fminf(e[0], e[1]) + fminf(e[2], e[3])) / (float)R;
+    float tot = 0.0f, sx = 0.0f, sy = 0.0f;
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            const float v = fmaxf(qs[(r * R + c) * 64] - b, 0.0f);
+            tot += v;
+            sx = fmaf(v, (float)c + 0.5f, sx);
+            sy = fmaf(v, (float)r + 0.5f, sy);
+        }
+    const float cx = tot > 0.0f ? sx / tot : 0.5f * R, cy = tot > 0.0f ? sy / tot : 0.5f * R;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        th[3 * k] = fmaxf(tot, 50.0f) / (float)K;
+        th[3 * k + 1] = cx + 0.3f * (float)k;
+        th[3 * k + 2] = cy;
+    }
+    th[3 * K] = sigma0;
+    th[3 * K + 1] = fmaxf(b, 0.5f);
+}
+
+// theta_out: n x P (physical units); info: n x 4 [iterations, flags, loss evaluations, 0];
+// ll: FP64 log-likelihood.  idx (optional) maps lane work item j -> ROI index; the count is
+// n or *n_dev.  init is indexed like theta_out (by j).
+template <int K, int R>
+__global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
+                                                  const int *__restrict__ n_dev, const float *__restrict__ init,
+                                                  float *__restrict__ theta_out, int *__restrict__ info,
+                                                  double *__restrict__ ll_out, CFitConst c)
+{
+    constexpr int P = CP<K>::P, NS = CP<K>::NS, RR = R * R, WR = R * (R + 1);
+    __shared__ float qsh[RR * 64];
+    __shared__ float Dsh[NS * 64];
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x * 64 + lane;
+    if (n_dev) n = *n_dev;
+    if ((int)(blockIdx.x * 64) >= n) return;   // whole wave beyond the count
+    const bool valid = j < n;
+    const int roi = valid ? (idx ? idx[j] : j) : 0;
+    float *qs = qsh + lane;
+    float *D = Dsh + lane;
+    {
+        const uint16_t *src = rois + (size_t)roi * WR;
+        for (int k = 0; k < RR; k++) {
+            const float v = valid ? ((float)src[k] - c.offset) * c.kadc : 1.0f;
+            qs[k * 64] = fmaxf(v, 1.0f) * c.inv_qe;
+        }
+    }
+    float th[P], g[P];
+    float zero[P + 1];
+#pragma unroll
+    for (int i = 0; i <= P; i++) zero[i] = 0.0f;
+    cf_init<K, R>(qs, (valid && init) ? init + (size_t)j * (c.init_mode == CF_INIT_ADD ? CP<(K > 1 ? K - 1 : 1)>::P : P) : zero,
+                  valid ? c.init_mode : CF_INIT_HEUR, c.sigma0, th);
+    cf_project<K, R>(th);
+    float L = cf_lossgrad<K, R, true>(qs, th, g);
+    // starting inverse Hessian: the inverse Fisher information at the initial point (identity if
+    // it is not positive definite; the first step is then normalised)
+    bool first = !cf_fisher_inverse<K, R>(th, D);
+    if (first) {
+#pragma unroll
+        for (int i = 0; i < P; i++)
+#pragma unroll
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+    }
+    int iters = 0, flags = 0, evals = 1;
+    bool done = !valid;
+    while (true) {
+        if (__all(done)) break;   // wave-uniform
+        // direction in scaled units: du = -D gu, gu = g * scale
+        float gu[P], du[P];
+#pragma unroll
+        for (int i = 0; i < P; i++) gu[i] = g[i] * cf_scale(i, P);
+        float slope = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
+            du[i] = -a;
+            slope = fmaf(du[i], gu[i], slope);
+        }
+        if (!done) {
+            float dmax = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
+            if (dmax < c.tol && slope < 0.0f) {   // the full quasi-Newton step is below tolerance
+                flags |= CF_CONVERGED;
+                done = true;
+            }
+        }
+        if (!(slope < 0.0f)) {   // not a descent direction: restart from the gradient
+#pragma unroll
+            for (int i = 0; i < P; i++) du[i] = -gu[i];
+            slope = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) slope = fmaf(du[i], gu[i], slope);
+#pragma unroll
+            for (int i = 0; i < P; i++)
+#pragma unroll
+                for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+            first = true;
+        }
+        if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
+            float m = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) m = fmaxf(m, fabsf(du[i]));
+            if (m > 0.1f) {
+                const float f = 0.1f / m;
+                slope = 0.0f;
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    du[i] *= f;
+                    slope = fmaf(du[i], gu[i], slope);
+                }
+            }
+        }
+        float dmax = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
+        // Armijo backtracking (every active lane evaluates each trial together)
+        float alpha = 1.0f, tn[P], Ln = L;
+        bool floor_step = false;
+#pragma unroll
+        for (int i = 0; i < P; i++) tn[i] = th[i];
+        bool accepted = done;
+        for (int t = 0; t < 16; t++) {
+            if (__all(accepted)) break;
+            if (!accepted) {
+#pragma unroll
+                for (int i = 0; i < P; i++) tn[i] = fmaf(alpha, du[i] * cf_scale(i, P), th[i]);
+                cf_project<K, R>(tn);
+            }
+            float gdum[P];
+            const float Lt = cf_lossgrad<K, R, false>(qs, tn, gdum);
+            if (!accepted) {
+                evals++;
+                if (Lt == Lt && Lt < L && Lt <= fmaf(1e-4f * alpha, slope, L)) {
+                    accepted = true;
+                    Ln = Lt;
+                } else if (t == 0 && !(Lt < L) && dmax < 20.0f * c.tol) {
+                    // the full step is tiny and the FP32 loss cannot resolve it: take it and stop
+                    accepted = true;
+                    floor_step = true;
+                    Ln = Lt;
+                } else {
+                    alpha *= 0.3f;
+                }
+            }
+        }
+        if (!done && floor_step) {
+            iters++;
+#pragma unroll
+            for (int i = 0; i < P; i++) th[i] = tn[i];
+            flags |= CF_CONVERGED;
+            done = true;
+        }
+        if (!done) {
+            iters++;
+            bool moved = accepted && Ln < L;
+            float s[P];
+#pragma unroll
+            for (int i = 0; i < P; i++) s[i] = moved ? (tn[i] - th[i]) / cf_scale(i, P) : 0.0f;
+            if (!moved) {
+                flags |= CF_STALLED;
+                done = true;
+            } else {
+                float gnew[P];
+#pragma unroll
+                for (int i = 0; i < P; i++) th[i] = tn[i];
+                L = cf_lossgrad<K, R, true>(qs, th, gnew);
+                evals++;
+                float y[P], sy = 0.0f, yy = 0.0f;
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    y[i] = (gnew[i] - g[i]) * cf_scale(i, P);
+                    sy = fmaf(s[i], y[i], sy);
+                    yy = fmaf(y[i], y[i], yy);
+                    g[i] = gnew[i];
+                }
+                if (sy > 1e-12f) {
+                    if (first) {
+                        const float sc = sy / yy;
+#pragma unroll
+                        for (int i = 0; i < P; i++)
+#pragma unroll
+                            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
+                        first = false;
+                    }
+                    // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
+                    const float rho = 1.0f / sy;
+                    float u[P], yu = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < P; i++) {
+                        float a = 0.0f;
+#pragma unroll
+                        for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
+                        u[i] = a;
+                        yu = fmaf(y[i], a, yu);
+                    }
+                    const float cs = fmaf(rho * rho, yu, rho);
+#pragma unroll
+                    for (int i = 0; i < P; i++)
+#pragma unroll
+                        for (int jj = i; jj < P; jj++) {
+                            float* e = &D[tri(i, jj, P) * 64];
+                            *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
+                        }
+                }
+                if (iters >= c.max_iter) {
+                    flags |= CF_MAXITER;
+                    done = true;
+                }
+            }
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int i = 0; i < P; i++) theta_out[(size_t)j * P + i] = th[i];
+    if (info) {
+        info[(size_t)j * 4 + 0] = iters;
+        info[(size_t)j * 4 + 1] = flags;
+        info[(size_t)j * 4 + 2] = evals;
+        info[(size_t)j * 4 + 3] = 0;
+    }
+    if (ll_out) ll_out[j] = cf_loglik64<K, R>(qs, th);
+}
+
+// ---- FP64 full-Fisher CRLB of one model (standard deviations of x_k and y_k, px) ----------
+// I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed in LDS per lane; Cholesky; then
+// (I^-1)_ii = |L^-1 e_i|^2 for the position indices.
+template <int K, int R>
+__device__ void cf_crlb64(const float (&th)[CP<K>::P], double *I, float (&cx)[K], float (&cy)[K])
+{
+    constexpr int P = CP<K>::P;
+    for (int i = 0; i < CP<K>::NS; i++) I[i * 64] = 0.0;
+    const double s = th[3 * K], is2 = 1.0 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            double J[P];
+            double mu = th[3 * K + 1];
+            double jw = 0.0;
+            for (int k = 0; k < K; k++) {
+                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
+                const double d2 = dx * dx + dy * dy;
+                const double mk = (double)th[3 * k] * nrm * exp(-0.5 * d2 * is2);
+                mu += mk;
+                J[3 * k] = mk / (double)th[3 * k];
+                J[3 * k + 1] = mk * dx * is2;
+                J[3 * k + 2] = mk * dy * is2;
+                jw += mk * (d2 * is2 - 2.0) / s;
+            }
+            J[3 * K] = jw;
+            J[3 * K + 1] = 1.0;
+            const double im = 1.0 / mu;
+            for (int a = 0; a < P; a++) {
+                const double ja = J[a] * im;
+                for (int b2 = a; b2 < P; b2++) I[tri(a, b2, P) * 64] += ja * J[b2];
+            }
+        }
+    // in-place Cholesky of the packed upper triangle: I = U^T U
+    bool ok = true;
+    for (int j = 0; j < P; j++) {
+        double d = I[tri(j, j, P) * 64];
+        for (int k = 0; k < j; k++) d -= I[tri(k, j, P) * 64] * I[tri(k, j, P) * 64];
+        if (!(d > 0.0)) { ok = false; break; }
+        const double u = sqrt(d);
+        I[tri(j, j, P) * 64] = u;
+        for (int i = j + 1; i < P; i++) {
+            double v = I[tri(j, i, P) * 64];
+            for (int k = 0; k < j; k++) v -= I[tri(k, j, P) * 64] * I[tri(k, i, P) * 64];
+            I[tri(j, i, P) * 64] = v / u;
+        }
+    }
+    for (int k = 0; k < K; k++) {
+        for (int w = 0; w < 2; w++) {
+            const int e = 3 * k + 1 + w;
+            double var = -1.0;
+            if (ok) {
+                // (I^-1)_ee = |U^-T e_e|^2: forward substitution U^T z = e_e
+                double z[P];
+                for (int i = 0; i < P; i++) {
+                    double v = (i == e) ? 1.0 : 0.0;
+                    for (int k2 = 0; k2 < i; k2++) v -= I[tri(k2, i, P) * 64] * z[k2];
+                    z[i] = v / I[tri(i, i, P) * 64];
+                }
+                var = 0.0;
+                for (int i = 0; i < P; i++) var += z[i] * z[i];
+            }
+            const float sd = var > 0.0 ? (float)sqrt(var) : -1.0f;
+            if (w == 0) cx[k] = sd; else cy[k] = sd;
+        }
+    }
+}
+
+struct CSelConst {
+    float llr_half;     // accept K+1 over K when ll(K+1) - ll(K) exceeds this (half the chi^2 bound)
+    float n_min;        // minimum photons of every emitter of an accepted model
+    float pixel_size, qe;
+    int kmax;
+};
+
+template <int K, int R>
+__device__ __forceinline__ bool cf_valid(const float *t, float n_min)
+{
+    for (int k = 0; k < K; k++) {
+        if (!(t[3 * k] >= n_min)) return false;
+        if (!(t[3 * k + 1] > 0.0f && t[3 * k + 1] < (float)R && t[3 * k + 2] > 0.0f && t[3 * k + 2] < (float)R)) return false;
+    }
+    return t[3 * K] > 0.45f && t[3 * K] < 0.45f * (float)R;
+}
+
+template <int K, int R>
+__device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, double *I, float *rows)
+{
+    float th[CP<K>::P];
+    for (int i = 0; i < CP<K>::P; i++) th[i] = t[i];
+    float cx[K], cy[K];
+    cf_crlb64<K, R>(th, I, cx, cy);
+    const float s = th[3 * K], b = th[3 * K + 1];
+    const int half = R / 2;
+    const float xc = (float)hdr[0], yc = (float)hdr[1];
+    const int fr = (int)hdr[2] + (int)hdr[3] * 65536;
+    for (int k = 0; k < K; k++) {
+        float *o = rows + (size_t)k * QCS_REC_FLOATS;
+        const float peak = th[3 * k] * 0.15915494309189535f / (s * s);
+        o[QCS_POS_PPHO] = peak;
+        o[QCS_POS_XPOS] = (th[3 * k + 1] - (float)half) + xc;
+        o[QCS_POS_YPOS] = (th[3 * k + 2] - (float)half) + yc;
+        o[QCS_POS_ZPOS] = 0.0f;
+        o[QCS_POS_SIGX] = s;
+        o[QCS_POS_SIGY] = s;
+        o[QCS_POS_TPHO] = th[3 * k];
+        o[QCS_POS_BAKG] = b;
+        o[QCS_POS_PSNR] = sqrtf(c.qe) * peak / sqrtf(peak + b);
+        o[QCS_POS_CRBX] = cx[k] > 0.0f ? cx[k] * c.pixel_size : -1.0f;
+        o[QCS_POS_CRBY] = cy[k] > 0.0f ? cy[k] * c.pixel_size : -1.0f;
+        o[QCS_POS_FRME] = (float)fr;
+    }
+}
+
+// Model selection (FP64 log-likelihood ratio) and rows: ROI j gets rows 3j .. 3j+2, the unused
+// ones zero.  sel[j] receives the chosen emitter count (0: no valid model).
+template <int R>
+__global__ void __launch_bounds__(64) cf_select_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
+                                                       const float *__restrict__ t1, const float *__restrict__ t2,
+                                                       const float *__restrict__ t3, const double *__restrict__ l1,
+                                                       const double *__restrict__ l2, const double *__restrict__ l3,
+                                                       CSelConst c, float *__restrict__ rows, int *__restrict__ sel)
+{
+    __shared__ double Ish[CP<3>::NS * 64];
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x * 64 + lane;
+    if (j >= n) return;
+    const int roi = idx ? idx[j] : j;
+    const uint16_t *hdr = rois + (size_t)roi * R * (R + 1) + R * R;
+    float *o = rows + (size_t)j * 3 * QCS_REC_FLOATS;
+    for (int i = 0; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
+    int K = 0;
+    if (cf_valid<1, R>(t1 + (size_t)j * 5, c.n_min)) K = 1;
+    if (c.kmax >= 2 && t2 && cf_valid<2, R>(t2 + (size_t)j * 8, c.n_min) && (K == 0 || l2[j] - l1[j] > (double)c.llr_half)) K = 2;
+    if (c.kmax >= 3 && t3 && K == 2 && cf_valid<3, R>(t3 + (size_t)j * 11, c.n_min) && l3[j] - l2[j] > (double)c.llr_half) K = 3;
+    if (sel) sel[j] = K;
+    double *I = Ish + lane;
+    if (K == 1) cf_write_rows<1, R>(t1 + (size_t)j * 5, hdr, c, I, o);
+    else if (K == 2) cf_write_rows<2, R>(t2 + (size_t)j * 8, hdr, c, I, o);
+    else if (K == 3) cf_write_rows<3, R>(t3 + (size_t)j * 11, hdr, c, I, o);
+}
+
+CFitConst cfit_const(const qcs_params &p, int init_mode)
+{
+    CFitConst c;
+    c.offset = p.Offset;
+    c.kadc = p.KAdc;
+    c.inv_qe = 1.0f / p.QE;
+    c.max_iter = 100;
+    c.tol = 1e-4f;
+    c.init_mode = init_mode;
+    c.sigma0 = 1.3f;
+    return c;
+}
+
+#define CF_DISPATCH_R(R_, ...)                                                            \
+    switch (R_) {                                                                          \
+    case 7: { constexpr int RT = 7; __VA_ARGS__; } break;                                  \
+    case 9: { constexpr int RT = 9; __VA_ARGS__; } break;                                  \
+    case 11: { constexpr int RT = 11; __VA_ARGS__; } break;                                \
+    default: return fail(QCS_ERR_ARG, "converged fitting supports ROISize 7, 9, 11 (got %d)", R_); \
+    }
+
+int cfit_launch(const qcs_params &p, int K, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n, int init_mode,
+                const float *d_init, float *d_theta, int *d_info, double *d_ll, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    if (p.LocType != QCS_LOCTYPE_GS2D) return fail(QCS_ERR_ARG, "converged fitting supports the 2D model only");
+    const CFitConst c = cfit_const(p, init_mode);
+    const dim3 grid((n + 63) / 64), block(64);
+    const int R = p.ROISize;
+    if (K == 1) {
+        CF_DISPATCH_R(R, hipLaunchKernelGGL((cfit_kernel<1, RT>), grid, block, 0, s, d_rois, d_idx, n, d_n, d_init, d_theta, d_info, d_ll, c))
+    } else if (K == 2) {
+        CF_DISPATCH_R(R, hipLaunchKernelGGL((cfit_kernel<2, RT>), grid, block, 0, s, d_rois, d_idx, n, d_n, d_init, d_theta, d_info, d_ll, c))
+    } else if (K == 3) {
+        CF_DISPATCH_R(R, hipLaunchKernelGGL((cfit_kernel<3, RT>), grid, block, 0, s, d_rois, d_idx, n, d_n, d_init, d_theta, d_info, d_ll, c))
+    } else {
+        return fail(QCS_ERR_ARG, "emitter count %d not in 1..3", K);
+    }
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+CSelConst csel_const(const qcs_params &p, int kmax)
+{
+    CSelConst c;
+    c.llr_half = 8.14f;   // chi^2(3 dof) at p = 1e-3 is 16.27
+    c.n_min = 50.0f;
+    c.pixel_size = p.PixelSize;
+    c.qe = p.QE;
+    c.kmax = kmax;
+    return c;
+}
+
+int cselect_launch(const qcs_params &p, int kmax, const uint16_t *d_rois, const int *d_idx, int n, const float *t1,
+                   const float *t2, const float *t3, const double *l1, const double *l2, const double *l3, float *d_rows,
+                   int *d_sel, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    const CSelConst c = csel_const(p, kmax);
+    const dim3 grid((n + 63) / 64), block(64);
+    CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_select_kernel<RT>), grid, block, 0, s, d_rois, d_idx, n, t1, t2, t3, l1, l2,
+                                                 l3, c, d_rows, d_sel))
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+}  // namespace qcs
+
+using namespace qcs;
+
+extern "C" int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, int init_mode, const float *d_init,
+                            float *d_theta, int *d_info, double *d_ll, void *stream)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_rois || !d_theta)) || init_mode < 0 || init_mode > 2 ||
+        (init_mode != CF_INIT_HEUR && !d_init))
+        return fail(QCS_ERR_ARG, "invalid argument");
+    if (init_mode == CF_INIT_ADD && K < 2) return fail(QCS_ERR_ARG, "adding an emitter needs K >= 2");
+    hipEvent_t ev;
+    int rc = timer_begin(ctx->timer, (hipStream_t)stream, &ev);
+    if (rc) return rc;
+    rc = cfit_launch(ctx->p, K, d_rois, nullptr, n, nullptr, init_mode, d_init, d_theta, d_info, d_ll, (hipStream_t)stream);
+    if (rc) return rc;
+    return timer_end(ctx->timer, TM_CFIT, (hipStream_t)stream, ev, n);
+}
+
+extern "C" int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, void *stream)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_rois || !d_loc)) || kmax < 1 || kmax > 3) return fail(QCS_ERR_ARG, "invalid argument");
+    if (n == 0) return QCS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nf = (size_t)n * (5 + 8 + 11), nd = (size_t)n * 3;
+    int rc;
+    if ((rc = ctx->cf_theta.ensure(nf * sizeof(float))) || (rc = ctx->cf_ll.ensure(nd * sizeof(double)))) return rc;
+    float *t1 = ctx->cf_theta.as<float>(), *t2 = t1 + (size_t)n * 5, *t3 = t2 + (size_t)n * 8;
+    double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
+    hipEvent_t ev;
+    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+    if ((rc = cfit_launch(ctx->p, 1, d_rois, nullptr, n, nullptr, CF_INIT_HEUR, nullptr, t1, nullptr, l1, s))) return rc;
+    if (kmax >= 2 && (rc = cfit_launch(ctx->p, 2, d_rois, nullptr, n, nullptr, CF_INIT_ADD, t1, t2, nullptr, l2, s))) return rc;
+    if (kmax >= 3 && (rc = cfit_launch(ctx->p, 3, d_rois, nullptr, n, nullptr, CF_INIT_ADD, t2, t3, nullptr, l3, s))) return rc;
+    if ((rc = cselect_launch(ctx->p, kmax, d_rois, nullptr, n, t1, kmax >= 2 ? t2 : nullptr, kmax >= 3 ? t3 : nullptr, l1, l2,
+                             l3, d_loc, d_sel, s)))
+        return rc;
+    return timer_end(ctx->timer, TM_CFIT, s, ev, n);
+}

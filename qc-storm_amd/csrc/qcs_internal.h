@@ -48,7 +48,7 @@ struct HostBuf {
 int validate_params(const qcs_params &p);
 
 // stage timing (api.cpp): a recorded interval per launch group, resolved on demand
-enum { TM_EXTRACT = 0, TM_FIT1E, TM_FITME, TM_CRLB, TM_COMPACT, TM_RENDER, TM_COUNT };
+enum { TM_EXTRACT = 0, TM_FIT1E, TM_FITME, TM_CRLB, TM_COMPACT, TM_RENDER, TM_CFIT, TM_NN, TM_COUNT };
 struct Timer {
     int on = 0;
     struct Pending {
@@ -97,6 +97,8 @@ struct qcs_ctx {
     qcs::DevBuf ex_bkg, ex_tmp, ex_det, ex_mark, ex_cnt, ex_xy, ex_var, ex_frames;
     // fit scratch
     qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;
+    // converged fitting scratch
+    qcs::DevBuf cf_theta, cf_ll, cf_aux;
     // host staging for the host-pointer API
     qcs::DevBuf h_rois_dev, h_wle_dev, h_loc_dev;
     qcs::HostBuf pin_cnt;

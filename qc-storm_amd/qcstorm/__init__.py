@@ -17,6 +17,8 @@ QCS_OK = 0
 QCS_ERR_ARG, QCS_ERR_HIP, QCS_ERR_CAPACITY, QCS_ERR_STATE, QCS_ERR_NOMEM, QCS_ERR_NODEVICE = 1, 2, 3, 4, 5, 6
 LOCTYPE_GS2D, LOCTYPE_AS3D, LOCTYPE_DH3D = 0, 1, 2
 MATH_EXACT, MATH_FAST = 0, 1
+CFIT_INIT_HEUR, CFIT_INIT_GIVEN, CFIT_INIT_ADD = 0, 1, 2
+CFIT_CONVERGED, CFIT_MAXITER, CFIT_STALLED = 1, 2, 4
 REC_FLOATS = 12
 WLE_FLOATS = 6
 TRUTH_FLOATS = 8
@@ -71,6 +73,8 @@ class QcsTiming(C.Structure):
         ("extract_ms", C.c_double), ("fit_1e_ms", C.c_double), ("fit_me_ms", C.c_double), ("crlb_ms", C.c_double),
         ("compact_ms", C.c_double), ("render_ms", C.c_double),
         ("fit_1e_launches", C.c_longlong), ("fit_1e_rois", C.c_longlong), ("fit_me_launches", C.c_longlong),
+        ("cfit_ms", C.c_double), ("nn_ms", C.c_double),
+        ("cfit_launches", C.c_longlong), ("cfit_rois", C.c_longlong), ("nn_launches", C.c_longlong),
     ]
 
 
@@ -98,6 +102,8 @@ PROTOTYPES = {
     "qcs_compact_dev": (_i, [_vp, _vp, _i, _vp, _P(_i), _vp]),
     "qcs_render_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "qcs_fit_rois": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i)]),
+    "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "qcs_extract": (_i, [_vp, _vp, _i, _i, _vp, _vp, _i, _P(_i), _P(_i)]),
     "qcs_session_create": (_i, [_P(QcsParams), _P(_vp)]),
     "qcs_session_feed": (_i, [_vp, _vp, _i]),
@@ -275,6 +281,14 @@ class Context:
         check(lib().qcs_fit_dev(self.h, self._dp(d_rois), self._dp(d_wle), n, self._dp(d_loc), cap, C.byref(rows),
                                 stream), "qcs_fit_dev")
         return rows.value
+
+    def cfit_dev(self, K, d_rois, n, init_mode, d_init, d_theta, d_info=None, d_ll=None, stream=None):
+        check(lib().qcs_cfit_dev(self.h, K, self._dp(d_rois), n, init_mode, self._dp(d_init), self._dp(d_theta),
+                                 self._dp(d_info), self._dp(d_ll), stream), "qcs_cfit_dev")
+
+    def fit_mle_dev(self, d_rois, n, kmax, d_loc, d_sel=None, stream=None):
+        check(lib().qcs_fit_mle_dev(self.h, self._dp(d_rois), n, kmax, self._dp(d_loc), self._dp(d_sel), stream),
+              "qcs_fit_mle_dev")
 
     def wle_dev(self, d_rois, n, d_wle, stream=None):
         check(lib().qcs_wle_dev(self.h, self._dp(d_rois), n, self._dp(d_wle), stream), "qcs_wle_dev")
