@@ -19,6 +19,12 @@ average launch duration, measured with HIP events recorded on the launch stream 
 The CPU baseline is the oracle's OpenMP fit + CRLB (same schedule, bit-identical output) on a
 bounded sample of the same ROIs, on the host cores.
 
+A second leg (`multi_emitter` in the JSON line) fits 2^20 simulator 7x7 ROIs with 1-3 emitters
+1-4 px apart twice: the plain converged path (K = 1, 2, 3 fits and FP64 likelihood-ratio model
+selection) and the learned warm start (network -> converged fit of the predicted count, plain
+fallback), reporting both throughputs, the speedup, and RMSE / recall / RMSE-over-CRLB against
+the simulated truth on the first 65,536 ROIs.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--math exact|fast] [--no-cpu-baseline]
 """
 import argparse
@@ -75,6 +81,80 @@ def cpu_baseline(params, rois, wle, seconds):
                        % (n, dt, threads)}, loc)
 
 
+def match_rmse(loc, truth, pixel_size):
+    """Greedy nearest matching (within 1 px) of fitted rows [n][3][12] to simulated emitters
+    [n][3][8] of 7x7 ROIs (header Xc = Yc = 100); returns rmse (px), recall, precision and the
+    RMS of the matched rows' CRLB (px, mean of x and y)."""
+    n = loc.shape[0]
+    fx, fy = loc[:, :, 1] - 97.0, loc[:, :, 2] - 97.0
+    fv = loc[:, :, 0] > 0
+    tv = truth[:, :, 4] > 0
+    D = (fx[:, None, :] - truth[:, :, 1][:, :, None]) ** 2 + (fy[:, None, :] - truth[:, :, 2][:, :, None]) ** 2
+    D[~(tv[:, :, None] & fv[:, None, :])] = np.inf
+    d2, crb = [], []
+    rows = np.arange(n)
+    for _ in range(3):
+        flat = D.reshape(n, 9)
+        k = flat.argmin(1)
+        best = flat[rows, k]
+        ok = best < 1.0
+        ti, fi = k // 3, k % 3
+        d2.append(best[ok])
+        c = loc[rows, fi, 9:11][ok] / pixel_size
+        crb.append((c ** 2).mean(1))
+        D[rows[ok], ti[ok], :] = np.inf
+        D[rows[ok], :, fi[ok]] = np.inf
+    d2, crb = np.concatenate(d2), np.concatenate(crb)
+    # RMSE against the CRLB over the matched emitters with a finite, informative bound (< 0.5 px);
+    # near-degenerate multi-emitter fits have unbounded CRLBs that would swamp the mean
+    ok = np.isfinite(crb) & (crb > 0) & (crb < 0.25)
+    return {"rmse_px": float(np.sqrt(d2.mean())), "recall": float(len(d2) / max(tv.sum(), 1)),
+            "precision": float(len(d2) / max(fv.sum(), 1)),
+            "rmse_over_crlb": float(np.sqrt(d2[ok].mean() / crb[ok].mean())) if ok.any() else None,
+            "crlb_informative_fraction": float(ok.mean()) if len(ok) else None}
+
+
+def multi_emitter_leg(n, seed, log):
+    """Plain (K = 1..3 converged fits + FP64 selection) vs warm-started multi-emitter fitting of
+    n simulator 7x7 ROIs with 1-3 emitters 1-4 px apart; throughputs from library HIP events."""
+    import torch
+    from qcstorm import warmstart as WS
+    p = WS.params()
+    rois, truth = WS.sim_dev(n, seed)
+    if os.path.exists(WS.DEFAULT_WEIGHTS):
+        net, wsrc = WS.load(), "qc-storm_amd/weights/warmstart_r7.npy"
+    else:
+        log("bench: no saved warm-start weights; training 2^19 ROIs in-run")
+        net, _ = WS.train(n_rois=1 << 19, log=log)
+        wsrc = "trained in-run on 2^19 ROIs"
+    ctx = Q.Context(p)
+    loc_p = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
+    loc_w = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
+    ctx.fit_mle_dev(rois, n, 3, loc_p)
+    st = ctx.fit_warm_dev(net, rois, n, loc_w)
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    ctx.fit_mle_dev(rois, n, 3, loc_p)
+    tp = ctx.timing()
+    ctx.timing_enable(True)
+    ctx.fit_warm_dev(net, rois, n, loc_w, stats=False)
+    tw = ctx.timing()
+    torch.cuda.synchronize()
+    m = min(n, 65536)
+    th = truth[:m].cpu().numpy()
+    rp = match_rmse(loc_p[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize)
+    rw = match_rmse(loc_w[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize)
+    plain_ms, warm_ms = tp["cfit_ms"], tw["cfit_ms"] + tw["nn_ms"]
+    ctx.close()
+    net.close()
+    return {"rois": n, "plain_fits_per_s": n / (plain_ms * 1e-3), "warm_fits_per_s": n / (warm_ms * 1e-3),
+            "warm_speedup": plain_ms / warm_ms, "nn_ms": tw["nn_ms"], "plain_ms": plain_ms, "warm_ms": warm_ms,
+            "plain": rp, "warm": rw, "rmse_ratio_warm_vs_plain": rw["rmse_px"] / rp["rmse_px"],
+            "rmse_over_crlb_plain": rp["rmse_over_crlb"], "rmse_over_crlb_warm": rw["rmse_over_crlb"],
+            "fallback_fraction": st["fallbacks"] / n, "warm_iterations_mean": st["warm_iters"] / n,
+            "weights": wsrc, "accuracy_sample_rois": m}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +164,8 @@ def main():
     ap.add_argument("--math", choices=("exact", "fast"), default="exact")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--multi-rois", type=int, default=1 << 20,
+                    help="ROIs of the multi-emitter plain vs warm-start leg (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -190,6 +272,11 @@ def main():
               "fit_kernel_fits_per_s": tm["fit_1e_rois"] / (tm["fit_1e_ms"] * 1e-3),
               "extract_frame_GBps": F * W * H * 2 * args.steps / (tm["extract_ms"] * 1e-3) / 1e9}
 
+    multi = None
+    if args.multi_rois > 0:
+        log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
+        multi = multi_emitter_leg(args.multi_rois, 1003, log)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -210,7 +297,8 @@ def main():
                                    "10 nm rendering" % F,
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last},
-            "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity}
+            "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
+            "multi_emitter": multi}
     ctx.close()
     print(json.dumps(line), flush=True)
 

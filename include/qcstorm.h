@@ -240,12 +240,64 @@ int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float 
  * NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
 int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, int init_mode, const float *d_init,
                  float *d_theta, int *d_info, double *d_ll, void *stream);
+/* Tuning of the converged fits and of the warm start (per context). */
+typedef struct qcs_tuning {
+    int   max_iter[3];      /* BFGS iteration caps for K = 1, 2, 3 (default 100, 100, 100) */
+    float tol;              /* convergence: quasi-Newton step below tol in every scaled coordinate (1e-4) */
+    float warm_pmin;        /* warm start: minimum predicted-count probability, else plain path (0.8) */
+} qcs_tuning;
+int qcs_tuning_get(qcs_ctx *ctx, qcs_tuning *out);
+int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t);
 /* The plain multi-emitter path: fits K = 1, 2, 3 in sequence (each started from the previous
  * one plus an emitter at the largest residual), picks the emitter count by the FP64
  * log-likelihood ratio (chi^2(3) at p = 1e-3) among valid models (every emitter >= 50 photons
  * inside the ROI), and writes 3 rows per ROI (rows 3j..3j+2, unused rows zero) with the FP64
  * full-Fisher CRLB in CRLBx/CRLBy.  d_sel (n, may be NULL) receives the chosen count. */
 int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, void *stream);
+
+/* ========================================================================================
+ * Learned warm start (no counterpart in the reference): a 3-conv + 2-layer MLP on 7x7 ROIs
+ * (62,192 parameters, BF16 weights on MFMA, FP32 accumulate, FP32 master weights and Adam),
+ * trained on the GPU by this library's forward and backward kernels on simulator ROIs labelled
+ * by the plain path (qcs_label_dev).  It predicts the emitter count (1-3) and the starting
+ * vector of the converged fit.
+ * ====================================================================================== */
+typedef struct qcs_net qcs_net;
+/* number of FP32 parameters (layout: W1[32][16] b1 W2[64][288] b2 W3[64][576] b3 W4[64][80] b4
+ * W5[16][64] b5; conv1 taps 9..15 are zero padding) */
+int qcs_net_param_count(void);
+/* He-uniform initialisation from `seed` (deterministic); roi_size must be 7 */
+int qcs_net_create(int roi_size, uint64_t seed, qcs_net **out);
+int qcs_net_destroy(qcs_net *net);
+int qcs_net_get_params(qcs_net *net, float *w);
+/* load parameters (resets the optimiser state) */
+int qcs_net_set_params(qcs_net *net, const float *w);
+/* outputs [n][16]: 3 count logits, (x/7, y/7, N / (2 pi scale)) for 3 slots, s - 1.3,
+ * (b - bhat) / scale; d_norm (n x 2, may be NULL): per-ROI (scale, bhat) of the input
+ * normalisation */
+int qcs_net_forward_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
+                        void *stream);
+/* labels [n][16]: K (0 = ignore), (x, y, N) x 3 (ROI pixels, photons, sorted by N), s, b.
+ * Gradient of the mean loss (count cross-entropy + squared parameter errors) into h_grad
+ * (qcs_net_param_count floats, may be NULL); *loss (may be NULL) receives the mean loss. */
+int qcs_net_grad_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_rois, const float *d_labels, int n, float *h_grad,
+                     float *loss, void *stream);
+/* one Adam step (beta 0.9 / 0.999) on the mean loss over the n ROIs */
+int qcs_net_train_step_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_rois, const float *d_labels, int n, float lr,
+                           float *loss, void *stream);
+/* labels from the plain path (qcs_fit_mle_dev's fits and model selection); d_sel (n, may be
+ * NULL) receives the selected counts */
+int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_labels, int *d_sel, void *stream);
+/* The warm-started path: network -> per-count work lists -> converged fit of the predicted count
+ * from the predicted start -> fallback to the plain path (qcs_fit_mle_dev) for every ROI whose
+ * warm fit did not converge, is invalid, or ends at a loss above that of the plain heuristic
+ * start.  Rows as qcs_fit_mle_dev.  h_stats (9 long longs, may be NULL; the call then waits):
+ * ROIs per final count [3], fallbacks, warm iterations, warm loss evaluations, ROIs per
+ * predicted count [3]. */
+int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
+                     long long *h_stats, void *stream);
+/* building block: C[M][N] (fp32) = A[M][K] B[N][K]^T, bf16 inputs, K a multiple of 16 */
+int qcs_dev_gemm_nt_bf16(const uint16_t *d_a, const uint16_t *d_b, float *d_c, int M, int N, int K, void *stream);
 
 /* ========================================================================================
  * Host-pointer stage API.  Same semantics as the *_dev calls, with the H2D / D2H copies the

@@ -68,6 +68,10 @@ class QcsFitStats(C.Structure):
     ]
 
 
+class QcsTuning(C.Structure):
+    _fields_ = [("max_iter", C.c_int * 3), ("tol", C.c_float), ("warm_pmin", C.c_float)]
+
+
 class QcsTiming(C.Structure):
     _fields_ = [
         ("extract_ms", C.c_double), ("fit_1e_ms", C.c_double), ("fit_me_ms", C.c_double), ("crlb_ms", C.c_double),
@@ -104,6 +108,19 @@ PROTOTYPES = {
     "qcs_fit_rois": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i)]),
     "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "qcs_tuning_get": (_i, [_vp, _P(QcsTuning)]),
+    "qcs_tuning_set": (_i, [_vp, _P(QcsTuning)]),
+    "qcs_net_param_count": (_i, []),
+    "qcs_net_create": (_i, [_i, _u64, _P(_vp)]),
+    "qcs_net_destroy": (_i, [_vp]),
+    "qcs_net_get_params": (_i, [_vp, _vp]),
+    "qcs_net_set_params": (_i, [_vp, _vp]),
+    "qcs_net_forward_dev": (_i, [_vp, _P(QcsParams), _vp, _i, _vp, _vp, _vp]),
+    "qcs_net_grad_dev": (_i, [_vp, _P(QcsParams), _vp, _vp, _i, _vp, _P(_f), _vp]),
+    "qcs_net_train_step_dev": (_i, [_vp, _P(QcsParams), _vp, _vp, _i, _f, _P(_f), _vp]),
+    "qcs_label_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "qcs_fit_warm_dev": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _P(_ll), _vp]),
+    "qcs_dev_gemm_nt_bf16": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
     "qcs_extract": (_i, [_vp, _vp, _i, _i, _vp, _vp, _i, _P(_i), _P(_i)]),
     "qcs_session_create": (_i, [_P(QcsParams), _P(_vp)]),
     "qcs_session_feed": (_i, [_vp, _vp, _i]),
@@ -290,6 +307,36 @@ class Context:
         check(lib().qcs_fit_mle_dev(self.h, self._dp(d_rois), n, kmax, self._dp(d_loc), self._dp(d_sel), stream),
               "qcs_fit_mle_dev")
 
+    def tuning(self):
+        t = QcsTuning()
+        check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
+        return {"max_iter": list(t.max_iter), "tol": t.tol, "warm_pmin": t.warm_pmin}
+
+    def set_tuning(self, max_iter=None, tol=None, warm_pmin=None):
+        t = QcsTuning()
+        check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
+        if max_iter is not None:
+            for k in range(3):
+                t.max_iter[k] = int(max_iter[k])
+        if tol is not None:
+            t.tol = tol
+        if warm_pmin is not None:
+            t.warm_pmin = warm_pmin
+        check(lib().qcs_tuning_set(self.h, C.byref(t)), "qcs_tuning_set")
+
+    def label_dev(self, d_rois, n, d_labels, d_sel=None, stream=None):
+        check(lib().qcs_label_dev(self.h, self._dp(d_rois), n, self._dp(d_labels), self._dp(d_sel), stream), "qcs_label_dev")
+
+    def fit_warm_dev(self, net, d_rois, n, d_loc, d_sel=None, stats=True, stream=None):
+        st = (C.c_longlong * 9)()
+        check(lib().qcs_fit_warm_dev(self.h, net.h, self._dp(d_rois), n, self._dp(d_loc), self._dp(d_sel),
+                                     st if stats else None, stream), "qcs_fit_warm_dev")
+        if not stats:
+            return None
+        keys = ("final_k1", "final_k2", "final_k3", "fallbacks", "warm_iters", "warm_evals", "pred_k1", "pred_k2",
+                "pred_k3")
+        return dict(zip(keys, list(st)))
+
     def wle_dev(self, d_rois, n, d_wle, stream=None):
         check(lib().qcs_wle_dev(self.h, self._dp(d_rois), n, self._dp(d_wle), stream), "qcs_wle_dev")
 
@@ -391,3 +438,55 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+class Net:
+    """The warm-start network (qcs_net): 7x7 ROIs, BF16 MFMA forward/backward, Adam."""
+
+    def __init__(self, seed=0, roi_size=7):
+        h = C.c_void_p()
+        check(lib().qcs_net_create(roi_size, seed, C.byref(h)), "qcs_net_create")
+        self.h = h
+        self.nparam = lib().qcs_net_param_count()
+
+    def close(self):
+        if self.h:
+            lib().qcs_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        w = np.zeros(self.nparam, np.float32)
+        check(lib().qcs_net_get_params(self.h, _ptr(w)), "qcs_net_get_params")
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.nparam
+        check(lib().qcs_net_set_params(self.h, _ptr(w)), "qcs_net_set_params")
+
+    def forward_dev(self, params, d_rois, n, d_out, d_norm=None, stream=None):
+        check(lib().qcs_net_forward_dev(self.h, C.byref(params), C.c_void_p(d_rois.data_ptr()), n,
+                                        C.c_void_p(d_out.data_ptr()),
+                                        C.c_void_p(d_norm.data_ptr()) if d_norm is not None else None, stream),
+              "qcs_net_forward_dev")
+
+    def grad_dev(self, params, d_rois, d_labels, n, stream=None):
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_float()
+        check(lib().qcs_net_grad_dev(self.h, C.byref(params), C.c_void_p(d_rois.data_ptr()),
+                                     C.c_void_p(d_labels.data_ptr()), n, _ptr(g), C.byref(loss), stream),
+              "qcs_net_grad_dev")
+        return g, loss.value
+
+    def train_step_dev(self, params, d_rois, d_labels, n, lr, stream=None, want_loss=True):
+        loss = C.c_float()
+        check(lib().qcs_net_train_step_dev(self.h, C.byref(params), C.c_void_p(d_rois.data_ptr()),
+                                           C.c_void_p(d_labels.data_ptr()), n, lr,
+                                           C.byref(loss) if want_loss else None, stream), "qcs_net_train_step_dev")
+        return loss.value if want_loss else None
