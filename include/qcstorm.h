@@ -245,6 +245,9 @@ typedef struct qcs_tuning {
     int   max_iter[3];      /* BFGS iteration caps for K = 1, 2, 3 (default 100, 100, 100) */
     float tol;              /* convergence: quasi-Newton step below tol in every scaled coordinate (1e-4) */
     float warm_pmin;        /* warm start: minimum predicted-count probability, else plain path (0.8) */
+    int   queue;            /* 1: persistent work-queue kernels (default); 0: one-pass kernels, a
+                             * wave per 64 ROIs (same results) */
+    int   waves_per_cu;     /* persistent waves per CU; 0 = as many as fit (default) */
 } qcs_tuning;
 int qcs_tuning_get(qcs_ctx *ctx, qcs_tuning *out);
 int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t);
@@ -254,6 +257,18 @@ int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t);
  * inside the ROI), and writes 3 rows per ROI (rows 3j..3j+2, unused rows zero) with the FP64
  * full-Fisher CRLB in CRLBx/CRLBy.  d_sel (n, may be NULL) receives the chosen count. */
 int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, void *stream);
+/* One set of same-size ROIs in a mixed-size call. */
+typedef struct qcs_roi_set {
+    int roi_size;             /* 7, 9 or 11 */
+    int n;                    /* ROIs in the set */
+    const uint16_t *d_rois;   /* n ROI records of roi_size */
+    float *d_loc;             /* 3n rows, rows 3j..3j+2 belong to ROI j (as qcs_fit_mle_dev) */
+    int *d_sel;               /* optional: chosen emitter count per ROI */
+} qcs_roi_set;
+/* qcs_fit_mle_dev over up to three sets of different ROI sizes: one persistent work-queue launch
+ * per emitter count covers every set (largest ROIs first); the context's ROISize is not used.
+ * Results equal per-set qcs_fit_mle_dev calls. */
+int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int nsets, int kmax, void *stream);
 
 /* ========================================================================================
  * Learned warm start (no counterpart in the reference): a 3-conv + 2-layer MLP on 7x7 ROIs
@@ -274,9 +289,12 @@ int qcs_net_get_params(qcs_net *net, float *w);
 int qcs_net_set_params(qcs_net *net, const float *w);
 /* outputs [n][16]: 3 count logits, (x/7, y/7, N / (2 pi scale)) for 3 slots, s - 1.3,
  * (b - bhat) / scale; d_norm (n x 2, may be NULL): per-ROI (scale, bhat) of the input
- * normalisation */
+ * normalisation.  One fused kernel (16 ROIs per wave, activations in registers). */
 int qcs_net_forward_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
                         void *stream);
+/* the same outputs through the training path's layer-by-layer GEMMs (activations in HBM) */
+int qcs_net_forward_layers_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
+                               void *stream);
 /* labels [n][16]: K (0 = ignore), (x, y, N) x 3 (ROI pixels, photons, sorted by N), s, b.
  * Gradient of the mean loss (count cross-entropy + squared parameter errors) into h_grad
  * (qcs_net_param_count floats, may be NULL); *loss (may be NULL) receives the mean loss. */

@@ -22,297 +22,12 @@
 // line search that finds no decrease in 16 backtracks ends the fit as stalled.
 #include <math.h>
 #include <string.h>
-#include "qcs_internal.h"
+#include "cfit_dev.h"
 
 extern "C" int qcs_net_forward_dev(qcs_net *nt, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
                                    void *stream);
 
 namespace qcs {
-
-struct CFitConst {
-    float offset, kadc, inv_qe;
-    int max_iter;
-    float tol;          // convergence: max |step| in scaled units
-    int init_mode;      // CF_INIT_*
-    float sigma0;       // heuristic initial PSF width (px)
-};
-enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
-enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4 };
-
-template <int K> struct CP { static constexpr int P = 3 * K + 2, NS = P * (P + 1) / 2; };
-
-__device__ __forceinline__ float cf_scale(int i, int P)
-{
-    // N -> kilo-photons, b -> hundreds of photons, positions and width in pixels
-    return (i == P - 1) ? 100.0f : ((i < P - 2 && i % 3 == 0) ? 1000.0f : 1.0f);
-}
-
-// packed upper-triangular index of (i, j), i <= j
-__device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
-
-// loss sum((mu - q) - q ln(mu / q)) (the Poisson negative log-likelihood up to a constant) and,
-// if G, its gradient with respect to theta (physical units)
-template <int K, int R, bool G>
-__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<K>::P], float (&g)[CP<K>::P])
-{
-    constexpr int P = CP<K>::P;
-    const float s = th[3 * K], b = th[3 * K + 1];
-    const float is2 = 1.0f / (s * s);
-    const float h = -0.5f * is2;
-    const float nrm = 0.15915494309189535f * is2;   // 1 / (2 pi s^2)
-    float ex[K][R], dx[K][R];
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            dx[k][c] = ((float)c + 0.5f) - th[3 * k + 1];
-            ex[k][c] = __expf(h * dx[k][c] * dx[k][c]);
-        }
-    float L = 0.0f;
-    float gN[K], gx[K], gy[K], gd2 = 0.0f, gw = 0.0f, gb = 0.0f;
-#pragma unroll
-    for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        float ay[K], dy[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
-            ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
-        }
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            float gk[K];
-            float mu = b;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                gk[k] = ay[k] * ex[k][c];
-                mu += gk[k];
-            }
-            mu = fmaxf(mu, 1e-6f);
-            const float q = qs[(r * R + c) * 64];
-            // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
-            L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
-            if (G) {
-                const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
-                gb += w;
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const float wg = w * gk[k];
-                    gN[k] += wg;
-                    gx[k] = fmaf(wg, dx[k][c], gx[k]);
-                    gy[k] = fmaf(wg, dy[k], gy[k]);
-                    gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
-                    gw += wg;
-                }
-            }
-        }
-    }
-    if (G) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            g[3 * k] = gN[k] / fmaxf(th[3 * k], 1e-3f);
-            g[3 * k + 1] = gx[k] * is2;
-            g[3 * k + 2] = gy[k] * is2;
-        }
-        g[3 * K] = (gd2 * is2 - 2.0f * gw) / s;
-        g[3 * K + 1] = gb;
-        (void)P;
-    }
-    return L;
-}
-
-template <int K, int R>
-__device__ __forceinline__ void cf_project(float (&th)[CP<K>::P])
-{
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        th[3 * k] = fmaxf(th[3 * k], 1.0f);
-        th[3 * k + 1] = fminf(fmaxf(th[3 * k + 1], -0.5f), (float)R + 0.5f);
-        th[3 * k + 2] = fminf(fmaxf(th[3 * k + 2], -0.5f), (float)R + 0.5f);
-    }
-    th[3 * K] = fminf(fmaxf(th[3 * K], 0.4f), 0.5f * (float)R);
-    th[3 * K + 1] = fmaxf(th[3 * K + 1], 0.01f);
-}
-
-// Inverse of the Fisher information sum grad(mu) grad(mu)^T / mu at theta, in scaled units,
-// into the packed symmetric D (the BFGS starting inverse Hessian).  Returns false when the
-// Cholesky factorisation fails (D untouched).
-template <int K, int R>
-__device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<K>::P], float *D)
-{
-    constexpr int P = CP<K>::P, NS = CP<K>::NS;
-    float F[NS];
-#pragma unroll
-    for (int i = 0; i < NS; i++) F[i] = 0.0f;
-    const float s = th[3 * K], is2 = 1.0f / (s * s), h = -0.5f * is2, nrm = 0.15915494309189535f * is2;
-    for (int r = 0; r < R; r++) {
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            float J[P];
-            float mu = th[3 * K + 1], jw = 0.0f;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const float dx = ((float)c + 0.5f) - th[3 * k + 1], dy = ((float)r + 0.5f) - th[3 * k + 2];
-                const float d2 = fmaf(dx, dx, dy * dy);
-                const float mk = th[3 * k] * nrm * __expf(h * d2);
-                mu += mk;
-                J[3 * k] = mk / fmaxf(th[3 * k], 1e-3f) * cf_scale(3 * k, P);
-                J[3 * k + 1] = mk * dx * is2;
-                J[3 * k + 2] = mk * dy * is2;
-                jw = fmaf(mk, fmaf(d2, is2, -2.0f), jw);
-            }
-            J[3 * K] = jw / s;
-            J[3 * K + 1] = cf_scale(3 * K + 1, P);
-            const float im = __builtin_amdgcn_rcpf(fmaxf(mu, 1e-6f));
-#pragma unroll
-            for (int a = 0; a < P; a++) {
-                const float ja = J[a] * im;
-#pragma unroll
-                for (int b2 = a; b2 < P; b2++) F[tri(a, b2, P)] = fmaf(ja, J[b2], F[tri(a, b2, P)]);
-            }
-        }
-    }
-    // F = U^T U (U upper, in place)
-#pragma unroll
-    for (int j = 0; j < P; j++) {
-        float d = F[tri(j, j, P)];
-#pragma unroll
-        for (int k = 0; k < j; k++) d = fmaf(-F[tri(k, j, P)], F[tri(k, j, P)], d);
-        if (!(d > 0.0f)) return false;
-        const float u = sqrtf(d), iu = 1.0f / u;
-        F[tri(j, j, P)] = u;
-#pragma unroll
-        for (int i = j + 1; i < P; i++) {
-            float v = F[tri(j, i, P)];
-#pragma unroll
-            for (int k = 0; k < j; k++) v = fmaf(-F[tri(k, j, P)], F[tri(k, i, P)], v);
-            F[tri(j, i, P)] = v * iu;
-        }
-    }
-    // U^-1 in place, rows bottom-up, each row right to left
-#pragma unroll
-    for (int i = P - 1; i >= 0; i--) {
-        const float iu = 1.0f / F[tri(i, i, P)];
-#pragma unroll
-        for (int j = P - 1; j > i; j--) {
-            float v = 0.0f;
-#pragma unroll
-            for (int k = i + 1; k <= j; k++) v = fmaf(F[tri(i, k, P)], F[tri(k, j, P)], v);
-            F[tri(i, j, P)] = -v * iu;
-        }
-        F[tri(i, i, P)] = iu;
-    }
-    // F^-1 = U^-1 U^-T, in place row by row (entries (i, j), j >= i, use row i and row j from k >= j)
-#pragma unroll
-    for (int i = 0; i < P; i++)
-#pragma unroll
-        for (int j = i; j < P; j++) {
-            float v = 0.0f;
-#pragma unroll
-            for (int k = j; k < P; k++) v = fmaf(F[tri(i, k, P)], F[tri(j, k, P)], v);
-            F[tri(i, j, P)] = v;
-        }
-#pragma unroll
-    for (int i = 0; i < NS; i++) D[i * 64] = F[i];
-    return true;
-}
-
-// FP64 log-likelihood sum(q ln mu - mu) (without the ln q! term, common to every model)
-template <int K, int R>
-__device__ double cf_loglik64(const float *qs, const float (&th)[CP<K>::P])
-{
-    const double s = th[3 * K], b = th[3 * K + 1];
-    const double h = -0.5 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
-    double ll = 0.0;
-    for (int r = 0; r < R; r++)
-        for (int c = 0; c < R; c++) {
-            double mu = b;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
-                mu += (double)th[3 * k] * nrm * exp(h * (dx * dx + dy * dy));
-            }
-            const double q = qs[(r * R + c) * 64];
-            ll += q * log(mu) - mu;
-        }
-    return ll;
-}
-
-// initial vector: heuristic (K = 1), given, or the previous model plus an emitter at the largest
-// positive residual (K >= 2; init holds the K-1 model's theta)
-template <int K, int R>
-__device__ __forceinline__ void cf_init(const float *qs, const float *init, int mode, float sigma0, float (&th)[CP<K>::P])
-{
-    constexpr int P = CP<K>::P;
-    if (mode == CF_INIT_GIVEN) {
-#pragma unroll
-        for (int i = 0; i < P; i++) th[i] = init[i];
-        return;
-    }
-    if constexpr (K > 1) {
-      if (mode == CF_INIT_ADD) {
-        constexpr int P0 = CP<K - 1>::P;
-        float t0[P0];
-#pragma unroll
-        for (int i = 0; i < P0; i++) t0[i] = init[i];
-        const float s = t0[3 * (K - 1)], b = t0[3 * (K - 1) + 1];
-        const float h = -0.5f / (s * s), nrm = 0.15915494309189535f / (s * s);
-        float best = -1e30f;
-        int bi = (R / 2) * R + R / 2;
-        for (int r = 0; r < R; r++)
-            for (int c = 0; c < R; c++) {
-                float mu = b;
-#pragma unroll
-                for (int k = 0; k < K - 1; k++) {
-                    const float dx = ((float)c + 0.5f) - t0[3 * k + 1], dy = ((float)r + 0.5f) - t0[3 * k + 2];
-                    mu += t0[3 * k] * nrm * __expf(h * (dx * dx + dy * dy));
-                }
-                const float res = qs[(r * R + c) * 64] - mu;
-                if (res > best) { best = res; bi = r * R + c; }
-            }
-#pragma unroll
-        for (int k = 0; k < K - 1; k++) {
-            th[3 * k] = t0[3 * k];
-            th[3 * k + 1] = t0[3 * k + 1];
-            th[3 * k + 2] = t0[3 * k + 2];
-        }
-        th[3 * (K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * s * s;
-        th[3 * (K - 1) + 1] = (float)(bi % R) + 0.5f;
-        th[3 * (K - 1) + 2] = (float)(bi / R) + 0.5f;
-        th[3 * K] = s;
-        th[3 * K + 1] = b;
-        return;
-      }
-    }
-    // heuristic: background from the smaller of the opposite edge means, photons from the excess,
-    // position at the excess-weighted centroid (single emitter; extra emitters share it)
-    float e[4] = {0, 0, 0, 0};
-    for (int i = 0; i < R; i++) {
-        e[0] += qs[(0 * R + i) * 64];
-        e[1] += qs[((R - 1) * R + i) * 64];
-        e[2] += qs[(i * R + 0) * 64];
-        e[3] += qs[(i * R + R - 1) * 64];
-    }
-    const float b = 0.5f * (fminf(e[0], e[1]) + fminf(e[2], e[3])) / (float)R;
-    float tot = 0.0f, sx = 0.0f, sy = 0.0f;
-    for (int r = 0; r < R; r++)
-        for (int c = 0; c < R; c++) {
-            const float v = fmaxf(qs[(r * R + c) * 64] - b, 0.0f);
-            tot += v;
-            sx = fmaf(v, (float)c + 0.5f, sx);
-            sy = fmaf(v, (float)r + 0.5f, sy);
-        }
-    const float cx = tot > 0.0f ? sx / tot : 0.5f * R, cy = tot > 0.0f ? sy / tot : 0.5f * R;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        th[3 * k] = fmaxf(tot, 50.0f) / (float)K;
-        th[3 * k + 1] = cx + 0.3f * (float)k;
-        th[3 * k + 2] = cy;
-    }
-    th[3 * K] = sigma0;
-    th[3 * K + 1] = fmaxf(b, 0.5f);
-}
 
 // Work item j fits ROI roi = idx ? idx[j] : j (j < n, or < *n_dev when given).  Everything per
 // ROI is indexed by roi: init (stride istride), theta_out (physical units, stride ostride),
@@ -520,119 +235,6 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
     if (ll_out) ll_out[roi] = cf_loglik64<K, R>(qs, th);
 }
 
-// ---- FP64 full-Fisher CRLB of one model (standard deviations of x_k and y_k, px) ----------
-// I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed in LDS per lane; Cholesky; then
-// (I^-1)_ii = |L^-1 e_i|^2 for the position indices.
-template <int K, int R>
-__device__ void cf_crlb64(const float (&th)[CP<K>::P], double *I, float (&cx)[K], float (&cy)[K])
-{
-    constexpr int P = CP<K>::P;
-    for (int i = 0; i < CP<K>::NS; i++) I[i * 64] = 0.0;
-    const double s = th[3 * K], is2 = 1.0 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
-    for (int r = 0; r < R; r++)
-        for (int c = 0; c < R; c++) {
-            double J[P];
-            double mu = th[3 * K + 1];
-            double jw = 0.0;
-            for (int k = 0; k < K; k++) {
-                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
-                const double d2 = dx * dx + dy * dy;
-                const double mk = (double)th[3 * k] * nrm * exp(-0.5 * d2 * is2);
-                mu += mk;
-                J[3 * k] = mk / (double)th[3 * k];
-                J[3 * k + 1] = mk * dx * is2;
-                J[3 * k + 2] = mk * dy * is2;
-                jw += mk * (d2 * is2 - 2.0) / s;
-            }
-            J[3 * K] = jw;
-            J[3 * K + 1] = 1.0;
-            const double im = 1.0 / mu;
-            for (int a = 0; a < P; a++) {
-                const double ja = J[a] * im;
-                for (int b2 = a; b2 < P; b2++) I[tri(a, b2, P) * 64] += ja * J[b2];
-            }
-        }
-    // in-place Cholesky of the packed upper triangle: I = U^T U
-    bool ok = true;
-    for (int j = 0; j < P; j++) {
-        double d = I[tri(j, j, P) * 64];
-        for (int k = 0; k < j; k++) d -= I[tri(k, j, P) * 64] * I[tri(k, j, P) * 64];
-        if (!(d > 0.0)) { ok = false; break; }
-        const double u = sqrt(d);
-        I[tri(j, j, P) * 64] = u;
-        for (int i = j + 1; i < P; i++) {
-            double v = I[tri(j, i, P) * 64];
-            for (int k = 0; k < j; k++) v -= I[tri(k, j, P) * 64] * I[tri(k, i, P) * 64];
-            I[tri(j, i, P) * 64] = v / u;
-        }
-    }
-    for (int k = 0; k < K; k++) {
-        for (int w = 0; w < 2; w++) {
-            const int e = 3 * k + 1 + w;
-            double var = -1.0;
-            if (ok) {
-                // (I^-1)_ee = |U^-T e_e|^2: forward substitution U^T z = e_e
-                double z[P];
-                for (int i = 0; i < P; i++) {
-                    double v = (i == e) ? 1.0 : 0.0;
-                    for (int k2 = 0; k2 < i; k2++) v -= I[tri(k2, i, P) * 64] * z[k2];
-                    z[i] = v / I[tri(i, i, P) * 64];
-                }
-                var = 0.0;
-                for (int i = 0; i < P; i++) var += z[i] * z[i];
-            }
-            const float sd = var > 0.0 ? (float)sqrt(var) : -1.0f;
-            if (w == 0) cx[k] = sd; else cy[k] = sd;
-        }
-    }
-}
-
-struct CSelConst {
-    float llr_half;     // accept K+1 over K when ll(K+1) - ll(K) exceeds this (half the chi^2 bound)
-    float n_min;        // minimum photons of every emitter of an accepted model
-    float pixel_size, qe;
-    int kmax;
-};
-
-template <int K, int R>
-__device__ __forceinline__ bool cf_valid(const float *t, float n_min)
-{
-    for (int k = 0; k < K; k++) {
-        if (!(t[3 * k] >= n_min)) return false;
-        if (!(t[3 * k + 1] > 0.0f && t[3 * k + 1] < (float)R && t[3 * k + 2] > 0.0f && t[3 * k + 2] < (float)R)) return false;
-    }
-    return t[3 * K] > 0.45f && t[3 * K] < 0.45f * (float)R;
-}
-
-template <int K, int R>
-__device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, double *I, float *rows)
-{
-    float th[CP<K>::P];
-    for (int i = 0; i < CP<K>::P; i++) th[i] = t[i];
-    float cx[K], cy[K];
-    cf_crlb64<K, R>(th, I, cx, cy);
-    const float s = th[3 * K], b = th[3 * K + 1];
-    const int half = R / 2;
-    const float xc = (float)hdr[0], yc = (float)hdr[1];
-    const int fr = (int)hdr[2] + (int)hdr[3] * 65536;
-    for (int k = 0; k < K; k++) {
-        float *o = rows + (size_t)k * QCS_REC_FLOATS;
-        const float peak = th[3 * k] * 0.15915494309189535f / (s * s);
-        o[QCS_POS_PPHO] = peak;
-        o[QCS_POS_XPOS] = (th[3 * k + 1] - (float)half) + xc;
-        o[QCS_POS_YPOS] = (th[3 * k + 2] - (float)half) + yc;
-        o[QCS_POS_ZPOS] = 0.0f;
-        o[QCS_POS_SIGX] = s;
-        o[QCS_POS_SIGY] = s;
-        o[QCS_POS_TPHO] = th[3 * k];
-        o[QCS_POS_BAKG] = b;
-        o[QCS_POS_PSNR] = sqrtf(c.qe) * peak / sqrtf(peak + b);
-        o[QCS_POS_CRBX] = cx[k] > 0.0f ? cx[k] * c.pixel_size : -1.0f;
-        o[QCS_POS_CRBY] = cy[k] > 0.0f ? cy[k] * c.pixel_size : -1.0f;
-        o[QCS_POS_FRME] = (float)fr;
-    }
-}
-
 // Model selection (FP64 log-likelihood ratio) and rows: ROI j gets rows 3j .. 3j+2, the unused
 // ones zero.  sel[j] receives the chosen emitter count (0: no valid model).
 template <int R>
@@ -678,20 +280,29 @@ CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_m
     return c;
 }
 
-#define CF_DISPATCH_R(R_, ...)                                                            \
-    switch (R_) {                                                                          \
-    case 7: { constexpr int RT = 7; __VA_ARGS__; } break;                                  \
-    case 9: { constexpr int RT = 9; __VA_ARGS__; } break;                                  \
-    case 11: { constexpr int RT = 11; __VA_ARGS__; } break;                                \
-    default: return fail(QCS_ERR_ARG, "converged fitting supports ROISize 7, 9, 11 (got %d)", R_); \
-    }
-
-int cfit_launch(const qcs_params &p, const qcs_tuning &tu, int K, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n,
-                int init_mode, const float *d_init, int istride, float *d_theta, int ostride, int *d_info, double *d_ll, hipStream_t s)
+int cfit_launch(qcs_ctx *ctx, int K, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n, int init_mode,
+                const float *d_init, int istride, float *d_theta, int ostride, int *d_info, double *d_ll, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
+    const qcs_params &p = ctx->p;
     if (p.LocType != QCS_LOCTYPE_GS2D) return fail(QCS_ERR_ARG, "converged fitting supports the 2D model only");
-    const CFitConst c = cfit_const(p, tu, K, init_mode);
+    if (ctx->tune.queue) {
+        CfqSet x;
+        memset(&x, 0, sizeof(x));
+        x.rois = d_rois;
+        x.idx = d_idx;
+        x.n_dev = d_n;
+        x.init = d_init;
+        x.theta = d_theta;
+        x.info = d_info;
+        x.ll = d_ll;
+        x.n = n;
+        x.R = p.ROISize;
+        x.istride = istride;
+        x.ostride = ostride;
+        return cfq_launch(ctx, K, init_mode, &x, 1, s);
+    }
+    const CFitConst c = cfit_const(p, ctx->tune, K, init_mode);
     const dim3 grid((n + 63) / 64), block(64);
     const int R = p.ROISize;
     if (K == 1) {
@@ -709,13 +320,13 @@ int cfit_launch(const qcs_params &p, const qcs_tuning &tu, int K, const uint16_t
 
 // K = 1, 2, 3 in sequence over the listed ROIs (or all n); t1/t2/t3 (strides 5/8/11) and l1..l3
 // are indexed by ROI
-int plain_multi_launch(const qcs_params &p, const qcs_tuning &tu, int kmax, const uint16_t *d_rois, const int *d_idx, int n,
-                       const int *d_n, float *t1, float *t2, float *t3, double *l1, double *l2, double *l3, hipStream_t s)
+int plain_multi_launch(qcs_ctx *ctx, int kmax, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n, float *t1,
+                       float *t2, float *t3, double *l1, double *l2, double *l3, hipStream_t s)
 {
     int rc;
-    if ((rc = cfit_launch(p, tu, 1, d_rois, d_idx, n, d_n, CF_INIT_HEUR, nullptr, 0, t1, 5, nullptr, l1, s))) return rc;
-    if (kmax >= 2 && (rc = cfit_launch(p, tu, 2, d_rois, d_idx, n, d_n, CF_INIT_ADD, t1, 5, t2, 8, nullptr, l2, s))) return rc;
-    if (kmax >= 3 && (rc = cfit_launch(p, tu, 3, d_rois, d_idx, n, d_n, CF_INIT_ADD, t2, 8, t3, 11, nullptr, l3, s))) return rc;
+    if ((rc = cfit_launch(ctx, 1, d_rois, d_idx, n, d_n, CF_INIT_HEUR, nullptr, 0, t1, 5, nullptr, l1, s))) return rc;
+    if (kmax >= 2 && (rc = cfit_launch(ctx, 2, d_rois, d_idx, n, d_n, CF_INIT_ADD, t1, 5, t2, 8, nullptr, l2, s))) return rc;
+    if (kmax >= 3 && (rc = cfit_launch(ctx, 3, d_rois, d_idx, n, d_n, CF_INIT_ADD, t2, 8, t3, 11, nullptr, l3, s))) return rc;
     return QCS_OK;
 }
 
@@ -750,34 +361,44 @@ int cselect_launch(const qcs_params &p, int kmax, const uint16_t *d_rois, const 
 // predicted emitter count and appends the ROI to that count's work list.
 __global__ void warm_decode_kernel(const float *__restrict__ out, const float *__restrict__ norm, int n, int R, float pmin,
                                    float *__restrict__ init11, int *__restrict__ khat, int *__restrict__ cnt,
-                                   int *__restrict__ lists, int *__restrict__ fb_list)
+                                   int *__restrict__ lists, int *__restrict__ fb_list, int *__restrict__ info)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    const float *o = out + (size_t)j * 16;
-    int K = 1;
-    if (o[1] > o[0]) K = 2;
-    if (o[2] > o[K - 1]) K = 3;
-    // probability of the predicted count; below pmin the ROI goes straight to the plain path
-    const float m = o[K - 1];
-    const float pk = 1.0f / (__expf(o[0] - m) + __expf(o[1] - m) + __expf(o[2] - m));
-    if (!(pk >= pmin)) {
-        khat[j] = 0;
-        fb_list[atomicAdd(&cnt[3], 1)] = j;
-        return;
+    int cls = -1;   // 0..2: warm list of count cls + 1; 3: fallback list
+    if (j < n) {
+        // the warm fits write info only for the ROIs they take
+        *reinterpret_cast<int4 *>(info + (size_t)j * 4) = make_int4(0, 0, 0, 0);
+        const float *o = out + (size_t)j * 16;
+        int K = 1;
+        if (o[1] > o[0]) K = 2;
+        if (o[2] > o[K - 1]) K = 3;
+        // probability of the predicted count; below pmin the ROI goes straight to the plain path
+        const float m = o[K - 1];
+        const float pk = 1.0f / (__expf(o[0] - m) + __expf(o[1] - m) + __expf(o[2] - m));
+        if (pk >= pmin) {
+            const float scale = norm[2 * j], bh = norm[2 * j + 1];
+            float *t = init11 + (size_t)j * 11;
+            for (int k = 0; k < K; k++) {
+                t[3 * k] = fmaxf(o[5 + 3 * k] * 6.283185307179586f * scale, 30.0f);
+                t[3 * k + 1] = fminf(fmaxf(o[3 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+                t[3 * k + 2] = fminf(fmaxf(o[4 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+            }
+            t[3 * K] = fminf(fmaxf(o[12] + 1.3f, 0.6f), 3.0f);
+            t[3 * K + 1] = fmaxf(bh + o[13] * scale, 0.5f);
+            khat[j] = K;
+            cls = K - 1;
+        } else {
+            khat[j] = 0;
+            cls = 3;
+        }
     }
-    const float scale = norm[2 * j], bh = norm[2 * j + 1];
-    float *t = init11 + (size_t)j * 11;
-    for (int k = 0; k < K; k++) {
-        t[3 * k] = fmaxf(o[5 + 3 * k] * 6.283185307179586f * scale, 30.0f);
-        t[3 * k + 1] = fminf(fmaxf(o[3 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
-        t[3 * k + 2] = fminf(fmaxf(o[4 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+    for (int c = 0; c < 4; c++) {
+        const int pos = wave_append(&cnt[c], cls == c);
+        if (pos >= 0) {
+            if (c == 3) fb_list[pos] = j;
+            else lists[(size_t)c * n + pos] = j;
+        }
     }
-    t[3 * K] = fminf(fmaxf(o[12] + 1.3f, 0.6f), 3.0f);
-    t[3 * K + 1] = fmaxf(bh + o[13] * scale, 0.5f);
-    khat[j] = K;
-    const int pos = atomicAdd(&cnt[K - 1], 1);
-    lists[(size_t)(K - 1) * n + pos] = j;
 }
 
 // Fallback test of the warm fit of ROI j (count K = khat[j]): not converged, an invalid model, or
@@ -807,15 +428,17 @@ __global__ void __launch_bounds__(64) warm_check_kernel(const uint16_t *__restri
     float *qs = qsh + lane;
     const uint16_t *src = rois + (size_t)(valid ? j : 0) * WR;
     for (int k = 0; k < RR; k++) qs[k * 64] = fmaxf(valid ? ((float)src[k] - c.offset) * c.kadc : 1.0f, 1.0f) * c.inv_qe;
-    if (!valid) return;
-    const int K = khat[j];
-    if (K == 0) return;   // already in the fallback list (low-confidence prediction)
-    const int flags = info[(size_t)j * 4 + 1];
-    bool fb = true;
-    if (K == 1) warm_check_one<1, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
-    else if (K == 2) warm_check_one<2, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
-    else if (K == 3) warm_check_one<3, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
-    if (fb) fb_list[atomicAdd(fb_cnt, 1)] = j;
+    bool fb = false;
+    const int K = valid ? khat[j] : 0;   // K = 0: already in the fallback list (low confidence)
+    if (K > 0) {
+        const int flags = info[(size_t)j * 4 + 1];
+        fb = true;
+        if (K == 1) warm_check_one<1, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
+        else if (K == 2) warm_check_one<2, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
+        else if (K == 3) warm_check_one<3, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
+    }
+    const int pos = wave_append(fb_cnt, fb);
+    if (pos >= 0) fb_list[pos] = j;
 }
 
 // rows of the warm fits (3 per ROI, unused zero), FP64 full-Fisher CRLB
@@ -902,7 +525,7 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     if (R != 7) return fail(QCS_ERR_ARG, "the warm start takes 7x7 ROIs");
     // scratch: out [n][16], norm [n][2], init/theta [n][11], khat, lists 3n, fb list n, counters,
     // info [n][4]; plain buffers for the fallback (roi-indexed)
-    const size_t nf = (size_t)n * (16 + 2 + 11 + 5 + 8 + 11), ni = (size_t)n * (1 + 3 + 1 + 4) + 32;
+    const size_t nf = (size_t)n * (16 + 2 + 11 + 5 + 8 + 11), ni = (size_t)n * (1 + 3 + 1 + 4) + 32 + 4;
     int rc;
     if ((rc = ctx->cf_theta.ensure(nf * 4)) || (rc = ctx->cf_aux.ensure(ni * 4 + 64)) || (rc = ctx->cf_ll.ensure((size_t)n * 3 * 8)) ||
         (rc = ctx->pin_cnt.ensure(128)))
@@ -913,6 +536,7 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     int *cnt = ctx->cf_aux.as<int>();
     long long *st = reinterpret_cast<long long *>(cnt + 8);
     int *khat = cnt + 32, *lists = khat + n, *fbl = lists + (size_t)3 * n, *info = fbl + n;
+    info = reinterpret_cast<int *>((reinterpret_cast<uintptr_t>(info) + 15) & ~(uintptr_t)15);   // 16-byte rows
     double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
     hipLaunchKernelGGL(warm_zero_kernel, dim3(1), dim3(64), 0, s, cnt, 8 + 12);
     hipEvent_t ev;
@@ -921,11 +545,11 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     if ((rc = timer_end(ctx->timer, TM_NN, s, ev, n))) return rc;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
     hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 255) / 256), dim3(256), 0, s, out, norm, n, R, ctx->tune.warm_pmin, t11, khat, cnt,
-                       lists, fbl);
+                       lists, fbl, info);
     QCS_HIP(hipGetLastError());
     for (int K = 1; K <= 3; K++)
-        if ((rc = cfit_launch(p, ctx->tune, K, d_rois, lists + (size_t)(K - 1) * n, n, cnt + K - 1, CF_INIT_GIVEN, t11, 11, t11,
-                              11, info, nullptr, s)))
+        if ((rc = cfit_launch(ctx, K, d_rois, lists + (size_t)(K - 1) * n, n, cnt + K - 1, CF_INIT_GIVEN, t11, 11, t11, 11, info,
+                              nullptr, s)))
             return rc;
     const CFitConst cc = cfit_const(p, ctx->tune, 1, CF_INIT_HEUR);
     const CSelConst sc = csel_const(p, 3);
@@ -934,7 +558,7 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     hipLaunchKernelGGL((warm_rows_kernel<7>), dim3((n + 63) / 64), dim3(64), 0, s, d_rois, n, khat, t11, sc, d_loc, d_sel);
     QCS_HIP(hipGetLastError());
     // fallback: the plain path over the listed ROIs (rows and counts overwrite the warm ones)
-    if ((rc = plain_multi_launch(p, ctx->tune, 3, d_rois, fbl, n, cnt + 3, t1, t2, t3, l1, l2, l3, s))) return rc;
+    if ((rc = plain_multi_launch(ctx, 3, d_rois, fbl, n, cnt + 3, t1, t2, t3, l1, l2, l3, s))) return rc;
     if ((rc = cselect_launch(p, 3, d_rois, fbl, n, cnt + 3, t1, t2, t3, l1, l2, l3, d_loc, d_sel, s))) return rc;
     if ((rc = timer_end(ctx->timer, TM_CFIT, s, ev, n))) return rc;
     if (h_stats) {
@@ -967,8 +591,8 @@ extern "C" int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, 
     hipEvent_t ev;
     int rc = timer_begin(ctx->timer, (hipStream_t)stream, &ev);
     if (rc) return rc;
-    rc = cfit_launch(ctx->p, ctx->tune, K, d_rois, nullptr, n, nullptr, init_mode, d_init, init_mode == CF_INIT_ADD ? 3 * K - 1 : 3 * K + 2,
-                     d_theta, 3 * K + 2, d_info, d_ll, (hipStream_t)stream);
+    rc = cfit_launch(ctx, K, d_rois, nullptr, n, nullptr, init_mode, d_init, init_mode == CF_INIT_ADD ? 3 * K - 1 : 3 * K + 2, d_theta,
+                     3 * K + 2, d_info, d_ll, (hipStream_t)stream);
     if (rc) return rc;
     return timer_end(ctx->timer, TM_CFIT, (hipStream_t)stream, ev, n);
 }
@@ -985,11 +609,73 @@ extern "C" int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int 
     double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
-    if ((rc = plain_multi_launch(ctx->p, ctx->tune, kmax, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
+    if ((rc = plain_multi_launch(ctx, kmax, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
     if ((rc = cselect_launch(ctx->p, kmax, d_rois, nullptr, n, nullptr, t1, kmax >= 2 ? t2 : nullptr, kmax >= 3 ? t3 : nullptr, l1,
                              l2, l3, d_loc, d_sel, s)))
         return rc;
     return timer_end(ctx->timer, TM_CFIT, s, ev, n);
+}
+
+extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int nsets, int kmax, void *stream)
+{
+    if (!ctx || !sets || nsets < 1 || nsets > 3 || kmax < 1 || kmax > 3) return fail(QCS_ERR_ARG, "invalid argument");
+    if (ctx->p.LocType != QCS_LOCTYPE_GS2D) return fail(QCS_ERR_ARG, "converged fitting supports the 2D model only");
+    size_t total = 0;
+    for (int i = 0; i < nsets; i++) {
+        const qcs_roi_set &x = sets[i];
+        if (x.roi_size != 7 && x.roi_size != 9 && x.roi_size != 11)
+            return fail(QCS_ERR_ARG, "set %d: ROI size %d not in {7, 9, 11}", i, x.roi_size);
+        if (x.n < 0 || (x.n > 0 && (!x.d_rois || !x.d_loc))) return fail(QCS_ERR_ARG, "set %d: invalid ROIs or rows", i);
+        total += (size_t)x.n;
+    }
+    if (total == 0) return QCS_OK;
+    if (total > (size_t)INT32_MAX / 32) return fail(QCS_ERR_ARG, "too many ROIs in one call");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if ((rc = ctx->cf_theta.ensure(total * (5 + 8 + 11) * sizeof(float))) || (rc = ctx->cf_ll.ensure(total * 3 * sizeof(double))))
+        return rc;
+    // per set: t1 | t2 | t3 (strides 5, 8, 11) and l1 | l2 | l3, each indexed by ROI
+    float *tb[3][3];
+    double *lb[3][3];
+    {
+        float *t = ctx->cf_theta.as<float>();
+        double *l = ctx->cf_ll.as<double>();
+        for (int i = 0; i < nsets; i++) {
+            const size_t m = (size_t)sets[i].n;
+            tb[i][0] = t;
+            tb[i][1] = t + m * 5;
+            tb[i][2] = t + m * 13;
+            t += m * 24;
+            for (int k = 0; k < 3; k++) lb[i][k] = l + m * k;
+            l += m * 3;
+        }
+    }
+    hipEvent_t ev;
+    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+    for (int K = 1; K <= kmax; K++) {
+        CfqSet xs[3];
+        memset(xs, 0, sizeof(xs));
+        for (int i = 0; i < nsets; i++) {
+            CfqSet &x = xs[i];
+            x.rois = sets[i].d_rois;
+            x.n = sets[i].n;
+            x.R = sets[i].roi_size;
+            x.init = K == 1 ? nullptr : tb[i][K - 2];
+            x.istride = K == 1 ? 0 : 3 * K - 1;
+            x.theta = tb[i][K - 1];
+            x.ostride = 3 * K + 2;
+            x.ll = lb[i][K - 1];
+        }
+        if ((rc = cfq_launch(ctx, K, K == 1 ? CF_INIT_HEUR : CF_INIT_ADD, xs, nsets, s))) return rc;
+    }
+    for (int i = 0; i < nsets; i++) {
+        qcs_params p = ctx->p;
+        p.ROISize = sets[i].roi_size;
+        if ((rc = cselect_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, nullptr, tb[i][0], kmax >= 2 ? tb[i][1] : nullptr,
+                                 kmax >= 3 ? tb[i][2] : nullptr, lb[i][0], lb[i][1], lb[i][2], sets[i].d_loc, sets[i].d_sel, s)))
+            return rc;
+    }
+    return timer_end(ctx->timer, TM_CFIT, s, ev, (long long)total);
 }
 
 extern "C" int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
@@ -1018,7 +704,7 @@ extern "C" int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float 
     float *t1 = ctx->cf_theta.as<float>(), *t2 = t1 + (size_t)n * 5, *t3 = t2 + (size_t)n * 8, *rows = t3 + (size_t)n * 11;
     double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
     int *sel = d_sel ? d_sel : ctx->cf_aux.as<int>();
-    if ((rc = plain_multi_launch(ctx->p, ctx->tune, 3, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
+    if ((rc = plain_multi_launch(ctx, 3, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
     if ((rc = cselect_launch(ctx->p, 3, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, rows, sel, s))) return rc;
     hipLaunchKernelGGL(label_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, t1, t2, t3, d_labels);
     QCS_HIP(hipGetLastError());
@@ -1038,6 +724,8 @@ extern "C" int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t)
     for (int k = 0; k < 3; k++)
         if (t->max_iter[k] < 1 || t->max_iter[k] > 1000) return fail(QCS_ERR_ARG, "max_iter must be in 1..1000");
     if (!(t->tol > 0.0f) || !(t->warm_pmin >= 0.0f && t->warm_pmin <= 1.0f)) return fail(QCS_ERR_ARG, "invalid tolerance or pmin");
+    if ((t->queue != 0 && t->queue != 1) || t->waves_per_cu < 0 || t->waves_per_cu > 64)
+        return fail(QCS_ERR_ARG, "queue must be 0 or 1 and waves_per_cu in 0..64");
     ctx->tune = *t;
     return QCS_OK;
 }

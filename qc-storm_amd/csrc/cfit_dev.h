@@ -1,0 +1,469 @@
+// cfit_dev.h -- device building blocks of the converged multi-emitter fitter (gfx950), shared by
+// the one-pass kernels (cfit.hip) and the persistent work-queue kernels (cfq.hip).
+//
+// Model, pixel (r, c) centred at (c + 0.5, r + 0.5):
+//
+//   mu = sum_k N_k / (2 pi s^2) exp(-((c + .5 - x_k)^2 + (r + .5 - y_k)^2) / (2 s^2)) + b
+//
+// with K = 1..3 emitters sharing the PSF width s; theta = [N_k, x_k, y_k]_k, s, b.  Per-lane
+// arrays in LDS are lane-interleaved (element i of lane l at [i * 64 + l]).
+#pragma once
+#include <math.h>
+#include "qcs_internal.h"
+
+namespace qcs {
+
+struct CFitConst {
+    float offset, kadc, inv_qe;
+    int max_iter;
+    float tol;          // convergence: max |step| in scaled units
+    int init_mode;      // CF_INIT_*
+    float sigma0;       // heuristic initial PSF width (px)
+};
+enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
+enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4 };
+
+template <int K> struct CP { static constexpr int P = 3 * K + 2, NS = P * (P + 1) / 2; };
+
+__device__ __forceinline__ float cf_scale(int i, int P)
+{
+    // N -> kilo-photons, b -> hundreds of photons, positions and width in pixels
+    return (i == P - 1) ? 100.0f : ((i < P - 2 && i % 3 == 0) ? 1000.0f : 1.0f);
+}
+
+// packed upper-triangular index of (i, j), i <= j
+__device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
+
+// loss sum((mu - q) - q ln(mu / q)) (the Poisson negative log-likelihood up to a constant) and,
+// if G, its gradient with respect to theta (physical units)
+template <int K, int R, bool G>
+__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<K>::P], float (&g)[CP<K>::P])
+{
+    constexpr int P = CP<K>::P;
+    const float s = th[3 * K], b = th[3 * K + 1];
+    const float is2 = 1.0f / (s * s);
+    const float h = -0.5f * is2;
+    const float nrm = 0.15915494309189535f * is2;   // 1 / (2 pi s^2)
+    float ex[K][R], dx[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            dx[k][c] = ((float)c + 0.5f) - th[3 * k + 1];
+            ex[k][c] = __expf(h * dx[k][c] * dx[k][c]);
+        }
+    float L = 0.0f;
+    float gN[K], gx[K], gy[K], gd2 = 0.0f, gw = 0.0f, gb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
+    // rows stay a loop: a full unroll lets the scheduler hoist every row's LDS loads and
+    // exponentials, which costs registers (occupancy) for no gain
+#pragma unroll 1
+    for (int r = 0; r < R; r++) {
+        float ay[K], dy[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
+            ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
+        }
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            float gk[K];
+            float mu = b;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                gk[k] = ay[k] * ex[k][c];
+                mu += gk[k];
+            }
+            mu = fmaxf(mu, 1e-6f);
+            const float q = qs[(r * R + c) * 64];
+            // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
+            L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
+            if (G) {
+                const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
+                gb += w;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const float wg = w * gk[k];
+                    gN[k] += wg;
+                    gx[k] = fmaf(wg, dx[k][c], gx[k]);
+                    gy[k] = fmaf(wg, dy[k], gy[k]);
+                    gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
+                    gw += wg;
+                }
+            }
+        }
+    }
+    if (G) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            g[3 * k] = gN[k] / fmaxf(th[3 * k], 1e-3f);
+            g[3 * k + 1] = gx[k] * is2;
+            g[3 * k + 2] = gy[k] * is2;
+        }
+        g[3 * K] = (gd2 * is2 - 2.0f * gw) / s;
+        g[3 * K + 1] = gb;
+        (void)P;
+    }
+    return L;
+}
+
+template <int K, int R>
+__device__ __forceinline__ void cf_project(float (&th)[CP<K>::P])
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        th[3 * k] = fmaxf(th[3 * k], 1.0f);
+        th[3 * k + 1] = fminf(fmaxf(th[3 * k + 1], -0.5f), (float)R + 0.5f);
+        th[3 * k + 2] = fminf(fmaxf(th[3 * k + 2], -0.5f), (float)R + 0.5f);
+    }
+    th[3 * K] = fminf(fmaxf(th[3 * K], 0.4f), 0.5f * (float)R);
+    th[3 * K + 1] = fmaxf(th[3 * K + 1], 0.01f);
+}
+
+// Inverse of the Fisher information sum grad(mu) grad(mu)^T / mu at theta, in scaled units,
+// into the packed symmetric D (the BFGS starting inverse Hessian).  Returns false when the
+// Cholesky factorisation fails (D untouched).
+template <int K, int R>
+__device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<K>::P], float *D)
+{
+    constexpr int P = CP<K>::P, NS = CP<K>::NS;
+    float F[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) F[i] = 0.0f;
+    const float s = th[3 * K], is2 = 1.0f / (s * s), h = -0.5f * is2, nrm = 0.15915494309189535f * is2;
+#pragma unroll 1
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            float J[P];
+            float mu = th[3 * K + 1], jw = 0.0f;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const float dx = ((float)c + 0.5f) - th[3 * k + 1], dy = ((float)r + 0.5f) - th[3 * k + 2];
+                const float d2 = fmaf(dx, dx, dy * dy);
+                const float mk = th[3 * k] * nrm * __expf(h * d2);
+                mu += mk;
+                J[3 * k] = mk / fmaxf(th[3 * k], 1e-3f) * cf_scale(3 * k, P);
+                J[3 * k + 1] = mk * dx * is2;
+                J[3 * k + 2] = mk * dy * is2;
+                jw = fmaf(mk, fmaf(d2, is2, -2.0f), jw);
+            }
+            J[3 * K] = jw / s;
+            J[3 * K + 1] = cf_scale(3 * K + 1, P);
+            const float im = __builtin_amdgcn_rcpf(fmaxf(mu, 1e-6f));
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                const float ja = J[a] * im;
+#pragma unroll
+                for (int b2 = a; b2 < P; b2++) F[tri(a, b2, P)] = fmaf(ja, J[b2], F[tri(a, b2, P)]);
+            }
+        }
+    }
+    // F = U^T U (U upper, in place)
+#pragma unroll
+    for (int j = 0; j < P; j++) {
+        float d = F[tri(j, j, P)];
+#pragma unroll
+        for (int k = 0; k < j; k++) d = fmaf(-F[tri(k, j, P)], F[tri(k, j, P)], d);
+        if (!(d > 0.0f)) return false;
+        const float u = sqrtf(d), iu = 1.0f / u;
+        F[tri(j, j, P)] = u;
+#pragma unroll
+        for (int i = j + 1; i < P; i++) {
+            float v = F[tri(j, i, P)];
+#pragma unroll
+            for (int k = 0; k < j; k++) v = fmaf(-F[tri(k, j, P)], F[tri(k, i, P)], v);
+            F[tri(j, i, P)] = v * iu;
+        }
+    }
+    // U^-1 in place, rows bottom-up, each row right to left
+#pragma unroll
+    for (int i = P - 1; i >= 0; i--) {
+        const float iu = 1.0f / F[tri(i, i, P)];
+#pragma unroll
+        for (int j = P - 1; j > i; j--) {
+            float v = 0.0f;
+#pragma unroll
+            for (int k = i + 1; k <= j; k++) v = fmaf(F[tri(i, k, P)], F[tri(k, j, P)], v);
+            F[tri(i, j, P)] = -v * iu;
+        }
+        F[tri(i, i, P)] = iu;
+    }
+    // F^-1 = U^-1 U^-T, in place row by row (entries (i, j), j >= i, use row i and row j from k >= j)
+#pragma unroll
+    for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int j = i; j < P; j++) {
+            float v = 0.0f;
+#pragma unroll
+            for (int k = j; k < P; k++) v = fmaf(F[tri(i, k, P)], F[tri(j, k, P)], v);
+            F[tri(i, j, P)] = v;
+        }
+#pragma unroll
+    for (int i = 0; i < NS; i++) D[i * 64] = F[i];
+    return true;
+}
+
+// FP64 log-likelihood sum(q ln mu - mu) (without the ln q! term, common to every model)
+template <int K, int R>
+__device__ double cf_loglik64(const float *qs, const float (&th)[CP<K>::P])
+{
+    const double s = th[3 * K], b = th[3 * K + 1];
+    const double h = -0.5 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+    double ll = 0.0;
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            double mu = b;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
+                mu += (double)th[3 * k] * nrm * exp(h * (dx * dx + dy * dy));
+            }
+            const double q = qs[(r * R + c) * 64];
+            ll += q * log(mu) - mu;
+        }
+    return ll;
+}
+
+// initial vector: heuristic (K = 1), given, or the previous model plus an emitter at the largest
+// positive residual (K >= 2; init holds the K-1 model's theta)
+template <int K, int R>
+__device__ __forceinline__ void cf_init(const float *qs, const float *init, int mode, float sigma0, float (&th)[CP<K>::P])
+{
+    constexpr int P = CP<K>::P;
+    if (mode == CF_INIT_GIVEN) {
+#pragma unroll
+        for (int i = 0; i < P; i++) th[i] = init[i];
+        return;
+    }
+    if constexpr (K > 1) {
+      if (mode == CF_INIT_ADD) {
+        constexpr int P0 = CP<K - 1>::P;
+        float t0[P0];
+#pragma unroll
+        for (int i = 0; i < P0; i++) t0[i] = init[i];
+        const float s = t0[3 * (K - 1)], b = t0[3 * (K - 1) + 1];
+        const float h = -0.5f / (s * s), nrm = 0.15915494309189535f / (s * s);
+        float best = -1e30f;
+        int bi = (R / 2) * R + R / 2;
+        for (int r = 0; r < R; r++)
+            for (int c = 0; c < R; c++) {
+                float mu = b;
+#pragma unroll
+                for (int k = 0; k < K - 1; k++) {
+                    const float dx = ((float)c + 0.5f) - t0[3 * k + 1], dy = ((float)r + 0.5f) - t0[3 * k + 2];
+                    mu += t0[3 * k] * nrm * __expf(h * (dx * dx + dy * dy));
+                }
+                const float res = qs[(r * R + c) * 64] - mu;
+                if (res > best) { best = res; bi = r * R + c; }
+            }
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) {
+            th[3 * k] = t0[3 * k];
+            th[3 * k + 1] = t0[3 * k + 1];
+            th[3 * k + 2] = t0[3 * k + 2];
+        }
+        th[3 * (K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * s * s;
+        th[3 * (K - 1) + 1] = (float)(bi % R) + 0.5f;
+        th[3 * (K - 1) + 2] = (float)(bi / R) + 0.5f;
+        th[3 * K] = s;
+        th[3 * K + 1] = b;
+        return;
+      }
+    }
+    // heuristic: background from the smaller of the opposite edge means, photons from the excess,
+    // position at the excess-weighted centroid (single emitter; extra emitters share it)
+    float e[4] = {0, 0, 0, 0};
+    for (int i = 0; i < R; i++) {
+        e[0] += qs[(0 * R + i) * 64];
+        e[1] += qs[((R - 1) * R + i) * 64];
+        e[2] += qs[(i * R + 0) * 64];
+        e[3] += qs[(i * R + R - 1) * 64];
+    }
+    const float b = 0.5f * (fminf(e[0], e[1]) + fminf(e[2], e[3])) / (float)R;
+    float tot = 0.0f, sx = 0.0f, sy = 0.0f;
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            const float v = fmaxf(qs[(r * R + c) * 64] - b, 0.0f);
+            tot += v;
+            sx = fmaf(v, (float)c + 0.5f, sx);
+            sy = fmaf(v, (float)r + 0.5f, sy);
+        }
+    const float cx = tot > 0.0f ? sx / tot : 0.5f * R, cy = tot > 0.0f ? sy / tot : 0.5f * R;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        th[3 * k] = fmaxf(tot, 50.0f) / (float)K;
+        th[3 * k + 1] = cx + 0.3f * (float)k;
+        th[3 * k + 2] = cy;
+    }
+    th[3 * K] = sigma0;
+    th[3 * K + 1] = fmaxf(b, 0.5f);
+}
+
+// ---- FP64 full-Fisher CRLB of one model (standard deviations of x_k and y_k, px) ----------
+// I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed in LDS per lane; Cholesky; then
+// (I^-1)_ii = |L^-1 e_i|^2 for the position indices.
+template <int K, int R>
+__device__ void cf_crlb64(const float (&th)[CP<K>::P], double *I, float (&cx)[K], float (&cy)[K])
+{
+    constexpr int P = CP<K>::P;
+    for (int i = 0; i < CP<K>::NS; i++) I[i * 64] = 0.0;
+    const double s = th[3 * K], is2 = 1.0 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+    for (int r = 0; r < R; r++)
+        for (int c = 0; c < R; c++) {
+            double J[P];
+            double mu = th[3 * K + 1];
+            double jw = 0.0;
+            for (int k = 0; k < K; k++) {
+                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
+                const double d2 = dx * dx + dy * dy;
+                const double mk = (double)th[3 * k] * nrm * exp(-0.5 * d2 * is2);
+                mu += mk;
+                J[3 * k] = mk / (double)th[3 * k];
+                J[3 * k + 1] = mk * dx * is2;
+                J[3 * k + 2] = mk * dy * is2;
+                jw += mk * (d2 * is2 - 2.0) / s;
+            }
+            J[3 * K] = jw;
+            J[3 * K + 1] = 1.0;
+            const double im = 1.0 / mu;
+            for (int a = 0; a < P; a++) {
+                const double ja = J[a] * im;
+                for (int b2 = a; b2 < P; b2++) I[tri(a, b2, P) * 64] += ja * J[b2];
+            }
+        }
+    // in-place Cholesky of the packed upper triangle: I = U^T U
+    bool ok = true;
+    for (int j = 0; j < P; j++) {
+        double d = I[tri(j, j, P) * 64];
+        for (int k = 0; k < j; k++) d -= I[tri(k, j, P) * 64] * I[tri(k, j, P) * 64];
+        if (!(d > 0.0)) { ok = false; break; }
+        const double u = sqrt(d);
+        I[tri(j, j, P) * 64] = u;
+        for (int i = j + 1; i < P; i++) {
+            double v = I[tri(j, i, P) * 64];
+            for (int k = 0; k < j; k++) v -= I[tri(k, j, P) * 64] * I[tri(k, i, P) * 64];
+            I[tri(j, i, P) * 64] = v / u;
+        }
+    }
+    for (int k = 0; k < K; k++) {
+        for (int w = 0; w < 2; w++) {
+            const int e = 3 * k + 1 + w;
+            double var = -1.0;
+            if (ok) {
+                // (I^-1)_ee = |U^-T e_e|^2: forward substitution U^T z = e_e
+                double z[P];
+                for (int i = 0; i < P; i++) {
+                    double v = (i == e) ? 1.0 : 0.0;
+                    for (int k2 = 0; k2 < i; k2++) v -= I[tri(k2, i, P) * 64] * z[k2];
+                    z[i] = v / I[tri(i, i, P) * 64];
+                }
+                var = 0.0;
+                for (int i = 0; i < P; i++) var += z[i] * z[i];
+            }
+            const float sd = var > 0.0 ? (float)sqrt(var) : -1.0f;
+            if (w == 0) cx[k] = sd; else cy[k] = sd;
+        }
+    }
+}
+
+struct CSelConst {
+    float llr_half;     // accept K+1 over K when ll(K+1) - ll(K) exceeds this (half the chi^2 bound)
+    float n_min;        // minimum photons of every emitter of an accepted model
+    float pixel_size, qe;
+    int kmax;
+};
+
+template <int K, int R>
+__device__ __forceinline__ bool cf_valid(const float *t, float n_min)
+{
+    for (int k = 0; k < K; k++) {
+        if (!(t[3 * k] >= n_min)) return false;
+        if (!(t[3 * k + 1] > 0.0f && t[3 * k + 1] < (float)R && t[3 * k + 2] > 0.0f && t[3 * k + 2] < (float)R)) return false;
+    }
+    return t[3 * K] > 0.45f && t[3 * K] < 0.45f * (float)R;
+}
+
+template <int K, int R>
+__device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, double *I, float *rows)
+{
+    float th[CP<K>::P];
+    for (int i = 0; i < CP<K>::P; i++) th[i] = t[i];
+    float cx[K], cy[K];
+    cf_crlb64<K, R>(th, I, cx, cy);
+    const float s = th[3 * K], b = th[3 * K + 1];
+    const int half = R / 2;
+    const float xc = (float)hdr[0], yc = (float)hdr[1];
+    const int fr = (int)hdr[2] + (int)hdr[3] * 65536;
+    for (int k = 0; k < K; k++) {
+        float *o = rows + (size_t)k * QCS_REC_FLOATS;
+        const float peak = th[3 * k] * 0.15915494309189535f / (s * s);
+        o[QCS_POS_PPHO] = peak;
+        o[QCS_POS_XPOS] = (th[3 * k + 1] - (float)half) + xc;
+        o[QCS_POS_YPOS] = (th[3 * k + 2] - (float)half) + yc;
+        o[QCS_POS_ZPOS] = 0.0f;
+        o[QCS_POS_SIGX] = s;
+        o[QCS_POS_SIGY] = s;
+        o[QCS_POS_TPHO] = th[3 * k];
+        o[QCS_POS_BAKG] = b;
+        o[QCS_POS_PSNR] = sqrtf(c.qe) * peak / sqrtf(peak + b);
+        o[QCS_POS_CRBX] = cx[k] > 0.0f ? cx[k] * c.pixel_size : -1.0f;
+        o[QCS_POS_CRBY] = cy[k] > 0.0f ? cy[k] * c.pixel_size : -1.0f;
+        o[QCS_POS_FRME] = (float)fr;
+    }
+}
+
+
+// Position of this lane's item in a list that the whole wave appends to (one atomic per wave),
+// -1 if the lane takes no slot.  Every lane of the wave must make the call.
+__device__ __forceinline__ int wave_append(int *count, bool take)
+{
+    const unsigned long long m = __ballot(take);
+    if (m == 0ull) return -1;
+    const int lane = threadIdx.x & 63, leader = __ffsll(m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return take ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// ---- host side ----------------------------------------------------------------------------
+
+// one set of ROIs of one size in a converged-fitting launch.  Work item j fits ROI
+// roi = idx ? idx[j] : j (j < n, or < *n_dev when given); init (stride istride), theta (physical
+// units, stride ostride), info (4 ints: iterations, flags, loss evaluations, 0) and ll (FP64
+// log-likelihood, may be null) are indexed by roi.
+struct CfqSet {
+    const uint16_t *rois;
+    const int *idx;
+    const int *n_dev;
+    const float *init;
+    float *theta;
+    int *info;
+    double *ll;
+    float *rec;      // work-item records of the persistent path (set by the launcher)
+    int n, R, istride, ostride;
+};
+
+// arguments of one persistent launch (up to three ROI sizes, each with its own queue head)
+struct CfqArgs {
+    CfqSet set[3];
+    int nsets;
+    int *heads;
+    CFitConst c;
+};
+
+CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_mode);
+// persistent work-queue fit of emitter count K over the sets (cfq.hip)
+int cfq_launch(qcs_ctx *ctx, int K, int init_mode, const CfqSet *sets, int nsets, hipStream_t s);
+
+#define CF_DISPATCH_R(R_, ...)                                                            \
+    switch (R_) {                                                                          \
+    case 7: { constexpr int RT = 7; __VA_ARGS__; } break;                                  \
+    case 9: { constexpr int RT = 9; __VA_ARGS__; } break;                                  \
+    case 11: { constexpr int RT = 11; __VA_ARGS__; } break;                                \
+    default: return fail(QCS_ERR_ARG, "converged fitting supports ROISize 7, 9, 11 (got %d)", R_); \
+    }
+
+}  // namespace qcs

@@ -1,0 +1,450 @@
+// cfq.hip -- persistent work-queue kernels of the converged multi-emitter fitter (gfx950).
+//
+// The one-pass kernel (cfit_kernel) keeps a wave busy until the slowest of its 64 fits ends.
+// The iteration counts of multi-emitter fits are heavy-tailed (most converge in 5-10 BFGS steps,
+// a few run to the cap), so most lanes of such a wave idle most of the time.  Here each lane runs
+// its own fit as a small state machine and takes the next ROI from a device queue as soon as
+// its fit ends:
+//
+//   prep pass (one lane per work item, uniform work): observed image, starting vector, loss and
+//     gradient there, inverse-Fisher starting inverse Hessian -> one record per item in HBM;
+//   persistent pass (as many single-wave work-groups as the chip holds at once): per tick, lanes
+//     without work take items from the queue (one atomic per wave), lanes holding a fresh
+//     gradient form the quasi-Newton direction (or finish), and every lane in a line search
+//     evaluates loss and gradient at its trial point -- the one heavy, uniform step of the tick.
+//     The gradient of an accepted trial is the gradient of the next iteration, so an iteration
+//     costs one evaluation where the one-pass kernel spends two;
+//   likelihood pass (one lane per work item): FP64 log-likelihood at the final parameters.
+//
+// A launch holds up to three sets of ROIs (sizes 7, 9, 11, each with its own queue): every wave
+// drains the sets in order with its LDS sized for the largest.  Each step's arithmetic is that
+// of the one-pass kernel, so both give bit-identical parameters, iteration counts and flags (the
+// evaluation counts differ by the saved gradient evaluations).
+#include <string.h>
+#include "cfit_dev.h"
+
+namespace qcs {
+
+// record of one work item: theta, loss, gradient, "identity start" flag, packed D0, padding
+template <int K> struct CQ {
+    static constexpr int P = CP<K>::P, NS = CP<K>::NS;
+    static constexpr int OL = P, OG = P + 1, OF = 2 * P + 1, OD = 2 * P + 2;
+    static constexpr int W = (2 * P + 2 + NS + 3) & ~3;   // floats, a multiple of 4
+};
+
+enum { CQ_IDLE = 0, CQ_DIR = 1, CQ_LS = 2 };
+
+// observed photons of ROI roi into the lane's LDS column: 4-byte loads (every ROI record starts
+// 4-byte aligned since R (R + 1) is even)
+template <int R>
+__device__ __forceinline__ void cq_load_q(const uint16_t *rois, int roi, const CFitConst &c, float *qs)
+{
+    constexpr int RR = R * R, WR = R * (R + 1);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(rois + (size_t)roi * WR);
+#pragma unroll
+    for (int w = 0; w < (RR + 1) / 2; w++) {
+        const uint32_t u = src[w];
+        qs[(2 * w) * 64] = fmaxf(((float)(u & 0xffffu) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
+        if (2 * w + 1 < RR) qs[(2 * w + 1) * 64] = fmaxf(((float)(u >> 16) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
+    }
+}
+
+template <int K, int R>
+__global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
+{
+    using Q = CQ<K>;
+    constexpr int P = Q::P, NS = Q::NS;
+    __shared__ float qsh[R * R * 64];
+    __shared__ float Dsh[NS * 64];
+    const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
+    const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
+    if (j >= n) return;   // no cross-lane work here
+    const int roi = S.idx ? S.idx[j] : j;
+    float *qs = qsh + lane, *D = Dsh + lane;
+    cq_load_q<R>(S.rois, roi, c, qs);
+    float th[P], g[P];
+    cf_init<K, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? c.init_mode : CF_INIT_HEUR, c.sigma0, th);
+    cf_project<K, R>(th);
+    const float L = cf_lossgrad<K, R, true>(qs, th, g);
+    const bool first = !cf_fisher_inverse<K, R>(th, D);
+    if (first) {
+#pragma unroll
+        for (int i = 0; i < P; i++)
+#pragma unroll
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+    }
+    float4 *rec = reinterpret_cast<float4 *>(S.rec + (size_t)j * Q::W);
+#pragma unroll
+    for (int v = 0; v < Q::W / 4; v++) {
+        float e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = 4 * v + u;
+            e[u] = i < P ? th[i < P ? i : 0]
+                 : i == Q::OL ? L
+                 : i < Q::OF ? g[(i > Q::OL && i < Q::OF) ? i - Q::OG : 0]
+                 : i == Q::OF ? (first ? 1.0f : 0.0f)
+                 : i < Q::OD + NS ? D[((i >= Q::OD && i < Q::OD + NS) ? i - Q::OD : 0) * 64] : 0.0f;
+        }
+        rec[v] = make_float4(e[0], e[1], e[2], e[3]);
+    }
+}
+
+template <int K, int R>
+__device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitConst &c, float *qsh, float *Dsh)
+{
+    using Q = CQ<K>;
+    constexpr int P = Q::P;
+    const int lane = threadIdx.x;
+    float *qs = qsh + lane, *D = Dsh + lane;
+    const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
+    int st = CQ_IDLE, roi = 0, iters = 0, flags = 0, evals = 0, t = 0;
+    bool first = false, dry = n <= 0;
+    float th[P], g[P], du[P], L = 0.0f, slope = 0.0f, alpha = 1.0f, dmax = 0.0f;
+#pragma unroll
+    for (int i = 0; i < P; i++) th[i] = g[i] = du[i] = 0.0f;
+
+    auto finish = [&]() {
+        float *o = S.theta + (size_t)roi * S.ostride;
+#pragma unroll
+        for (int i = 0; i < P; i++) o[i] = th[i];
+        if (S.info) {
+            int *inf = S.info + (size_t)roi * 4;
+            inf[0] = iters;
+            inf[1] = flags;
+            inf[2] = evals;
+            inf[3] = 0;
+        }
+        st = CQ_IDLE;
+    };
+    // lanes without work take the next items of the queue (one atomic per wave)
+    auto refill = [&]() {
+        const bool want = st == CQ_IDLE;
+        const unsigned long long m = __ballot(want);
+        if (m == 0ull || dry) return;   // wave-uniform
+        const int cnt = __popcll(m), leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(head, cnt);
+        base = __shfl(base, leader, 64);
+        if (base + cnt >= n) dry = true;   // the queue holds nothing more
+        if (!want) return;
+        const int j = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (j >= n) return;
+        roi = S.idx ? S.idx[j] : j;
+        cq_load_q<R>(S.rois, roi, c, qs);
+        const float4 *rec = reinterpret_cast<const float4 *>(S.rec + (size_t)j * Q::W);
+#pragma unroll
+        for (int v = 0; v < Q::W / 4; v++) {
+            const float4 x = rec[v];
+            const float e[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = 4 * v + u;
+                if (i < P) th[i < P ? i : 0] = e[u];
+                else if (i == Q::OL) L = e[u];
+                else if (i < Q::OF) g[(i > Q::OL && i < Q::OF) ? i - Q::OG : 0] = e[u];
+                else if (i == Q::OF) first = e[u] != 0.0f;
+                else if (i < Q::OD + Q::NS) D[((i >= Q::OD && i < Q::OD + Q::NS) ? i - Q::OD : 0) * 64] = e[u];
+            }
+        }
+        iters = 0;
+        flags = 0;
+        evals = 1;
+        st = CQ_DIR;
+    };
+    // quasi-Newton direction from a fresh gradient; converged fits end here
+    auto direction = [&]() {
+        if (st != CQ_DIR) return;
+        float gu[P];
+#pragma unroll
+        for (int i = 0; i < P; i++) gu[i] = g[i] * cf_scale(i, P);
+        slope = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
+            du[i] = -a;
+            slope = fmaf(du[i], gu[i], slope);
+        }
+        {
+            float dm = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) dm = fmaxf(dm, fabsf(du[i]));
+            if (dm < c.tol && slope < 0.0f) {
+                flags |= CF_CONVERGED;
+                finish();
+                return;
+            }
+        }
+        if (!(slope < 0.0f)) {   // not a descent direction: restart from the gradient
+#pragma unroll
+            for (int i = 0; i < P; i++) du[i] = -gu[i];
+            slope = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) slope = fmaf(du[i], gu[i], slope);
+#pragma unroll
+            for (int i = 0; i < P; i++)
+#pragma unroll
+                for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+            first = true;
+        }
+        if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
+            float m = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) m = fmaxf(m, fabsf(du[i]));
+            if (m > 0.1f) {
+                const float f = 0.1f / m;
+                slope = 0.0f;
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    du[i] *= f;
+                    slope = fmaf(du[i], gu[i], slope);
+                }
+            }
+        }
+        dmax = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
+        alpha = 1.0f;
+        t = 0;
+        st = CQ_LS;
+    };
+
+    while (true) {
+        refill();
+        direction();
+        refill();      // lanes whose fit converged just now
+        direction();
+        if (dry && __all(st == CQ_IDLE)) break;
+        if (st != CQ_LS) continue;
+        float tn[P], gt[P];
+#pragma unroll
+        for (int i = 0; i < P; i++) tn[i] = fmaf(alpha, du[i] * cf_scale(i, P), th[i]);
+        cf_project<K, R>(tn);
+        const float Lt = cf_lossgrad<K, R, true>(qs, tn, gt);
+        evals++;
+        if (Lt == Lt && Lt < L && Lt <= fmaf(1e-4f * alpha, slope, L)) {
+            // accepted: BFGS update of the inverse Hessian with s = step, y = gradient change
+            iters++;
+            float s[P], y[P], sy = 0.0f, yy = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) s[i] = (tn[i] - th[i]) / cf_scale(i, P);
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                th[i] = tn[i];
+                y[i] = (gt[i] - g[i]) * cf_scale(i, P);
+                sy = fmaf(s[i], y[i], sy);
+                yy = fmaf(y[i], y[i], yy);
+                g[i] = gt[i];
+            }
+            L = Lt;
+            if (sy > 1e-12f) {
+                if (first) {
+                    const float sc = sy / yy;
+#pragma unroll
+                    for (int i = 0; i < P; i++)
+#pragma unroll
+                        for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
+                    first = false;
+                }
+                // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
+                const float rho = 1.0f / sy;
+                float u[P], yu = 0.0f;
+#pragma unroll
+                for (int i = 0; i < P; i++) {
+                    float a = 0.0f;
+#pragma unroll
+                    for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
+                    u[i] = a;
+                    yu = fmaf(y[i], a, yu);
+                }
+                const float cs = fmaf(rho * rho, yu, rho);
+#pragma unroll
+                for (int i = 0; i < P; i++)
+#pragma unroll
+                    for (int jj = i; jj < P; jj++) {
+                        float *e = &D[tri(i, jj, P) * 64];
+                        *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
+                    }
+            }
+            if (iters >= c.max_iter) {
+                flags |= CF_MAXITER;
+                finish();
+            } else {
+                st = CQ_DIR;
+            }
+        } else if (t == 0 && !(Lt < L) && dmax < 20.0f * c.tol) {
+            // the full step is tiny and the FP32 loss cannot resolve it: take it and stop
+            iters++;
+#pragma unroll
+            for (int i = 0; i < P; i++) th[i] = tn[i];
+            flags |= CF_CONVERGED;
+            finish();
+        } else {
+            alpha *= 0.3f;
+            if (++t >= 16) {   // no decrease in 16 backtracks
+                iters++;
+                flags |= CF_STALLED;
+                finish();
+            }
+        }
+    }
+}
+
+template <int K, int RMAX>
+__global__ void __launch_bounds__(64) cfq_kernel(CfqArgs a)
+{
+    __shared__ float qsh[RMAX * RMAX * 64];
+    __shared__ float Dsh[CP<K>::NS * 64];
+    for (int si = 0; si < a.nsets; si++) {
+        const int R = a.set[si].R;
+        if constexpr (RMAX >= 11) {
+            if (R == 11) cfq_run<K, 11>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+        }
+        if constexpr (RMAX >= 9) {
+            if (R == 9) cfq_run<K, 9>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+        }
+        if (R == 7) cfq_run<K, 7>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+    }
+}
+
+template <int K, int R>
+__global__ void __launch_bounds__(64) cfq_ll_kernel(CfqSet S, CFitConst c)
+{
+    __shared__ float qsh[R * R * 64];
+    const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
+    const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
+    if (j >= n) return;
+    const int roi = S.idx ? S.idx[j] : j;
+    float *qs = qsh + lane;
+    cq_load_q<R>(S.rois, roi, c, qs);
+    float th[CP<K>::P];
+#pragma unroll
+    for (int i = 0; i < CP<K>::P; i++) th[i] = S.theta[(size_t)roi * S.ostride + i];
+    S.ll[roi] = cf_loglik64<K, R>(qs, th);
+}
+
+// ---- host -----------------------------------------------------------------------------------
+
+static int cq_rec_w(int K) { return K == 1 ? CQ<1>::W : (K == 2 ? CQ<2>::W : CQ<3>::W); }
+
+// resident work-groups per CU of kernel f
+static int cq_occupancy(const void *f)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+template <int K, int RMAX> static const void *cq_kernel_ptr() { return reinterpret_cast<const void *>(&cfq_kernel<K, RMAX>); }
+
+static int cq_waves_per_cu(int K, int RMAX)
+{
+    static int cache[3][3] = {{0}};
+    const int ri = RMAX == 7 ? 0 : (RMAX == 9 ? 1 : 2);
+    int &v = cache[K - 1][ri];
+    if (v == 0) {
+        const void *f = nullptr;
+        switch (K * 100 + RMAX) {
+        case 107: f = cq_kernel_ptr<1, 7>(); break;
+        case 109: f = cq_kernel_ptr<1, 9>(); break;
+        case 111: f = cq_kernel_ptr<1, 11>(); break;
+        case 207: f = cq_kernel_ptr<2, 7>(); break;
+        case 209: f = cq_kernel_ptr<2, 9>(); break;
+        case 211: f = cq_kernel_ptr<2, 11>(); break;
+        case 307: f = cq_kernel_ptr<3, 7>(); break;
+        case 309: f = cq_kernel_ptr<3, 9>(); break;
+        default: f = cq_kernel_ptr<3, 11>(); break;
+        }
+        v = cq_occupancy(f);
+    }
+    return v;
+}
+
+int cfq_launch(qcs_ctx *ctx, int K, int init_mode, const CfqSet *sets_in, int nsets, hipStream_t s)
+{
+    if (K < 1 || K > 3) return fail(QCS_ERR_ARG, "emitter count %d not in 1..3", K);
+    if (nsets < 1 || nsets > 3) return fail(QCS_ERR_ARG, "1..3 ROI sets per launch");
+    CfqArgs a;
+    memset(&a, 0, sizeof(a));
+    a.c = cfit_const(ctx->p, ctx->tune, K, init_mode);
+    const int W = cq_rec_w(K);
+    size_t total = 0;
+    int rmax = 0;
+    long long items = 0;
+    for (int i = 0; i < nsets; i++) {
+        const CfqSet &x = sets_in[i];
+        if (x.R != 7 && x.R != 9 && x.R != 11) return fail(QCS_ERR_ARG, "converged fitting supports ROISize 7, 9, 11 (got %d)", x.R);
+        if (x.n < 0 || (x.n > 0 && (!x.rois || !x.theta))) return fail(QCS_ERR_ARG, "invalid ROI set");
+        if (init_mode != CF_INIT_HEUR && x.n > 0 && !x.init) return fail(QCS_ERR_ARG, "this start needs an initial vector");
+        total += (size_t)x.n * W;
+        items += x.n;
+        rmax = x.R > rmax ? x.R : rmax;
+    }
+    if (items == 0) return QCS_OK;
+    int rc;
+    if ((rc = ctx->cf_queue.ensure(256 + total * sizeof(float)))) return rc;
+    int *heads = ctx->cf_queue.as<int>();
+    float *rec = reinterpret_cast<float *>(ctx->cf_queue.as<char>() + 256);
+    // largest ROIs first: the longest fits start early and the tail stays short
+    int order[3] = {0, 1, 2};
+    for (int i = 0; i < nsets; i++)
+        for (int j2 = i + 1; j2 < nsets; j2++)
+            if (sets_in[order[j2]].R > sets_in[order[i]].R) { const int x = order[i]; order[i] = order[j2]; order[j2] = x; }
+    size_t off = 0;
+    a.nsets = 0;
+    for (int i = 0; i < nsets; i++) {
+        const CfqSet &x = sets_in[order[i]];
+        if (x.n == 0) continue;
+        CfqSet &y = a.set[a.nsets++];
+        y = x;
+        y.rec = rec + off;
+        off += (size_t)x.n * W;
+    }
+    a.heads = heads;
+    fill_async(heads, 4, 0, s);
+    for (int i = 0; i < a.nsets; i++) {
+        const CfqSet &x = a.set[i];
+        const dim3 grid((x.n + 63) / 64), block(64);
+        if (K == 1) {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<1, RT>), grid, block, 0, s, x, a.c))
+        } else if (K == 2) {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<2, RT>), grid, block, 0, s, x, a.c))
+        } else {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<3, RT>), grid, block, 0, s, x, a.c))
+        }
+    }
+    QCS_HIP(hipGetLastError());
+    int cus = 0;
+    QCS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    int wpc = cq_waves_per_cu(K, rmax);
+    if (ctx->tune.waves_per_cu > 0 && ctx->tune.waves_per_cu < wpc) wpc = ctx->tune.waves_per_cu;
+    long long waves = (items + 63) / 64;
+    if (waves > (long long)cus * wpc) waves = (long long)cus * wpc;
+    const dim3 pgrid((unsigned)waves), block(64);
+    if (K == 1) {
+        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<1, RT>), pgrid, block, 0, s, a))
+    } else if (K == 2) {
+        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<2, RT>), pgrid, block, 0, s, a))
+    } else {
+        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<3, RT>), pgrid, block, 0, s, a))
+    }
+    QCS_HIP(hipGetLastError());
+    for (int i = 0; i < a.nsets; i++) {
+        const CfqSet &x = a.set[i];
+        if (!x.ll) continue;
+        const dim3 grid((x.n + 63) / 64);
+        if (K == 1) {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<1, RT>), grid, block, 0, s, x, a.c))
+        } else if (K == 2) {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<2, RT>), grid, block, 0, s, x, a.c))
+        } else {
+            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<3, RT>), grid, block, 0, s, x, a.c))
+        }
+    }
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+}  // namespace qcs

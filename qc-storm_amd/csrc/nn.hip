@@ -256,6 +256,167 @@ __global__ void adam_kernel(float *__restrict__ w, const float *__restrict__ g, 
     w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
+// ---- fused inference ------------------------------------------------------------------------
+// One wave computes the whole network for 16 ROIs.  Every layer is evaluated transposed,
+// D^T[out channel][roi] = W[out][k] . act^T[k][roi]: the weights are the MFMA A operand (8-byte
+// fragments straight from L2) and the activations the B operand.  An MFMA result then already has
+// the B-operand layout of the next layer (lane l holds channels 4 (l / 16) .. +3 of ROI l % 16),
+// so bias, ReLU and the bf16 rounding are applied in registers and no activation touches LDS or
+// HBM.  Per output the k-chunks are accumulated in the order of the layer-by-layer GEMMs, with
+// the same bf16 rounding points.
+__device__ __forceinline__ bf16x4_t act4(const f32x4_t &d, const float *bias)
+{
+    bf16x4_t v;
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[i] = (short)f2bf(fmaxf(d[i] + bias[i], 0.0f));
+    return v;
+}
+
+__device__ __forceinline__ bf16x4_t wfrag(const uint16_t *W, int row, int ld, int k)
+{
+    return *reinterpret_cast<const bf16x4_t *>(W + (size_t)row * ld + k);
+}
+
+__device__ __forceinline__ f32x4_t mfma(const bf16x4_t &a, const bf16x4_t &b, const f32x4_t &c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(64) fwd_fused_kernel(const uint16_t *__restrict__ rois, int n, float offset, float kadc,
+                                                       float inv_qe, const uint16_t *__restrict__ W, const float *__restrict__ wf,
+                                                       float *__restrict__ out, float *__restrict__ norm)
+{
+    __shared__ uint16_t xs[16][52];   // normalised input, bf16 (49 used)
+    __shared__ uint16_t sides[16][2];
+    __shared__ bf16x4_t c1s[3][5][2][64];   // conv1 tiles of three input rows, in B-operand layout
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int base = blockIdx.x * 16;
+    if (lane < 16) {   // input normalisation of ROI base + lane (as prep_kernel)
+        const int i = base + lane;
+        if (i < n) {
+            const uint16_t *src = rois + (size_t)i * NR * (NR + 1);
+            float q[NRR];
+            float mx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < NRR; k++) {
+                q[k] = fmaxf(((float)src[k] - offset) * kadc, 1.0f) * inv_qe;
+                mx = fmaxf(mx, q[k]);
+            }
+            float e0 = 0, e1 = 0, e2 = 0, e3 = 0;
+#pragma unroll
+            for (int k = 0; k < NR; k++) {
+                e0 += q[k];
+                e1 += q[(NR - 1) * NR + k];
+                e2 += q[k * NR];
+                e3 += q[k * NR + NR - 1];
+            }
+            const float bh = 0.5f * (fminf(e0, e1) + fminf(e2, e3)) / (float)NR;
+            const float scale = fmaxf(mx - bh, 0.0f) + 1.0f, is = 1.0f / scale;
+            if (norm) {
+                norm[2 * i] = scale;
+                norm[2 * i + 1] = bh;
+            }
+#pragma unroll
+            for (int k = 0; k < NRR; k++) xs[lane][k] = f2bf((q[k] - bh) * is);
+            sides[lane][0] = f2bf(logf(scale) * 0.1f);
+            sides[lane][1] = f2bf(bh * is);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NRR; k++) xs[lane][k] = 0;
+            sides[lane][0] = sides[lane][1] = 0;
+        }
+    }
+    __syncthreads();
+    const float *b1 = wf + OB1, *b2 = wf + OB2, *b3 = wf + OB3, *b4 = wf + OB4, *b5 = wf + OB5;
+    const f32x4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x4_t w1[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) w1[t] = wfrag(W + OW1, t * 16 + r, K1, 4 * g);
+    f32x4_t acc3[4] = {zero, zero, zero, zero};
+#pragma unroll 1
+    for (int py = 0; py < 3; py++) {
+        // conv1 at input rows py .. py + 2 (all 5 columns, both 16-channel tiles); each lane
+        // reads back only what it wrote, so no barrier is needed
+#pragma unroll
+        for (int yy = 0; yy < 3; yy++)
+#pragma unroll
+            for (int xx = 0; xx < 5; xx++) {
+                bf16x4_t xb;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int t = 4 * g + e;   // tap; 9..15 are padding
+                    const int ty = t / 3, tx = t % 3;
+                    xb[e] = t < 9 ? (short)xs[r][(py + yy + ty) * NR + xx + tx] : (short)0;
+                }
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) c1s[yy][xx][ct][lane] = act4(mfma(w1[ct], xb, zero), b1 + ct * 16 + 4 * g);
+            }
+        // conv2 at the three outputs of row py; k = tap * 32 + channel, ascending
+        f32x4_t acc2[3][4];
+#pragma unroll
+        for (int px = 0; px < 3; px++)
+#pragma unroll
+            for (int ot = 0; ot < 4; ot++) acc2[px][ot] = zero;
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int ty = tap / 3, tx = tap % 3;
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++) {
+                const int k0 = tap * C1 + ct * 16 + 4 * g;
+                bf16x4_t wa[4];
+#pragma unroll
+                for (int ot = 0; ot < 4; ot++) wa[ot] = wfrag(W + OW2, ot * 16 + r, K2, k0);
+#pragma unroll
+                for (int px = 0; px < 3; px++)
+#pragma unroll
+                    for (int ot = 0; ot < 4; ot++) acc2[px][ot] = mfma(wa[ot], c1s[ty][px + tx][ct][lane], acc2[px][ot]);
+            }
+        }
+        // conv3 over these positions; k = position * 64 + channel, ascending
+#pragma unroll
+        for (int px = 0; px < 3; px++) {
+            const int p2 = py * 3 + px;
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+                const bf16x4_t a = act4(acc2[px][ct], b2 + ct * 16 + 4 * g);
+                const int k0 = p2 * C2 + ct * 16 + 4 * g;
+#pragma unroll
+                for (int ot = 0; ot < 4; ot++) acc3[ot] = mfma(wfrag(W + OW3, ot * 16 + r, K3, k0), a, acc3[ot]);
+            }
+        }
+    }
+    // fc1 over [conv3 features, side features, zero padding]
+    f32x4_t acc4[4] = {zero, zero, zero, zero};
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const bf16x4_t a = act4(acc3[ct], b3 + ct * 16 + 4 * g);
+#pragma unroll
+        for (int ot = 0; ot < 4; ot++) acc4[ot] = mfma(wfrag(W + OW4, ot * 16 + r, F0, ct * 16 + 4 * g), a, acc4[ot]);
+    }
+    {
+        bf16x4_t sb = {0, 0, 0, 0};
+        if (g == 0) {
+            sb[0] = (short)sides[r][0];
+            sb[1] = (short)sides[r][1];
+        }
+#pragma unroll
+        for (int ot = 0; ot < 4; ot++) acc4[ot] = mfma(wfrag(W + OW4, ot * 16 + r, F0, C3 + 4 * g), sb, acc4[ot]);
+    }
+    // fc2
+    f32x4_t acc5 = zero;
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc5 = mfma(wfrag(W + OW5, r, F1, ct * 16 + 4 * g), act4(acc4[ct], b4 + ct * 16 + 4 * g), acc5);
+    const int i = base + r;
+    if (i < n) {
+        float4 o;
+        o.x = acc5[0] + b5[4 * g + 0];
+        o.y = acc5[1] + b5[4 * g + 1];
+        o.z = acc5[2] + b5[4 * g + 2];
+        o.w = acc5[3] + b5[4 * g + 3];
+        *reinterpret_cast<float4 *>(out + (size_t)i * NO + 4 * g) = o;
+    }
+}
+
 }  // namespace nnk
 
 void nn_gemm_nt(const uint16_t *A, const uint16_t *B, float *C, int M, int N, int K, int lda, int ldb, int ldc, bool acc,
@@ -523,6 +684,19 @@ int qcs_net_set_params(qcs_net *nt, const float *w)
 
 int qcs_net_forward_dev(qcs_net *nt, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
                         void *stream)
+{
+    if (!nt || !p || n < 0 || (n > 0 && (!d_rois || !d_out))) return fail(QCS_ERR_ARG, "invalid argument");
+    if (p->ROISize != NR) return fail(QCS_ERR_ARG, "the warm-start network takes 7x7 ROIs");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) return QCS_OK;
+    hipLaunchKernelGGL(fwd_fused_kernel, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, s, d_rois, n, p->Offset, p->KAdc, 1.0f / p->QE,
+                       nt->wb.as<uint16_t>(), nt->w.as<float>(), d_out, d_norm);
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+int qcs_net_forward_layers_dev(qcs_net *nt, const qcs_params *p, const uint16_t *d_rois, int n, float *d_out, float *d_norm,
+                               void *stream)
 {
     if (!nt || !p || n < 0 || (n > 0 && (!d_rois || !d_out))) return fail(QCS_ERR_ARG, "invalid argument");
     if (p->ROISize != NR) return fail(QCS_ERR_ARG, "the warm-start network takes 7x7 ROIs");

@@ -69,7 +69,13 @@ class QcsFitStats(C.Structure):
 
 
 class QcsTuning(C.Structure):
-    _fields_ = [("max_iter", C.c_int * 3), ("tol", C.c_float), ("warm_pmin", C.c_float)]
+    _fields_ = [("max_iter", C.c_int * 3), ("tol", C.c_float), ("warm_pmin", C.c_float), ("queue", C.c_int),
+                ("waves_per_cu", C.c_int)]
+
+
+class QcsRoiSet(C.Structure):
+    _fields_ = [("roi_size", C.c_int), ("n", C.c_int), ("d_rois", C.c_void_p), ("d_loc", C.c_void_p),
+                ("d_sel", C.c_void_p)]
 
 
 class QcsTiming(C.Structure):
@@ -108,6 +114,7 @@ PROTOTYPES = {
     "qcs_fit_rois": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i)]),
     "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
     "qcs_tuning_get": (_i, [_vp, _P(QcsTuning)]),
     "qcs_tuning_set": (_i, [_vp, _P(QcsTuning)]),
     "qcs_net_param_count": (_i, []),
@@ -116,6 +123,7 @@ PROTOTYPES = {
     "qcs_net_get_params": (_i, [_vp, _vp]),
     "qcs_net_set_params": (_i, [_vp, _vp]),
     "qcs_net_forward_dev": (_i, [_vp, _P(QcsParams), _vp, _i, _vp, _vp, _vp]),
+    "qcs_net_forward_layers_dev": (_i, [_vp, _P(QcsParams), _vp, _i, _vp, _vp, _vp]),
     "qcs_net_grad_dev": (_i, [_vp, _P(QcsParams), _vp, _vp, _i, _vp, _P(_f), _vp]),
     "qcs_net_train_step_dev": (_i, [_vp, _P(QcsParams), _vp, _vp, _i, _f, _P(_f), _vp]),
     "qcs_label_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
@@ -307,12 +315,20 @@ class Context:
         check(lib().qcs_fit_mle_dev(self.h, self._dp(d_rois), n, kmax, self._dp(d_loc), self._dp(d_sel), stream),
               "qcs_fit_mle_dev")
 
+    def fit_mle_mixed_dev(self, sets, kmax, stream=None):
+        """sets: list of (roi_size, d_rois, n, d_loc, d_sel or None)."""
+        arr = (QcsRoiSet * len(sets))()
+        for i, (R, d_rois, n, d_loc, d_sel) in enumerate(sets):
+            arr[i] = QcsRoiSet(R, n, d_rois.data_ptr(), d_loc.data_ptr(), d_sel.data_ptr() if d_sel is not None else None)
+        check(lib().qcs_fit_mle_mixed_dev(self.h, arr, len(sets), kmax, stream), "qcs_fit_mle_mixed_dev")
+
     def tuning(self):
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
-        return {"max_iter": list(t.max_iter), "tol": t.tol, "warm_pmin": t.warm_pmin}
+        return {"max_iter": list(t.max_iter), "tol": t.tol, "warm_pmin": t.warm_pmin, "queue": t.queue,
+                "waves_per_cu": t.waves_per_cu}
 
-    def set_tuning(self, max_iter=None, tol=None, warm_pmin=None):
+    def set_tuning(self, max_iter=None, tol=None, warm_pmin=None, queue=None, waves_per_cu=None):
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
         if max_iter is not None:
@@ -322,6 +338,10 @@ class Context:
             t.tol = tol
         if warm_pmin is not None:
             t.warm_pmin = warm_pmin
+        if queue is not None:
+            t.queue = int(queue)
+        if waves_per_cu is not None:
+            t.waves_per_cu = int(waves_per_cu)
         check(lib().qcs_tuning_set(self.h, C.byref(t)), "qcs_tuning_set")
 
     def label_dev(self, d_rois, n, d_labels, d_sel=None, stream=None):
@@ -470,11 +490,11 @@ class Net:
         assert w.size == self.nparam
         check(lib().qcs_net_set_params(self.h, _ptr(w)), "qcs_net_set_params")
 
-    def forward_dev(self, params, d_rois, n, d_out, d_norm=None, stream=None):
-        check(lib().qcs_net_forward_dev(self.h, C.byref(params), C.c_void_p(d_rois.data_ptr()), n,
-                                        C.c_void_p(d_out.data_ptr()),
-                                        C.c_void_p(d_norm.data_ptr()) if d_norm is not None else None, stream),
-              "qcs_net_forward_dev")
+    def forward_dev(self, params, d_rois, n, d_out, d_norm=None, stream=None, layers=False):
+        """Fused inference kernel, or (layers=True) the training path's layer-by-layer GEMMs."""
+        fn = lib().qcs_net_forward_layers_dev if layers else lib().qcs_net_forward_dev
+        check(fn(self.h, C.byref(params), C.c_void_p(d_rois.data_ptr()), n, C.c_void_p(d_out.data_ptr()),
+                 C.c_void_p(d_norm.data_ptr()) if d_norm is not None else None, stream), "qcs_net_forward_dev")
 
     def grad_dev(self, params, d_rois, d_labels, n, stream=None):
         g = np.zeros(self.nparam, np.float32)

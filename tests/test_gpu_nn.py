@@ -85,6 +85,26 @@ def test_forward_and_gradient_match_reference(gpu):
         assert rel < 3e-2 and cos > 0.999
 
 
+@pytest.mark.parametrize("n", [1, 1003, 70001])
+def test_fused_forward_matches_layer_by_layer(gpu, n):
+    """The fused inference kernel (transposed MFMA chain, activations in registers) against the
+    training path's layer-by-layer GEMMs: same k-chunk order and rounding points."""
+    d_rois, _ = _sim(n, 21 + n)
+    p = _params()
+    net = Q.Net(seed=9)
+    o_f = torch.zeros((n, 16), dtype=torch.float32, device="cuda")
+    o_l = torch.zeros((n, 16), dtype=torch.float32, device="cuda")
+    m_f = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+    m_l = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+    net.forward_dev(p, d_rois, n, o_f, m_f)
+    net.forward_dev(p, d_rois, n, o_l, m_l, layers=True)
+    torch.cuda.synchronize()
+    a, b = o_f.cpu().numpy(), o_l.cpu().numpy()
+    print("n", n, "max |fused - layers|", np.abs(a - b).max(), "bitwise rows", (a.view(np.uint32) == b.view(np.uint32)).all(1).mean())
+    np.testing.assert_array_equal(m_f.cpu().numpy().view(np.uint32), m_l.cpu().numpy().view(np.uint32))
+    np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.fixture(scope="module")
 def trained():
     """A network trained on 2^23 labelled simulator ROIs (2048 Adam steps of 4096, cosine decay)."""
