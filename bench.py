@@ -151,7 +151,7 @@ def multi_emitter_leg(n, seed, log):
             "warm_speedup": plain_ms / warm_ms, "nn_ms": tw["nn_ms"], "plain_ms": plain_ms, "warm_ms": warm_ms,
             "plain": rp, "warm": rw, "rmse_ratio_warm_vs_plain": rw["rmse_px"] / rp["rmse_px"],
             "rmse_over_crlb_plain": rp["rmse_over_crlb"], "rmse_over_crlb_warm": rw["rmse_over_crlb"],
-            "fallback_fraction": st["fallbacks"] / n, "warm_iterations_mean": st["warm_iters"] / n,
+            "fallback_fraction": st["fallbacks"] / n, "warm_iterations_mean": st["iters"] / n,
             "weights": wsrc, "accuracy_sample_rois": m}
 
 

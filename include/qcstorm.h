@@ -235,6 +235,7 @@ int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float 
 #define QCS_CFIT_CONVERGED  1   /* info[1] flags */
 #define QCS_CFIT_MAXITER    2
 #define QCS_CFIT_STALLED    4
+#define QCS_CFIT_FLAT       8   /* converged by ftol (set together with QCS_CFIT_CONVERGED) */
 /* Fit model K to every ROI.  theta (n x (3K+2)): [N_k photons, x_k, y_k px (pixel edges)]_k, s px,
  * b photons/px.  info (n x 4, may be NULL): iterations, flags, loss evaluations, 0.  ll (n, may be
  * NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
@@ -242,21 +243,30 @@ int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, int init_mo
                  float *d_theta, int *d_info, double *d_ll, void *stream);
 /* Tuning of the converged fits and of the warm start (per context). */
 typedef struct qcs_tuning {
-    int   max_iter[3];      /* BFGS iteration caps for K = 1, 2, 3 (default 100, 100, 100) */
+    int   max_iter[3];      /* BFGS iteration caps for K = 1, 2, 3 (default 100, 40, 40) */
     float tol;              /* convergence: quasi-Newton step below tol in every scaled coordinate (1e-4) */
     float warm_pmin;        /* warm start: minimum predicted-count probability, else plain path (0.8) */
     int   queue;            /* 1: persistent work-queue kernels (default); 0: one-pass kernels, a
                              * wave per 64 ROIs (same results) */
     int   waves_per_cu;     /* persistent waves per CU; 0 = as many as fit (default) */
+    float ftol;             /* multi-emitter fits (K >= 2) also converge when an accepted step
+                             * lowers the negative log-likelihood by less than ftol (1e-3; 0 = off) */
+    int   active_set;       /* 1: parameters held at a box bound with an outward gradient are left
+                             * out of the quasi-Newton step (default 0) */
 } qcs_tuning;
 int qcs_tuning_get(qcs_ctx *ctx, qcs_tuning *out);
 int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t);
-/* The plain multi-emitter path: fits K = 1, 2, 3 in sequence (each started from the previous
- * one plus an emitter at the largest residual), picks the emitter count by the FP64
- * log-likelihood ratio (chi^2(3) at p = 1e-3) among valid models (every emitter >= 50 photons
- * inside the ROI), and writes 3 rows per ROI (rows 3j..3j+2, unused rows zero) with the FP64
- * full-Fisher CRLB in CRLBx/CRLBy.  d_sel (n, may be NULL) receives the chosen count. */
+/* The plain multi-emitter path: the emitter count follows the FP64 log-likelihood-ratio rule
+ * (chi^2(3) at p = 1e-3) among valid models (every emitter >= 50 photons inside the ROI): K = 1
+ * if its fit is valid, K = 2 if valid and (K = 0 or ll2 - ll1 > 8.14), K = 3 if K = 2 and valid
+ * and ll3 - ll2 > 8.14.  K = 1 starts from the centroid heuristic, K = 2 from the K = 1 fit plus
+ * an emitter at the largest residual, K = 3 likewise from K = 2 and only for ROIs at K = 2 (the
+ * only ones the rule can move to 3).  3 rows per ROI (rows 3j..3j+2, unused rows zero) with the
+ * FP64 full-Fisher CRLB in CRLBx/CRLBy.  d_sel (n, may be NULL) receives the chosen count. */
 int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, void *stream);
+/* qcs_fit_mle_dev, then waits and fills h_stats (9 long longs) as qcs_fit_warm_dev does */
+int qcs_fit_plain_stats_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel, long long *h_stats,
+                            void *stream);
 /* One set of same-size ROIs in a mixed-size call. */
 typedef struct qcs_roi_set {
     int roi_size;             /* 7, 9 or 11 */
@@ -306,12 +316,17 @@ int qcs_net_train_step_dev(qcs_net *net, const qcs_params *p, const uint16_t *d_
 /* labels from the plain path (qcs_fit_mle_dev's fits and model selection); d_sel (n, may be
  * NULL) receives the selected counts */
 int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_labels, int *d_sel, void *stream);
-/* The warm-started path: network -> per-count work lists -> converged fit of the predicted count
- * from the predicted start -> fallback to the plain path (qcs_fit_mle_dev) for every ROI whose
- * warm fit did not converge, is invalid, or ends at a loss above that of the plain heuristic
- * start.  Rows as qcs_fit_mle_dev.  h_stats (9 long longs, may be NULL; the call then waits):
- * ROIs per final count [3], fallbacks, warm iterations, warm loss evaluations, ROIs per
- * predicted count [3]. */
+/* The warm-started path (7x7 ROIs): the selection rule of qcs_fit_mle_dev with the network
+ * choosing starts and pruning fits.  Every one-emitter fit starts from the network's brightest
+ * emitter; a count the network predicts with probability >= warm_pmin caps the ROI's count
+ * (predicted singles skip the multi-emitter fits, predicted doubles the three-emitter fit); the
+ * others get the two-emitter fit from the network's two brightest emitters (where it predicts
+ * two or more) or from the plain start, and K = 3 runs as on the plain path.  A warm
+ * fit that does not converge or ends at a higher loss than its plain start (the heuristic, or
+ * the K = 1 fit plus an emitter at the largest residual) is redone from that plain start.  Rows
+ * as qcs_fit_mle_dev.  h_stats (9 long longs, may be NULL; the call then waits): ROIs per final
+ * count [3], fits redone from the plain start, iterations, loss evaluations (all fits), ROIs
+ * whose count the network capped, ROIs the network put at 2 and at 3 emitters. */
 int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
                      long long *h_stats, void *stream);
 /* building block: C[M][N] (fp32) = A[M][K] B[N][K]^T, bf16 inputs, K a multiple of 16 */

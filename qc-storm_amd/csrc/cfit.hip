@@ -76,57 +76,13 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
     bool done = !valid;
     while (true) {
         if (__all(done)) break;   // wave-uniform
-        // direction in scaled units: du = -D gu, gu = g * scale
-        float gu[P], du[P];
+        float du[P], slope = 0.0f, dmax = 0.0f;
 #pragma unroll
-        for (int i = 0; i < P; i++) gu[i] = g[i] * cf_scale(i, P);
-        float slope = 0.0f;
-#pragma unroll
-        for (int i = 0; i < P; i++) {
-            float a = 0.0f;
-#pragma unroll
-            for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
-            du[i] = -a;
-            slope = fmaf(du[i], gu[i], slope);
+        for (int i = 0; i < P; i++) du[i] = 0.0f;
+        if (!done && cf_direction<K, R>(th, g, D, first, c.tol, c.active_set != 0, du, slope, dmax)) {
+            flags |= CF_CONVERGED;
+            done = true;
         }
-        if (!done) {
-            float dmax = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
-            if (dmax < c.tol && slope < 0.0f) {   // the full quasi-Newton step is below tolerance
-                flags |= CF_CONVERGED;
-                done = true;
-            }
-        }
-        if (!(slope < 0.0f)) {   // not a descent direction: restart from the gradient
-#pragma unroll
-            for (int i = 0; i < P; i++) du[i] = -gu[i];
-            slope = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) slope = fmaf(du[i], gu[i], slope);
-#pragma unroll
-            for (int i = 0; i < P; i++)
-#pragma unroll
-                for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
-            first = true;
-        }
-        if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
-            float m = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) m = fmaxf(m, fabsf(du[i]));
-            if (m > 0.1f) {
-                const float f = 0.1f / m;
-                slope = 0.0f;
-#pragma unroll
-                for (int i = 0; i < P; i++) {
-                    du[i] *= f;
-                    slope = fmaf(du[i], gu[i], slope);
-                }
-            }
-        }
-        float dmax = 0.0f;
-#pragma unroll
-        for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
         // Armijo backtracking (every active lane evaluates each trial together)
         float alpha = 1.0f, tn[P], Ln = L;
         bool floor_step = false;
@@ -166,57 +122,25 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
         }
         if (!done) {
             iters++;
-            bool moved = accepted && Ln < L;
-            float s[P];
-#pragma unroll
-            for (int i = 0; i < P; i++) s[i] = moved ? (tn[i] - th[i]) / cf_scale(i, P) : 0.0f;
+            const bool moved = accepted && Ln < L;
             if (!moved) {
                 flags |= CF_STALLED;
                 done = true;
             } else {
                 float gnew[P];
-#pragma unroll
-                for (int i = 0; i < P; i++) th[i] = tn[i];
-                L = cf_lossgrad<K, R, true>(qs, th, gnew);
+                const float Lold = L;
+                L = cf_lossgrad<K, R, true>(qs, tn, gnew);
                 evals++;
-                float y[P], sy = 0.0f, yy = 0.0f;
+                cf_bfgs_update<K>(D, first, th, tn, g, gnew);
 #pragma unroll
                 for (int i = 0; i < P; i++) {
-                    y[i] = (gnew[i] - g[i]) * cf_scale(i, P);
-                    sy = fmaf(s[i], y[i], sy);
-                    yy = fmaf(y[i], y[i], yy);
+                    th[i] = tn[i];
                     g[i] = gnew[i];
                 }
-                if (sy > 1e-12f) {
-                    if (first) {
-                        const float sc = sy / yy;
-#pragma unroll
-                        for (int i = 0; i < P; i++)
-#pragma unroll
-                            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
-                        first = false;
-                    }
-                    // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
-                    const float rho = 1.0f / sy;
-                    float u[P], yu = 0.0f;
-#pragma unroll
-                    for (int i = 0; i < P; i++) {
-                        float a = 0.0f;
-#pragma unroll
-                        for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
-                        u[i] = a;
-                        yu = fmaf(y[i], a, yu);
-                    }
-                    const float cs = fmaf(rho * rho, yu, rho);
-#pragma unroll
-                    for (int i = 0; i < P; i++)
-#pragma unroll
-                        for (int jj = i; jj < P; jj++) {
-                            float* e = &D[tri(i, jj, P) * 64];
-                            *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
-                        }
-                }
-                if (iters >= c.max_iter) {
+                if (Lold - L < c.ftol) {   // the likelihood has stopped moving
+                    flags |= CF_CONVERGED | CF_FLAT;
+                    done = true;
+                } else if (iters >= c.max_iter) {
                     flags |= CF_MAXITER;
                     done = true;
                 }
@@ -235,38 +159,6 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
     if (ll_out) ll_out[roi] = cf_loglik64<K, R>(qs, th);
 }
 
-// Model selection (FP64 log-likelihood ratio) and rows: ROI j gets rows 3j .. 3j+2, the unused
-// ones zero.  sel[j] receives the chosen emitter count (0: no valid model).
-template <int R>
-__global__ void __launch_bounds__(64) cf_select_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
-                                                       const int *__restrict__ n_dev, const float *__restrict__ t1,
-                                                       const float *__restrict__ t2,
-                                                       const float *__restrict__ t3, const double *__restrict__ l1,
-                                                       const double *__restrict__ l2, const double *__restrict__ l3,
-                                                       CSelConst c, float *__restrict__ rows, int *__restrict__ sel)
-{
-    __shared__ double Ish[CP<3>::NS * 64];
-    const int lane = threadIdx.x;
-    const int j = blockIdx.x * 64 + lane;
-    if (n_dev) n = *n_dev;
-    if (j >= n) return;
-    const int roi = idx ? idx[j] : j;
-    const uint16_t *hdr = rois + (size_t)roi * R * (R + 1) + R * R;
-    float *o = rows + (size_t)roi * 3 * QCS_REC_FLOATS;
-    for (int i = 0; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
-    int K = 0;
-    if (cf_valid<1, R>(t1 + (size_t)roi * 5, c.n_min)) K = 1;
-    if (c.kmax >= 2 && t2 && cf_valid<2, R>(t2 + (size_t)roi * 8, c.n_min) && (K == 0 || l2[roi] - l1[roi] > (double)c.llr_half))
-        K = 2;
-    if (c.kmax >= 3 && t3 && K == 2 && cf_valid<3, R>(t3 + (size_t)roi * 11, c.n_min) && l3[roi] - l2[roi] > (double)c.llr_half)
-        K = 3;
-    if (sel) sel[roi] = K;
-    double *I = Ish + lane;
-    if (K == 1) cf_write_rows<1, R>(t1 + (size_t)roi * 5, hdr, c, I, o);
-    else if (K == 2) cf_write_rows<2, R>(t2 + (size_t)roi * 8, hdr, c, I, o);
-    else if (K == 3) cf_write_rows<3, R>(t3 + (size_t)roi * 11, hdr, c, I, o);
-}
-
 CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_mode)
 {
     CFitConst c;
@@ -277,6 +169,8 @@ CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_m
     c.tol = t.tol;
     c.init_mode = init_mode;
     c.sigma0 = 1.3f;
+    c.ftol = K >= 2 ? t.ftol : 0.0f;   // one-emitter fits always run to the step tolerance
+    c.active_set = t.active_set;
     return c;
 }
 
@@ -300,7 +194,8 @@ int cfit_launch(qcs_ctx *ctx, int K, const uint16_t *d_rois, const int *d_idx, i
         x.R = p.ROISize;
         x.istride = istride;
         x.ostride = ostride;
-        return cfq_launch(ctx, K, init_mode, &x, 1, s);
+        x.init_mode = init_mode;
+        return cfq_launch(ctx, K, &x, 1, s);
     }
     const CFitConst c = cfit_const(p, ctx->tune, K, init_mode);
     const dim3 grid((n + 63) / 64), block(64);
@@ -318,18 +213,12 @@ int cfit_launch(qcs_ctx *ctx, int K, const uint16_t *d_rois, const int *d_idx, i
     return QCS_OK;
 }
 
-// K = 1, 2, 3 in sequence over the listed ROIs (or all n); t1/t2/t3 (strides 5/8/11) and l1..l3
-// are indexed by ROI
-int plain_multi_launch(qcs_ctx *ctx, int kmax, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n, float *t1,
-                       float *t2, float *t3, double *l1, double *l2, double *l3, hipStream_t s)
-{
-    int rc;
-    if ((rc = cfit_launch(ctx, 1, d_rois, d_idx, n, d_n, CF_INIT_HEUR, nullptr, 0, t1, 5, nullptr, l1, s))) return rc;
-    if (kmax >= 2 && (rc = cfit_launch(ctx, 2, d_rois, d_idx, n, d_n, CF_INIT_ADD, t1, 5, t2, 8, nullptr, l2, s))) return rc;
-    if (kmax >= 3 && (rc = cfit_launch(ctx, 3, d_rois, d_idx, n, d_n, CF_INIT_ADD, t2, 8, t3, 11, nullptr, l3, s))) return rc;
-    return QCS_OK;
-}
-
+// ---- staged model selection --------------------------------------------------------------------
+// The emitter count follows the likelihood-ratio rule: K = 1 if the one-emitter fit is valid;
+// K = 2 if the two-emitter fit is valid and (K = 0 or ll2 - ll1 > llr_half); K = 3 if K = 2, the
+// three-emitter fit is valid and ll3 - ll2 > llr_half (valid: every emitter >= n_min photons
+// inside the ROI, a plausible width).  K = 3 is reachable only from K = 2, so the three-emitter
+// fit runs only for the ROIs the first stage leaves at K = 2; their list is built on the device.
 CSelConst csel_const(const qcs_params &p, int kmax)
 {
     CSelConst c;
@@ -341,125 +230,197 @@ CSelConst csel_const(const qcs_params &p, int kmax)
     return c;
 }
 
-int cselect_launch(const qcs_params &p, int kmax, const uint16_t *d_rois, const int *d_idx, int n, const int *d_n, const float *t1,
-                   const float *t2, const float *t3, const double *l1, const double *l2, const double *l3, float *d_rows,
-                   int *d_sel, hipStream_t s)
+// stage 1: K in {0, 1, 2} into sel[roi]; kcap (may be null) caps the count per ROI (a cap of 1:
+// the two-emitter fit was not run).  ROIs at K = 2 with kmax and cap >= 3 are appended to list3.
+template <int R>
+__global__ void __launch_bounds__(64) cf_select12_kernel(const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
+                                                         const float *__restrict__ t1, const float *__restrict__ t2,
+                                                         const double *__restrict__ l1, const double *__restrict__ l2,
+                                                         const int *__restrict__ kcap, CSelConst c, int *__restrict__ sel,
+                                                         int *__restrict__ cnt3, int *__restrict__ list3)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (n_dev) n = min(*n_dev, n);
+    if ((int)(blockIdx.x * 64) >= n) return;   // whole wave beyond the count
+    int K = -1, roi = 0, cap = 3;
+    if (j < n) {
+        roi = idx ? idx[j] : j;
+        if (kcap) cap = kcap[roi];
+        K = 0;
+        if (cf_valid<1, R>(t1 + (size_t)roi * 5, c.n_min)) K = 1;
+        const bool have2 = c.kmax >= 2 && t2 && cap >= 2;
+        if (have2 && cf_valid<2, R>(t2 + (size_t)roi * 8, c.n_min) && (K == 0 || l2[roi] - l1[roi] > (double)c.llr_half)) K = 2;
+        sel[roi] = K;
+    }
+    if (c.kmax >= 3) {
+        const int pos = wave_append(cnt3, K == 2 && cap >= 3);
+        if (pos >= 0) list3[pos] = roi;
+    }
+}
+
+// stage 2: K = 3 where it beats K = 2, final count into sel[roi]; rows 3 roi .. 3 roi + 2 (unused
+// ones zero) with the FP64 full-Fisher CRLB
+template <int R>
+__global__ void __launch_bounds__(64) cf_rows_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
+                                                     const int *__restrict__ n_dev, const float *__restrict__ t1,
+                                                     const float *__restrict__ t2, const float *__restrict__ t3,
+                                                     const double *__restrict__ l2, const double *__restrict__ l3, CSelConst c,
+                                                     int *__restrict__ sel, float *__restrict__ rows)
+{
+    __shared__ double Ish[CP<3>::NS * 64];
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x * 64 + lane;
+    if (n_dev) n = min(*n_dev, n);
+    if (j >= n) return;
+    const int roi = idx ? idx[j] : j;
+    const uint16_t *hdr = rois + (size_t)roi * R * (R + 1) + R * R;
+    float *o = rows + (size_t)roi * 3 * QCS_REC_FLOATS;
+    for (int i = 0; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
+    int K = sel[roi];
+    if (K == 2 && c.kmax >= 3 && t3 && cf_valid<3, R>(t3 + (size_t)roi * 11, c.n_min) && l3[roi] - l2[roi] > (double)c.llr_half)
+        K = 3;
+    sel[roi] = K;
+    double *I = Ish + lane;
+    if (K == 1) cf_write_rows<1, R>(t1 + (size_t)roi * 5, hdr, c, I, o);
+    else if (K == 2) cf_write_rows<2, R>(t2 + (size_t)roi * 8, hdr, c, I, o);
+    else if (K == 3) cf_write_rows<3, R>(t3 + (size_t)roi * 11, hdr, c, I, o);
+}
+
+int select12_launch(const qcs_params &p, int kmax, const int *idx, int n, const int *d_n, const float *t1, const float *t2,
+                    const double *l1, const double *l2, const int *kcap, int *sel, int *cnt3, int *list3, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
     const CSelConst c = csel_const(p, kmax);
     const dim3 grid((n + 63) / 64), block(64);
-    CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_select_kernel<RT>), grid, block, 0, s, d_rois, d_idx, n, d_n, t1, t2, t3, l1,
-                                                 l2, l3, c, d_rows, d_sel))
+    CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_select12_kernel<RT>), grid, block, 0, s, idx, n, d_n, t1, kmax >= 2 ? t2 : nullptr,
+                                                 l1, l2, kcap, c, sel, cnt3, list3))
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
 
+int rows_launch(const qcs_params &p, int kmax, const uint16_t *rois, const int *idx, int n, const int *d_n, const float *t1,
+                const float *t2, const float *t3, const double *l2, const double *l3, int *sel, float *rows, hipStream_t s)
+{
+    if (n <= 0) return QCS_OK;
+    const CSelConst c = csel_const(p, kmax);
+    const dim3 grid((n + 63) / 64), block(64);
+    CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_rows_kernel<RT>), grid, block, 0, s, rois, idx, n, d_n, t1, t2,
+                                                 kmax >= 3 ? t3 : nullptr, l2, l3, c, sel, rows))
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
 
-// ---- learned warm start -------------------------------------------------------------------
-// Network outputs [n][16] (nn.hip): 3 count logits, (x/R, y/R, N/(2 pi scale)) for 3 slots,
-// s - 1.3, (b - bhat)/scale.  The decode kernel turns them into a starting vector for the
-// predicted emitter count and appends the ROI to that count's work list.
+// The plain path over the listed ROIs (or all n): K = 1 from the heuristic start, K = 2 from the
+// K = 1 fit plus an emitter at the largest residual, stage 1, K = 3 (likewise from K = 2) for the
+// ROIs at K = 2, stage 2 and rows.  Fit vectors t1/t2/t3 (strides 5/8/11), l1..l3, sel and the
+// optional info arrays are indexed by ROI; cnt3 is one device int, list3 holds n ints.
+int plain_path(qcs_ctx *ctx, int kmax, const uint16_t *rois, const int *idx, int n, const int *d_n, float *t1, float *t2,
+               float *t3, double *l1, double *l2, double *l3, int *sel, int *cnt3, int *list3, float *rows, int *info1, int *info2,
+               int *info3, hipStream_t s)
+{
+    int rc;
+    if ((rc = cfit_launch(ctx, 1, rois, idx, n, d_n, CF_INIT_HEUR, nullptr, 0, t1, 5, info1, l1, s))) return rc;
+    if (kmax >= 2 && (rc = cfit_launch(ctx, 2, rois, idx, n, d_n, CF_INIT_ADD, t1, 5, t2, 8, info2, l2, s))) return rc;
+    fill_async(cnt3, 1, 0, s);
+    if ((rc = select12_launch(ctx->p, kmax, idx, n, d_n, t1, t2, l1, l2, nullptr, sel, cnt3, list3, s))) return rc;
+    if (kmax >= 3 && (rc = cfit_launch(ctx, 3, rois, list3, n, cnt3, CF_INIT_ADD, t2, 8, t3, 11, info3, l3, s))) return rc;
+    return rows_launch(ctx->p, kmax, rois, idx, n, d_n, t1, t2, t3, l2, l3, sel, rows, s);
+}
+
+// ---- learned warm start -----------------------------------------------------------------------
+// Network outputs [n][16] (nn.hip): 3 count logits, (x/R, y/R, N/(2 pi scale)) for 3 slots
+// (brightest first), s - 1.3, (b - bhat)/scale.  The warm path keeps the selection rule of the
+// plain path and changes where the fits start and which fits run:
+//   - every ROI's one-emitter fit starts from the brightest slot;
+//   - a count predicted with probability >= pmin caps the ROI's count: predicted single ROIs skip
+//     the multi-emitter fits, predicted doubles the three-emitter fit;
+//   - the others get the two-emitter fit, started from the network's two brightest slots where it
+//     predicts two or more emitters and from the plain start otherwise;
+//   - the three-emitter fit runs, from the plain start, for the uncapped ROIs stage 1 leaves at
+//     K = 2.
+// A warm fit that does not converge, or ends at a higher loss than its plain start has, is redone
+// from the plain start (the fallback).
+enum { WC_L2W = 0, WC_L2A = 1, WC_F1 = 2, WC_F2 = 3, WC_L3 = 4, WC_N = 16 };
+
 __global__ void warm_decode_kernel(const float *__restrict__ out, const float *__restrict__ norm, int n, int R, float pmin,
-                                   float *__restrict__ init11, int *__restrict__ khat, int *__restrict__ cnt,
-                                   int *__restrict__ lists, int *__restrict__ fb_list, int *__restrict__ info)
+                                   float *__restrict__ tw1, float *__restrict__ tw2, int *__restrict__ cs, int *__restrict__ khat,
+                                   int *__restrict__ cnt, int *__restrict__ l2w, int *__restrict__ l2a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    int cls = -1;   // 0..2: warm list of count cls + 1; 3: fallback list
+    int cls = -1;   // 0: two-emitter fit from the network start; 1: from the plain start
     if (j < n) {
-        // the warm fits write info only for the ROIs they take
-        *reinterpret_cast<int4 *>(info + (size_t)j * 4) = make_int4(0, 0, 0, 0);
         const float *o = out + (size_t)j * 16;
         int K = 1;
         if (o[1] > o[0]) K = 2;
         if (o[2] > o[K - 1]) K = 3;
-        // probability of the predicted count; below pmin the ROI goes straight to the plain path
         const float m = o[K - 1];
         const float pk = 1.0f / (__expf(o[0] - m) + __expf(o[1] - m) + __expf(o[2] - m));
-        if (pk >= pmin) {
-            const float scale = norm[2 * j], bh = norm[2 * j + 1];
-            float *t = init11 + (size_t)j * 11;
-            for (int k = 0; k < K; k++) {
-                t[3 * k] = fmaxf(o[5 + 3 * k] * 6.283185307179586f * scale, 30.0f);
-                t[3 * k + 1] = fminf(fmaxf(o[3 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
-                t[3 * k + 2] = fminf(fmaxf(o[4 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+        const float scale = norm[2 * j], bh = norm[2 * j + 1];
+        const float s0 = fminf(fmaxf(o[12] + 1.3f, 0.6f), 3.0f), b0 = fmaxf(bh + o[13] * scale, 0.5f);
+        float slot[2][3];
+        for (int k = 0; k < 2; k++) {
+            slot[k][0] = fmaxf(o[5 + 3 * k] * 6.283185307179586f * scale, 30.0f);
+            slot[k][1] = fminf(fmaxf(o[3 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+            slot[k][2] = fminf(fmaxf(o[4 + 3 * k] * (float)R, 0.3f), (float)R - 0.3f);
+        }
+        float *t = tw1 + (size_t)j * 5;
+        t[0] = slot[0][0];
+        t[1] = slot[0][1];
+        t[2] = slot[0][2];
+        t[3] = s0;
+        t[4] = b0;
+        khat[j] = K;
+        const bool single = K == 1 && pk >= pmin;
+        cs[j] = pk >= pmin ? K : 3;   // per-ROI count cap
+        if (!single) {
+            if (K >= 2) {
+                float *u = tw2 + (size_t)j * 8;
+                for (int k = 0; k < 2; k++)
+                    for (int e = 0; e < 3; e++) u[3 * k + e] = slot[k][e];
+                u[6] = s0;
+                u[7] = b0;
+                cls = 0;
+            } else {
+                cls = 1;
             }
-            t[3 * K] = fminf(fmaxf(o[12] + 1.3f, 0.6f), 3.0f);
-            t[3 * K + 1] = fmaxf(bh + o[13] * scale, 0.5f);
-            khat[j] = K;
-            cls = K - 1;
-        } else {
-            khat[j] = 0;
-            cls = 3;
         }
     }
-    for (int c = 0; c < 4; c++) {
-        const int pos = wave_append(&cnt[c], cls == c);
-        if (pos >= 0) {
-            if (c == 3) fb_list[pos] = j;
-            else lists[(size_t)c * n + pos] = j;
-        }
-    }
+    int pos = wave_append(&cnt[WC_L2W], cls == 0);
+    if (pos >= 0) l2w[pos] = j;
+    pos = wave_append(&cnt[WC_L2A], cls == 1);
+    if (pos >= 0) l2a[pos] = j;
 }
 
-// Fallback test of the warm fit of ROI j (count K = khat[j]): not converged, an invalid model, or
-// a final loss above the loss at the plain heuristic start -> the ROI joins the fallback list.
+// fallback test of the warm K-emitter fits of the listed ROIs (or all n): not converged, or a loss
+// above the loss at the plain start (K = 1: heuristic; K = 2: the K = 1 fit t_prev plus an emitter
+// at the largest residual) -> the ROI is appended to flist
 template <int K, int R>
-__device__ void warm_check_one(const float *qs, const float *t11, int flags, float n_min, float sigma0, bool &fallback)
+__global__ void __launch_bounds__(64) warm_check_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
+                                                        const int *__restrict__ n_dev, const float *__restrict__ t_fit,
+                                                        const int *__restrict__ info, const float *__restrict__ t_prev, CFitConst c,
+                                                        int *__restrict__ fcnt, int *__restrict__ flist)
 {
-    float th[CP<K>::P], h[CP<K>::P], g[CP<K>::P];
-    for (int i = 0; i < CP<K>::P; i++) th[i] = t11[i];
-    cf_init<K, R>(qs, nullptr, CF_INIT_HEUR, sigma0, h);
-    cf_project<K, R>(h);
-    const float Lw = cf_lossgrad<K, R, false>(qs, th, g);
-    const float Lh = cf_lossgrad<K, R, false>(qs, h, g);
-    fallback = !(flags & CF_CONVERGED) || !cf_valid<K, R>(t11, n_min) || !(Lw <= Lh);
-}
-
-template <int R>
-__global__ void __launch_bounds__(64) warm_check_kernel(const uint16_t *__restrict__ rois, int n, const int *__restrict__ khat,
-                                                        const float *__restrict__ t11, const int *__restrict__ info, CFitConst c,
-                                                        float n_min, int *__restrict__ fb_cnt, int *__restrict__ fb_list)
-{
-    constexpr int RR = R * R, WR = R * (R + 1);
-    __shared__ float qsh[RR * 64];
+    constexpr int P = CP<K>::P;
+    __shared__ float qsh[R * R * 64];
     const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
-    if ((int)(blockIdx.x * 64) >= n) return;
-    const bool valid = j < n;
-    float *qs = qsh + lane;
-    const uint16_t *src = rois + (size_t)(valid ? j : 0) * WR;
-    for (int k = 0; k < RR; k++) qs[k * 64] = fmaxf(valid ? ((float)src[k] - c.offset) * c.kadc : 1.0f, 1.0f) * c.inv_qe;
+    if (n_dev) n = min(*n_dev, n);
+    if ((int)(blockIdx.x * 64) >= n) return;   // whole wave beyond the count
     bool fb = false;
-    const int K = valid ? khat[j] : 0;   // K = 0: already in the fallback list (low confidence)
-    if (K > 0) {
-        const int flags = info[(size_t)j * 4 + 1];
-        fb = true;
-        if (K == 1) warm_check_one<1, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
-        else if (K == 2) warm_check_one<2, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
-        else if (K == 3) warm_check_one<3, R>(qs, t11 + (size_t)j * 11, flags, n_min, c.sigma0, fb);
+    int roi = 0;
+    if (j < n) {
+        roi = idx ? idx[j] : j;
+        float *qs = qsh + lane;
+        cf_load_q<R>(rois, roi, c, qs);
+        float th[P], h[P], g[P];
+        for (int i = 0; i < P; i++) th[i] = t_fit[(size_t)roi * P + i];
+        cf_init<K, R>(qs, K == 1 ? nullptr : t_prev + (size_t)roi * 5, K == 1 ? CF_INIT_HEUR : CF_INIT_ADD, c.sigma0, h);
+        cf_project<K, R>(h);
+        const float Lw = cf_lossgrad<K, R, false>(qs, th, g);
+        const float Lh = cf_lossgrad<K, R, false>(qs, h, g);
+        fb = !(info[(size_t)roi * 4 + 1] & CF_CONVERGED) || !(Lw <= Lh);
     }
-    const int pos = wave_append(fb_cnt, fb);
-    if (pos >= 0) fb_list[pos] = j;
-}
-
-// rows of the warm fits (3 per ROI, unused zero), FP64 full-Fisher CRLB
-template <int R>
-__global__ void __launch_bounds__(64) warm_rows_kernel(const uint16_t *__restrict__ rois, int n, const int *__restrict__ khat,
-                                                       const float *__restrict__ t11, CSelConst c, float *__restrict__ rows,
-                                                       int *__restrict__ sel)
-{
-    __shared__ double Ish[CP<3>::NS * 64];
-    const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
-    if (j >= n) return;
-    const uint16_t *hdr = rois + (size_t)j * R * (R + 1) + R * R;
-    float *o = rows + (size_t)j * 3 * QCS_REC_FLOATS;
-    for (int i = 0; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
-    const int K = khat[j];
-    if (sel) sel[j] = K;
-    double *I = Ish + lane;
-    const float *t = t11 + (size_t)j * 11;
-    if (K == 1) cf_write_rows<1, R>(t, hdr, c, I, o);
-    else if (K == 2) cf_write_rows<2, R>(t, hdr, c, I, o);
-    else if (K == 3) cf_write_rows<3, R>(t, hdr, c, I, o);
+    const int pos = wave_append(fcnt, fb);
+    if (pos >= 0) flist[pos] = roi;
 }
 
 // training labels from the plain path: [K, (x, y, N) x 3 sorted by N descending, s, b, 0...]
@@ -487,34 +448,111 @@ __global__ void label_kernel(int n, const int *__restrict__ sel, const float *__
     lb[11] = t[3 * K + 1];
 }
 
-__global__ void warm_zero_kernel(int *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-// per-ROI statistics of a path: [0..2] ROIs per count, [3] fallbacks, [4] iterations, [5] evaluations
-__global__ void warm_stats_kernel(int n, const int *__restrict__ sel, const int *__restrict__ info, int stride_info,
-                                  long long *__restrict__ st)
+// statistics of a multi-emitter call: [0..2] ROIs per final count, [4] iterations and [5] loss
+// evaluations over the fits (info1..info3), [6] ROIs whose count the network capped below 3,
+// [7] and [8] ROIs the network put at two and three emitters
+__global__ void cf_stats_kernel(int n, const int *__restrict__ sel, const int *__restrict__ info1, const int *__restrict__ info2,
+                                const int *__restrict__ info3, const int *__restrict__ cs, const int *__restrict__ khat,
+                                long long *__restrict__ st)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    long long it = 0, ev = 0;
-    if (j < n && info) {
-        it = info[(size_t)j * stride_info];
-        ev = info[(size_t)j * stride_info + 2];
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        it += __shfl_down(it, off, 64);
-        ev += __shfl_down(ev, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd((unsigned long long *)&st[4], (unsigned long long)it);
-        atomicAdd((unsigned long long *)&st[5], (unsigned long long)ev);
-    }
-    if (j < n && sel) {
+    long long v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (j < n) {
         const int K = sel[j];
-        if (K >= 1 && K <= 3) atomicAdd((unsigned long long *)&st[K - 1], 1ull);
+        if (K >= 1 && K <= 3) v[K - 1] = 1;
+        const int *inf[3] = {info1, info2, info3};
+        for (int k = 0; k < 3; k++)
+            if (inf[k]) {
+                v[4] += inf[k][(size_t)j * 4];
+                v[5] += inf[k][(size_t)j * 4 + 2];
+            }
+        if (cs) v[6] = cs[j] < 3;
+        if (khat) {
+            v[7] = khat[j] == 2;
+            v[8] = khat[j] == 3;
+        }
     }
+    for (int i = 0; i < 9; i++) {
+        long long x = v[i];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicAdd((unsigned long long *)&st[i], (unsigned long long)x);
+    }
+}
+
+// ROI-indexed scratch of the multi-emitter paths, carved from the context's buffers
+struct MeScratch {
+    float *t1, *t2, *t3;
+    double *l1, *l2, *l3;
+    int *cnt;                  // WC_N counters
+    int *sel, *list3;
+    int *info1, *info2, *info3;
+    // warm path only
+    float *out, *norm, *tw1, *tw2;
+    int *cs, *khat, *l2w, *l2a, *f1, *f2;
+    long long *st;             // 9 statistics (8-byte aligned)
+};
+
+int me_scratch(qcs_ctx *ctx, int n, bool warm, bool info, MeScratch &m)
+{
+    const size_t n4 = ((size_t)n + 3) & ~(size_t)3;   // keeps every int array 16-byte aligned
+    const size_t nf = n4 * (5 + 8 + 11) + (warm ? n4 * (16 + 2 + 5 + 8) : 0);
+    const size_t ni = 64 + n4 * 2 + (info ? n4 * 12 : 0) + (warm ? n4 * 6 : 0);
+    int rc;
+    if ((rc = ctx->cf_theta.ensure(nf * sizeof(float))) || (rc = ctx->cf_ll.ensure(n4 * 3 * sizeof(double))) ||
+        (rc = ctx->cf_list.ensure(ni * sizeof(int))))
+        return rc;
+    memset(&m, 0, sizeof(m));
+    float *f = ctx->cf_theta.as<float>();
+    m.t1 = f;
+    m.t2 = f + n4 * 5;
+    m.t3 = f + n4 * 13;
+    f += n4 * 24;
+    double *d = ctx->cf_ll.as<double>();
+    m.l1 = d;
+    m.l2 = d + n4;
+    m.l3 = d + 2 * n4;
+    int *p = ctx->cf_list.as<int>();
+    m.cnt = p;
+    m.st = reinterpret_cast<long long *>(p + 32);
+    p += 64;
+    m.sel = p;
+    m.list3 = p + n4;
+    p += 2 * n4;
+    if (info) {
+        m.info1 = p;
+        m.info2 = p + 4 * n4;
+        m.info3 = p + 8 * n4;
+        p += 12 * n4;
+    }
+    if (warm) {
+        m.out = f;
+        m.norm = f + n4 * 16;
+        m.tw1 = f + n4 * 18;
+        m.tw2 = f + n4 * 23;
+        m.cs = p;
+        m.khat = p + n4;
+        m.l2w = p + 2 * n4;
+        m.l2a = p + 3 * n4;
+        m.f1 = p + 4 * n4;
+        m.f2 = p + 5 * n4;
+    }
+    return QCS_OK;
+}
+
+int stats_copy(qcs_ctx *ctx, const MeScratch &m, int n, const int *sel, long long *h_stats, hipStream_t s)
+{
+    int rc;
+    if ((rc = ctx->pin_cnt.ensure(9 * sizeof(long long) + WC_N * sizeof(int)))) return rc;
+    hipLaunchKernelGGL(cf_stats_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, m.info1, m.info2, m.info3, m.cs, m.khat, m.st);
+    QCS_HIP(hipGetLastError());
+    long long *h = ctx->pin_cnt.as<long long>();
+    int *hc = reinterpret_cast<int *>(h + 9);
+    QCS_HIP(hipMemcpyAsync(h, m.st, 9 * sizeof(long long), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipMemcpyAsync(hc, m.cnt, WC_N * sizeof(int), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < 9; i++) h_stats[i] = h[i];
+    h_stats[3] = (long long)hc[WC_F1] + hc[WC_F2];
+    return QCS_OK;
 }
 
 int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel, long long *h_stats,
@@ -523,57 +561,62 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     const qcs_params &p = ctx->p;
     const int R = p.ROISize;
     if (R != 7) return fail(QCS_ERR_ARG, "the warm start takes 7x7 ROIs");
-    // scratch: out [n][16], norm [n][2], init/theta [n][11], khat, lists 3n, fb list n, counters,
-    // info [n][4]; plain buffers for the fallback (roi-indexed)
-    const size_t nf = (size_t)n * (16 + 2 + 11 + 5 + 8 + 11), ni = (size_t)n * (1 + 3 + 1 + 4) + 32 + 4;
+    MeScratch m;
     int rc;
-    if ((rc = ctx->cf_theta.ensure(nf * 4)) || (rc = ctx->cf_aux.ensure(ni * 4 + 64)) || (rc = ctx->cf_ll.ensure((size_t)n * 3 * 8)) ||
-        (rc = ctx->pin_cnt.ensure(128)))
-        return rc;
-    float *out = ctx->cf_theta.as<float>(), *norm = out + (size_t)n * 16, *t11 = norm + (size_t)n * 2;
-    float *t1 = t11 + (size_t)n * 11, *t2 = t1 + (size_t)n * 5, *t3 = t2 + (size_t)n * 8;
-    // counters first (8-byte aligned statistics): cnt[0..2] per predicted count, cnt[3] fallbacks
-    int *cnt = ctx->cf_aux.as<int>();
-    long long *st = reinterpret_cast<long long *>(cnt + 8);
-    int *khat = cnt + 32, *lists = khat + n, *fbl = lists + (size_t)3 * n, *info = fbl + n;
-    info = reinterpret_cast<int *>((reinterpret_cast<uintptr_t>(info) + 15) & ~(uintptr_t)15);   // 16-byte rows
-    double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
-    hipLaunchKernelGGL(warm_zero_kernel, dim3(1), dim3(64), 0, s, cnt, 8 + 12);
+    if ((rc = me_scratch(ctx, n, true, true, m))) return rc;   // the fallback tests read the fits' flags
+    int *sel = d_sel ? d_sel : m.sel;
+    int *cnt = m.cnt;
+    fill_async(cnt, 64, 0, s);   // counters and statistics
+    if (h_stats) fill_async(m.info1, (size_t)12 * (((size_t)n + 3) & ~(size_t)3), 0, s);
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
-    if ((rc = qcs_net_forward_dev(net, &p, d_rois, n, out, norm, s))) return rc;
+    if ((rc = qcs_net_forward_dev(net, &p, d_rois, n, m.out, m.norm, s))) return rc;
     if ((rc = timer_end(ctx->timer, TM_NN, s, ev, n))) return rc;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
-    hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 255) / 256), dim3(256), 0, s, out, norm, n, R, ctx->tune.warm_pmin, t11, khat, cnt,
-                       lists, fbl, info);
+    hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m.out, m.norm, n, R, ctx->tune.warm_pmin, m.tw1,
+                       m.tw2, m.cs, m.khat, cnt, m.l2w, m.l2a);
     QCS_HIP(hipGetLastError());
-    for (int K = 1; K <= 3; K++)
-        if ((rc = cfit_launch(ctx, K, d_rois, lists + (size_t)(K - 1) * n, n, cnt + K - 1, CF_INIT_GIVEN, t11, 11, t11, 11, info,
-                              nullptr, s)))
-            return rc;
-    const CFitConst cc = cfit_const(p, ctx->tune, 1, CF_INIT_HEUR);
-    const CSelConst sc = csel_const(p, 3);
-    hipLaunchKernelGGL((warm_check_kernel<7>), dim3((n + 63) / 64), dim3(64), 0, s, d_rois, n, khat, t11, info, cc, sc.n_min,
-                       cnt + 3, fbl);
-    hipLaunchKernelGGL((warm_rows_kernel<7>), dim3((n + 63) / 64), dim3(64), 0, s, d_rois, n, khat, t11, sc, d_loc, d_sel);
+    const CFitConst c1 = cfit_const(p, ctx->tune, 1, CF_INIT_HEUR), c2 = cfit_const(p, ctx->tune, 2, CF_INIT_ADD);
+    const dim3 grid((n + 63) / 64), block(64);
+    // one-emitter fits: warm start, fallback from the heuristic
+    if ((rc = cfit_launch(ctx, 1, d_rois, nullptr, n, nullptr, CF_INIT_GIVEN, m.tw1, 5, m.t1, 5, m.info1, m.l1, s))) return rc;
+    hipLaunchKernelGGL((warm_check_kernel<1, 7>), grid, block, 0, s, d_rois, nullptr, n, nullptr, m.t1, m.info1, nullptr, c1,
+                       cnt + WC_F1, m.f1);
     QCS_HIP(hipGetLastError());
-    // fallback: the plain path over the listed ROIs (rows and counts overwrite the warm ones)
-    if ((rc = plain_multi_launch(ctx, 3, d_rois, fbl, n, cnt + 3, t1, t2, t3, l1, l2, l3, s))) return rc;
-    if ((rc = cselect_launch(p, 3, d_rois, fbl, n, cnt + 3, t1, t2, t3, l1, l2, l3, d_loc, d_sel, s))) return rc;
-    if ((rc = timer_end(ctx->timer, TM_CFIT, s, ev, n))) return rc;
-    if (h_stats) {
-        hipLaunchKernelGGL(warm_stats_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, d_sel, info, 4, st);
-        long long *h = ctx->pin_cnt.as<long long>();
-        QCS_HIP(hipMemcpyAsync(h, st, 6 * sizeof(long long), hipMemcpyDeviceToHost, s));
-        int *hc = reinterpret_cast<int *>(h + 8);
-        QCS_HIP(hipMemcpyAsync(hc, cnt, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
-        QCS_HIP(hipStreamSynchronize(s));
-        for (int i = 0; i < 6; i++) h_stats[i] = h[i];
-        h_stats[3] = hc[3];
-        h_stats[6] = hc[0];
-        h_stats[7] = hc[1];
-        h_stats[8] = hc[2];
+    if ((rc = cfit_launch(ctx, 1, d_rois, m.f1, n, cnt + WC_F1, CF_INIT_HEUR, nullptr, 0, m.t1, 5, m.info1, m.l1, s))) return rc;
+    // two-emitter fits of the ROIs not confidently single
+    if (ctx->tune.queue) {   // both starts in one persistent launch
+        CfqSet xs[2];
+        memset(xs, 0, sizeof(xs));
+        for (int i = 0; i < 2; i++) {
+            xs[i].rois = d_rois;
+            xs[i].idx = i == 0 ? m.l2w : m.l2a;
+            xs[i].n_dev = cnt + (i == 0 ? WC_L2W : WC_L2A);
+            xs[i].init = i == 0 ? m.tw2 : m.t1;
+            xs[i].istride = i == 0 ? 8 : 5;
+            xs[i].init_mode = i == 0 ? CF_INIT_GIVEN : CF_INIT_ADD;
+            xs[i].theta = m.t2;
+            xs[i].ostride = 8;
+            xs[i].info = m.info2;
+            xs[i].ll = m.l2;
+            xs[i].n = n;
+            xs[i].R = R;
+        }
+        if ((rc = cfq_launch(ctx, 2, xs, 2, s))) return rc;
+    } else {
+        if ((rc = cfit_launch(ctx, 2, d_rois, m.l2w, n, cnt + WC_L2W, CF_INIT_GIVEN, m.tw2, 8, m.t2, 8, m.info2, m.l2, s))) return rc;
+        if ((rc = cfit_launch(ctx, 2, d_rois, m.l2a, n, cnt + WC_L2A, CF_INIT_ADD, m.t1, 5, m.t2, 8, m.info2, m.l2, s))) return rc;
     }
+    hipLaunchKernelGGL((warm_check_kernel<2, 7>), grid, block, 0, s, d_rois, m.l2w, n, cnt + WC_L2W, m.t2, m.info2, m.t1, c2,
+                       cnt + WC_F2, m.f2);
+    QCS_HIP(hipGetLastError());
+    if ((rc = cfit_launch(ctx, 2, d_rois, m.f2, n, cnt + WC_F2, CF_INIT_ADD, m.t1, 5, m.t2, 8, m.info2, m.l2, s))) return rc;
+    // selection as on the plain path
+    if ((rc = select12_launch(p, 3, nullptr, n, nullptr, m.t1, m.t2, m.l1, m.l2, m.cs, sel, cnt + WC_L3, m.list3, s))) return rc;
+    if ((rc = cfit_launch(ctx, 3, d_rois, m.list3, n, cnt + WC_L3, CF_INIT_ADD, m.t2, 8, m.t3, 11, m.info3, m.l3, s))) return rc;
+    if ((rc = rows_launch(p, 3, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l2, m.l3, sel, d_loc, s))) return rc;
+    if ((rc = timer_end(ctx->timer, TM_CFIT, s, ev, n))) return rc;
+    if (h_stats) return stats_copy(ctx, m, n, sel, h_stats, s);
     return QCS_OK;
 }
 
@@ -602,16 +645,13 @@ extern "C" int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int 
     if (!ctx || n < 0 || (n > 0 && (!d_rois || !d_loc)) || kmax < 1 || kmax > 3) return fail(QCS_ERR_ARG, "invalid argument");
     if (n == 0) return QCS_OK;
     hipStream_t s = (hipStream_t)stream;
-    const size_t nf = (size_t)n * (5 + 8 + 11), nd = (size_t)n * 3;
+    MeScratch m;
     int rc;
-    if ((rc = ctx->cf_theta.ensure(nf * sizeof(float))) || (rc = ctx->cf_ll.ensure(nd * sizeof(double)))) return rc;
-    float *t1 = ctx->cf_theta.as<float>(), *t2 = t1 + (size_t)n * 5, *t3 = t2 + (size_t)n * 8;
-    double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
+    if ((rc = me_scratch(ctx, n, false, false, m))) return rc;
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
-    if ((rc = plain_multi_launch(ctx, kmax, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
-    if ((rc = cselect_launch(ctx->p, kmax, d_rois, nullptr, n, nullptr, t1, kmax >= 2 ? t2 : nullptr, kmax >= 3 ? t3 : nullptr, l1,
-                             l2, l3, d_loc, d_sel, s)))
+    if ((rc = plain_path(ctx, kmax, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, d_sel ? d_sel : m.sel, m.cnt,
+                         m.list3, d_loc, nullptr, nullptr, nullptr, s)))
         return rc;
     return timer_end(ctx->timer, TM_CFIT, s, ev, n);
 }
@@ -631,48 +671,55 @@ extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int 
     if (total == 0) return QCS_OK;
     if (total > (size_t)INT32_MAX / 32) return fail(QCS_ERR_ARG, "too many ROIs in one call");
     hipStream_t s = (hipStream_t)stream;
+    // one scratch carve for all sets; set i uses the ROI range [off_i, off_i + n_i)
+    MeScratch m;
     int rc;
-    if ((rc = ctx->cf_theta.ensure(total * (5 + 8 + 11) * sizeof(float))) || (rc = ctx->cf_ll.ensure(total * 3 * sizeof(double))))
-        return rc;
-    // per set: t1 | t2 | t3 (strides 5, 8, 11) and l1 | l2 | l3, each indexed by ROI
-    float *tb[3][3];
-    double *lb[3][3];
-    {
-        float *t = ctx->cf_theta.as<float>();
-        double *l = ctx->cf_ll.as<double>();
-        for (int i = 0; i < nsets; i++) {
-            const size_t m = (size_t)sets[i].n;
-            tb[i][0] = t;
-            tb[i][1] = t + m * 5;
-            tb[i][2] = t + m * 13;
-            t += m * 24;
-            for (int k = 0; k < 3; k++) lb[i][k] = l + m * k;
-            l += m * 3;
-        }
-    }
+    if ((rc = me_scratch(ctx, (int)total, false, false, m))) return rc;
+    size_t off[3] = {0, 0, 0};
+    for (int i = 1; i < nsets; i++) off[i] = off[i - 1] + (size_t)sets[i - 1].n;
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+    fill_async(m.cnt, 8, 0, s);
     for (int K = 1; K <= kmax; K++) {
         CfqSet xs[3];
         memset(xs, 0, sizeof(xs));
         for (int i = 0; i < nsets; i++) {
             CfqSet &x = xs[i];
+            const size_t o = off[i];
             x.rois = sets[i].d_rois;
             x.n = sets[i].n;
             x.R = sets[i].roi_size;
-            x.init = K == 1 ? nullptr : tb[i][K - 2];
+            x.init = K == 1 ? nullptr : (K == 2 ? m.t1 + o * 5 : m.t2 + o * 8);
             x.istride = K == 1 ? 0 : 3 * K - 1;
-            x.theta = tb[i][K - 1];
+            x.theta = K == 1 ? m.t1 + o * 5 : (K == 2 ? m.t2 + o * 8 : m.t3 + o * 11);
             x.ostride = 3 * K + 2;
-            x.ll = lb[i][K - 1];
+            x.ll = (K == 1 ? m.l1 : (K == 2 ? m.l2 : m.l3)) + o;
+            x.init_mode = K == 1 ? CF_INIT_HEUR : CF_INIT_ADD;
+            if (K == 3) {   // only the ROIs that stage 1 left at two emitters
+                x.idx = m.list3 + o;
+                x.n_dev = m.cnt + i;
+            }
         }
-        if ((rc = cfq_launch(ctx, K, K == 1 ? CF_INIT_HEUR : CF_INIT_ADD, xs, nsets, s))) return rc;
+        if ((rc = cfq_launch(ctx, K, xs, nsets, s))) return rc;
+        if (K == 2 || (K == 1 && kmax == 1)) {
+            for (int i = 0; i < nsets; i++) {
+                qcs_params p = ctx->p;
+                p.ROISize = sets[i].roi_size;
+                const size_t o = off[i];
+                int *sel = sets[i].d_sel ? sets[i].d_sel : m.sel + o;
+                if ((rc = select12_launch(p, kmax, nullptr, sets[i].n, nullptr, m.t1 + o * 5, m.t2 + o * 8, m.l1 + o, m.l2 + o, nullptr,
+                                          sel, m.cnt + i, m.list3 + o, s)))
+                    return rc;
+            }
+        }
     }
     for (int i = 0; i < nsets; i++) {
         qcs_params p = ctx->p;
         p.ROISize = sets[i].roi_size;
-        if ((rc = cselect_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, nullptr, tb[i][0], kmax >= 2 ? tb[i][1] : nullptr,
-                                 kmax >= 3 ? tb[i][2] : nullptr, lb[i][0], lb[i][1], lb[i][2], sets[i].d_loc, sets[i].d_sel, s)))
+        const size_t o = off[i];
+        int *sel = sets[i].d_sel ? sets[i].d_sel : m.sel + o;
+        if ((rc = rows_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, nullptr, m.t1 + o * 5, m.t2 + o * 8, m.t3 + o * 11, m.l2 + o,
+                              m.l3 + o, sel, sets[i].d_loc, s)))
             return rc;
     }
     return timer_end(ctx->timer, TM_CFIT, s, ev, (long long)total);
@@ -683,12 +730,27 @@ extern "C" int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_ro
 {
     if (!ctx || !net || n < 0 || (n > 0 && (!d_rois || !d_loc))) return fail(QCS_ERR_ARG, "invalid argument");
     if (n == 0) return QCS_OK;
-    if (!d_sel && h_stats) {   // the statistics count the selected emitters: keep them in scratch
-        int rc = ctx->cf_sel.ensure((size_t)n * sizeof(int));
-        if (rc) return rc;
-        d_sel = ctx->cf_sel.as<int>();
-    }
     return warm_launch(ctx, net, d_rois, n, d_loc, d_sel, h_stats, (hipStream_t)stream);
+}
+
+extern "C" int qcs_fit_plain_stats_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel,
+                                       long long *h_stats, void *stream)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_rois || !d_loc)) || kmax < 1 || kmax > 3 || !h_stats)
+        return fail(QCS_ERR_ARG, "invalid argument");
+    for (int i = 0; i < 9; i++) h_stats[i] = 0;
+    if (n == 0) return QCS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    MeScratch m;
+    int rc;
+    if ((rc = me_scratch(ctx, n, false, true, m))) return rc;
+    int *sel = d_sel ? d_sel : m.sel;
+    fill_async(m.cnt, 64, 0, s);
+    fill_async(m.info1, (size_t)12 * (((size_t)n + 3) & ~(size_t)3), 0, s);
+    if ((rc = plain_path(ctx, kmax, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, sel, m.cnt, m.list3, d_loc,
+                         m.info1, m.info2, m.info3, s)))
+        return rc;
+    return stats_copy(ctx, m, n, sel, h_stats, s);
 }
 
 extern "C" int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_labels, int *d_sel, void *stream)
@@ -696,17 +758,15 @@ extern "C" int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float 
     if (!ctx || n < 0 || (n > 0 && (!d_rois || !d_labels))) return fail(QCS_ERR_ARG, "invalid argument");
     if (n == 0) return QCS_OK;
     hipStream_t s = (hipStream_t)stream;
-    const size_t nf = (size_t)n * (5 + 8 + 11 + 36);
+    MeScratch m;
     int rc;
-    if ((rc = ctx->cf_theta.ensure(nf * sizeof(float))) || (rc = ctx->cf_ll.ensure((size_t)n * 3 * sizeof(double))) ||
-        (rc = ctx->cf_aux.ensure((size_t)n * sizeof(int) + 64)))
+    if ((rc = me_scratch(ctx, n, false, false, m))) return rc;
+    if ((rc = ctx->cf_rows.ensure((size_t)n * 3 * QCS_REC_FLOATS * sizeof(float)))) return rc;
+    int *sel = d_sel ? d_sel : m.sel;
+    if ((rc = plain_path(ctx, 3, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, sel, m.cnt, m.list3,
+                         ctx->cf_rows.as<float>(), nullptr, nullptr, nullptr, s)))
         return rc;
-    float *t1 = ctx->cf_theta.as<float>(), *t2 = t1 + (size_t)n * 5, *t3 = t2 + (size_t)n * 8, *rows = t3 + (size_t)n * 11;
-    double *l1 = ctx->cf_ll.as<double>(), *l2 = l1 + n, *l3 = l2 + n;
-    int *sel = d_sel ? d_sel : ctx->cf_aux.as<int>();
-    if ((rc = plain_multi_launch(ctx, 3, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, s))) return rc;
-    if ((rc = cselect_launch(ctx->p, 3, d_rois, nullptr, n, nullptr, t1, t2, t3, l1, l2, l3, rows, sel, s))) return rc;
-    hipLaunchKernelGGL(label_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, t1, t2, t3, d_labels);
+    hipLaunchKernelGGL(label_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, m.t1, m.t2, m.t3, d_labels);
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
@@ -726,6 +786,7 @@ extern "C" int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t)
     if (!(t->tol > 0.0f) || !(t->warm_pmin >= 0.0f && t->warm_pmin <= 1.0f)) return fail(QCS_ERR_ARG, "invalid tolerance or pmin");
     if ((t->queue != 0 && t->queue != 1) || t->waves_per_cu < 0 || t->waves_per_cu > 64)
         return fail(QCS_ERR_ARG, "queue must be 0 or 1 and waves_per_cu in 0..64");
+    if (!(t->ftol >= 0.0f) || (t->active_set != 0 && t->active_set != 1)) return fail(QCS_ERR_ARG, "ftol must be >= 0 and active_set 0 or 1");
     ctx->tune = *t;
     return QCS_OK;
 }

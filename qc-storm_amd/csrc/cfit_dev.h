@@ -19,9 +19,11 @@ struct CFitConst {
     float tol;          // convergence: max |step| in scaled units
     int init_mode;      // CF_INIT_*
     float sigma0;       // heuristic initial PSF width (px)
+    float ftol;         // convergence: an accepted step lowered the loss by less than this (0: off)
+    int active_set;     // leave bound-pinned parameters out of the step
 };
 enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
-enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4 };
+enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4, CF_FLAT = 8 };
 
 template <int K> struct CP { static constexpr int P = 3 * K + 2, NS = P * (P + 1) / 2; };
 
@@ -415,6 +417,147 @@ __device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelCon
 }
 
 
+// observed photons of ROI roi into the lane's LDS column: 4-byte loads (every ROI record starts
+// 4-byte aligned since R (R + 1) is even)
+template <int R>
+__device__ __forceinline__ void cf_load_q(const uint16_t *rois, int roi, const CFitConst &c, float *qs)
+{
+    constexpr int RR = R * R, WR = R * (R + 1);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(rois + (size_t)roi * WR);
+#pragma unroll
+    for (int w = 0; w < (RR + 1) / 2; w++) {
+        const uint32_t u = src[w];
+        qs[(2 * w) * 64] = fmaxf(((float)(u & 0xffffu) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
+        if (2 * w + 1 < RR) qs[(2 * w + 1) * 64] = fmaxf(((float)(u >> 16) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
+    }
+}
+
+// A parameter held at its box bound (cf_project) whose descent direction -gu points further out:
+// it is left out of the quasi-Newton step (active set), so the line search is not defeated by a
+// projection that cancels part of the step.
+template <int K, int R>
+__device__ __forceinline__ bool cf_pinned(const float (&th)[CP<K>::P], int i, float gu)
+{
+    constexpr int P = CP<K>::P;
+    const float v = th[i];
+    if (i == P - 1) return v <= 0.01f && gu > 0.0f;                                          // b
+    if (i == P - 2) return (v <= 0.4f && gu > 0.0f) || (v >= 0.5f * (float)R && gu < 0.0f);  // s
+    if (i % 3 == 0) return v <= 1.0f && gu > 0.0f;                                           // N_k
+    return (v <= -0.5f && gu > 0.0f) || (v >= (float)R + 0.5f && gu < 0.0f);                // x_k, y_k
+}
+
+// Quasi-Newton direction du = -D gu over the free parameters (scaled units, gu = g * scale;
+// every parameter is free unless active_set).
+// Returns true when the fit has converged: the free step is below tol in every coordinate with a
+// non-positive slope (this includes a zero free gradient).  Otherwise du, slope = du . gu < 0 and
+// dmax = max |du| are set; a non-descent direction restarts from the gradient with D = I (first
+// = true), and with D = I the step is limited to 0.1 scaled units.
+template <int K, int R>
+__device__ __forceinline__ bool cf_direction(const float (&th)[CP<K>::P], const float (&g)[CP<K>::P], float *D, bool &first,
+                                             float tol, bool active_set, float (&du)[CP<K>::P], float &slope, float &dmax)
+{
+    constexpr int P = CP<K>::P;
+    float gu[P];
+    bool pin[P];
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        gu[i] = g[i] * cf_scale(i, P);
+        pin[i] = active_set && cf_pinned<K, R>(th, i, gu[i]);
+    }
+    slope = 0.0f;
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        float a = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < P; jj++)
+            if (!pin[jj]) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
+        du[i] = pin[i] ? 0.0f : -a;
+        slope = fmaf(du[i], gu[i], slope);
+    }
+    {
+        float dm = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) dm = fmaxf(dm, fabsf(du[i]));
+        if (dm < tol && slope <= 0.0f) return true;
+    }
+    if (!(slope < 0.0f)) {   // not a descent direction: restart from the gradient
+#pragma unroll
+        for (int i = 0; i < P; i++) du[i] = pin[i] ? 0.0f : -gu[i];
+        slope = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) slope = fmaf(du[i], gu[i], slope);
+#pragma unroll
+        for (int i = 0; i < P; i++)
+#pragma unroll
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+        first = true;
+    }
+    if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
+        float m = 0.0f;
+#pragma unroll
+        for (int i = 0; i < P; i++) m = fmaxf(m, fabsf(du[i]));
+        if (m > 0.1f) {
+            const float f = 0.1f / m;
+            slope = 0.0f;
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                du[i] *= f;
+                slope = fmaf(du[i], gu[i], slope);
+            }
+        }
+    }
+    dmax = 0.0f;
+#pragma unroll
+    for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
+    return false;
+}
+
+// BFGS update of the packed inverse Hessian after an accepted step from th to tn with gradients
+// g -> gn (physical units); with D = I (first) D is first rescaled to (s.y / y.y) I.
+template <int K>
+__device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const float (&th)[CP<K>::P], const float (&tn)[CP<K>::P],
+                                               const float (&g)[CP<K>::P], const float (&gn)[CP<K>::P])
+{
+    constexpr int P = CP<K>::P;
+    float s[P], y[P], sy = 0.0f, yy = 0.0f;
+#pragma unroll
+    for (int i = 0; i < P; i++) s[i] = (tn[i] - th[i]) / cf_scale(i, P);
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        y[i] = (gn[i] - g[i]) * cf_scale(i, P);
+        sy = fmaf(s[i], y[i], sy);
+        yy = fmaf(y[i], y[i], yy);
+    }
+    if (!(sy > 1e-12f)) return;   // curvature condition fails: keep D
+    if (first) {
+        const float sc = sy / yy;
+#pragma unroll
+        for (int i = 0; i < P; i++)
+#pragma unroll
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
+        first = false;
+    }
+    // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
+    const float rho = 1.0f / sy;
+    float u[P], yu = 0.0f;
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        float a = 0.0f;
+#pragma unroll
+        for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
+        u[i] = a;
+        yu = fmaf(y[i], a, yu);
+    }
+    const float cs = fmaf(rho * rho, yu, rho);
+#pragma unroll
+    for (int i = 0; i < P; i++)
+#pragma unroll
+        for (int jj = i; jj < P; jj++) {
+            float *e = &D[tri(i, jj, P) * 64];
+            *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
+        }
+}
+
 // Position of this lane's item in a list that the whole wave appends to (one atomic per wave),
 // -1 if the lane takes no slot.  Every lane of the wave must make the call.
 __device__ __forceinline__ int wave_append(int *count, bool take)
@@ -444,6 +587,7 @@ struct CfqSet {
     double *ll;
     float *rec;      // work-item records of the persistent path (set by the launcher)
     int n, R, istride, ostride;
+    int init_mode;   // start of this set's fits (CF_INIT_*; a launch may mix them)
 };
 
 // arguments of one persistent launch (up to three ROI sizes, each with its own queue head)
@@ -455,8 +599,9 @@ struct CfqArgs {
 };
 
 CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_mode);
-// persistent work-queue fit of emitter count K over the sets (cfq.hip)
-int cfq_launch(qcs_ctx *ctx, int K, int init_mode, const CfqSet *sets, int nsets, hipStream_t s);
+// persistent work-queue fit of emitter count K over the sets (cfq.hip), each set started by its
+// own init_mode
+int cfq_launch(qcs_ctx *ctx, int K, const CfqSet *sets, int nsets, hipStream_t s);
 
 #define CF_DISPATCH_R(R_, ...)                                                            \
     switch (R_) {                                                                          \

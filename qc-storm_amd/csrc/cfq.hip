@@ -34,21 +34,6 @@ template <int K> struct CQ {
 
 enum { CQ_IDLE = 0, CQ_DIR = 1, CQ_LS = 2 };
 
-// observed photons of ROI roi into the lane's LDS column: 4-byte loads (every ROI record starts
-// 4-byte aligned since R (R + 1) is even)
-template <int R>
-__device__ __forceinline__ void cq_load_q(const uint16_t *rois, int roi, const CFitConst &c, float *qs)
-{
-    constexpr int RR = R * R, WR = R * (R + 1);
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(rois + (size_t)roi * WR);
-#pragma unroll
-    for (int w = 0; w < (RR + 1) / 2; w++) {
-        const uint32_t u = src[w];
-        qs[(2 * w) * 64] = fmaxf(((float)(u & 0xffffu) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
-        if (2 * w + 1 < RR) qs[(2 * w + 1) * 64] = fmaxf(((float)(u >> 16) - c.offset) * c.kadc, 1.0f) * c.inv_qe;
-    }
-}
-
 template <int K, int R>
 __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
 {
@@ -61,9 +46,9 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
     if (j >= n) return;   // no cross-lane work here
     const int roi = S.idx ? S.idx[j] : j;
     float *qs = qsh + lane, *D = Dsh + lane;
-    cq_load_q<R>(S.rois, roi, c, qs);
+    cf_load_q<R>(S.rois, roi, c, qs);
     float th[P], g[P];
-    cf_init<K, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? c.init_mode : CF_INIT_HEUR, c.sigma0, th);
+    cf_init<K, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? S.init_mode : CF_INIT_HEUR, c.sigma0, th);
     cf_project<K, R>(th);
     const float L = cf_lossgrad<K, R, true>(qs, th, g);
     const bool first = !cf_fisher_inverse<K, R>(th, D);
@@ -131,7 +116,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         const int j = base + __popcll(m & ((1ull << lane) - 1ull));
         if (j >= n) return;
         roi = S.idx ? S.idx[j] : j;
-        cq_load_q<R>(S.rois, roi, c, qs);
+        cf_load_q<R>(S.rois, roi, c, qs);
         const float4 *rec = reinterpret_cast<const float4 *>(S.rec + (size_t)j * Q::W);
 #pragma unroll
         for (int v = 0; v < Q::W / 4; v++) {
@@ -155,57 +140,11 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
     // quasi-Newton direction from a fresh gradient; converged fits end here
     auto direction = [&]() {
         if (st != CQ_DIR) return;
-        float gu[P];
-#pragma unroll
-        for (int i = 0; i < P; i++) gu[i] = g[i] * cf_scale(i, P);
-        slope = 0.0f;
-#pragma unroll
-        for (int i = 0; i < P; i++) {
-            float a = 0.0f;
-#pragma unroll
-            for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
-            du[i] = -a;
-            slope = fmaf(du[i], gu[i], slope);
+        if (cf_direction<K, R>(th, g, D, first, c.tol, c.active_set != 0, du, slope, dmax)) {
+            flags |= CF_CONVERGED;
+            finish();
+            return;
         }
-        {
-            float dm = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) dm = fmaxf(dm, fabsf(du[i]));
-            if (dm < c.tol && slope < 0.0f) {
-                flags |= CF_CONVERGED;
-                finish();
-                return;
-            }
-        }
-        if (!(slope < 0.0f)) {   // not a descent direction: restart from the gradient
-#pragma unroll
-            for (int i = 0; i < P; i++) du[i] = -gu[i];
-            slope = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) slope = fmaf(du[i], gu[i], slope);
-#pragma unroll
-            for (int i = 0; i < P; i++)
-#pragma unroll
-                for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
-            first = true;
-        }
-        if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
-            float m = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) m = fmaxf(m, fabsf(du[i]));
-            if (m > 0.1f) {
-                const float f = 0.1f / m;
-                slope = 0.0f;
-#pragma unroll
-                for (int i = 0; i < P; i++) {
-                    du[i] *= f;
-                    slope = fmaf(du[i], gu[i], slope);
-                }
-            }
-        }
-        dmax = 0.0f;
-#pragma unroll
-        for (int i = 0; i < P; i++) dmax = fmaxf(dmax, fabsf(du[i]));
         alpha = 1.0f;
         t = 0;
         st = CQ_LS;
@@ -227,48 +166,18 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         if (Lt == Lt && Lt < L && Lt <= fmaf(1e-4f * alpha, slope, L)) {
             // accepted: BFGS update of the inverse Hessian with s = step, y = gradient change
             iters++;
-            float s[P], y[P], sy = 0.0f, yy = 0.0f;
-#pragma unroll
-            for (int i = 0; i < P; i++) s[i] = (tn[i] - th[i]) / cf_scale(i, P);
+            cf_bfgs_update<K>(D, first, th, tn, g, gt);
 #pragma unroll
             for (int i = 0; i < P; i++) {
                 th[i] = tn[i];
-                y[i] = (gt[i] - g[i]) * cf_scale(i, P);
-                sy = fmaf(s[i], y[i], sy);
-                yy = fmaf(y[i], y[i], yy);
                 g[i] = gt[i];
             }
+            const float Lold = L;
             L = Lt;
-            if (sy > 1e-12f) {
-                if (first) {
-                    const float sc = sy / yy;
-#pragma unroll
-                    for (int i = 0; i < P; i++)
-#pragma unroll
-                        for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
-                    first = false;
-                }
-                // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
-                const float rho = 1.0f / sy;
-                float u[P], yu = 0.0f;
-#pragma unroll
-                for (int i = 0; i < P; i++) {
-                    float a = 0.0f;
-#pragma unroll
-                    for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
-                    u[i] = a;
-                    yu = fmaf(y[i], a, yu);
-                }
-                const float cs = fmaf(rho * rho, yu, rho);
-#pragma unroll
-                for (int i = 0; i < P; i++)
-#pragma unroll
-                    for (int jj = i; jj < P; jj++) {
-                        float *e = &D[tri(i, jj, P) * 64];
-                        *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
-                    }
-            }
-            if (iters >= c.max_iter) {
+            if (Lold - L < c.ftol) {   // the likelihood has stopped moving
+                flags |= CF_CONVERGED | CF_FLAT;
+                finish();
+            } else if (iters >= c.max_iter) {
                 flags |= CF_MAXITER;
                 finish();
             } else {
@@ -318,7 +227,7 @@ __global__ void __launch_bounds__(64) cfq_ll_kernel(CfqSet S, CFitConst c)
     if (j >= n) return;
     const int roi = S.idx ? S.idx[j] : j;
     float *qs = qsh + lane;
-    cq_load_q<R>(S.rois, roi, c, qs);
+    cf_load_q<R>(S.rois, roi, c, qs);
     float th[CP<K>::P];
 #pragma unroll
     for (int i = 0; i < CP<K>::P; i++) th[i] = S.theta[(size_t)roi * S.ostride + i];
@@ -362,13 +271,13 @@ static int cq_waves_per_cu(int K, int RMAX)
     return v;
 }
 
-int cfq_launch(qcs_ctx *ctx, int K, int init_mode, const CfqSet *sets_in, int nsets, hipStream_t s)
+int cfq_launch(qcs_ctx *ctx, int K, const CfqSet *sets_in, int nsets, hipStream_t s)
 {
     if (K < 1 || K > 3) return fail(QCS_ERR_ARG, "emitter count %d not in 1..3", K);
     if (nsets < 1 || nsets > 3) return fail(QCS_ERR_ARG, "1..3 ROI sets per launch");
     CfqArgs a;
     memset(&a, 0, sizeof(a));
-    a.c = cfit_const(ctx->p, ctx->tune, K, init_mode);
+    a.c = cfit_const(ctx->p, ctx->tune, K, CF_INIT_HEUR);
     const int W = cq_rec_w(K);
     size_t total = 0;
     int rmax = 0;
@@ -377,7 +286,9 @@ int cfq_launch(qcs_ctx *ctx, int K, int init_mode, const CfqSet *sets_in, int ns
         const CfqSet &x = sets_in[i];
         if (x.R != 7 && x.R != 9 && x.R != 11) return fail(QCS_ERR_ARG, "converged fitting supports ROISize 7, 9, 11 (got %d)", x.R);
         if (x.n < 0 || (x.n > 0 && (!x.rois || !x.theta))) return fail(QCS_ERR_ARG, "invalid ROI set");
-        if (init_mode != CF_INIT_HEUR && x.n > 0 && !x.init) return fail(QCS_ERR_ARG, "this start needs an initial vector");
+        if (x.init_mode < CF_INIT_HEUR || x.init_mode > CF_INIT_ADD) return fail(QCS_ERR_ARG, "invalid start mode");
+        if (x.init_mode == CF_INIT_ADD && K < 2) return fail(QCS_ERR_ARG, "adding an emitter needs K >= 2");
+        if (x.init_mode != CF_INIT_HEUR && x.n > 0 && !x.init) return fail(QCS_ERR_ARG, "this start needs an initial vector");
         total += (size_t)x.n * W;
         items += x.n;
         rmax = x.R > rmax ? x.R : rmax;

@@ -18,7 +18,9 @@ QCS_ERR_ARG, QCS_ERR_HIP, QCS_ERR_CAPACITY, QCS_ERR_STATE, QCS_ERR_NOMEM, QCS_ER
 LOCTYPE_GS2D, LOCTYPE_AS3D, LOCTYPE_DH3D = 0, 1, 2
 MATH_EXACT, MATH_FAST = 0, 1
 CFIT_INIT_HEUR, CFIT_INIT_GIVEN, CFIT_INIT_ADD = 0, 1, 2
-CFIT_CONVERGED, CFIT_MAXITER, CFIT_STALLED = 1, 2, 4
+CFIT_CONVERGED, CFIT_MAXITER, CFIT_STALLED, CFIT_FLAT = 1, 2, 4, 8
+# statistics of the multi-emitter paths (qcs_fit_warm_dev / qcs_fit_plain_stats_dev)
+STAT_KEYS = ("final_k1", "final_k2", "final_k3", "fallbacks", "iters", "evals", "capped", "pred_k2", "pred_k3")
 REC_FLOATS = 12
 WLE_FLOATS = 6
 TRUTH_FLOATS = 8
@@ -70,7 +72,7 @@ class QcsFitStats(C.Structure):
 
 class QcsTuning(C.Structure):
     _fields_ = [("max_iter", C.c_int * 3), ("tol", C.c_float), ("warm_pmin", C.c_float), ("queue", C.c_int),
-                ("waves_per_cu", C.c_int)]
+                ("waves_per_cu", C.c_int), ("ftol", C.c_float), ("active_set", C.c_int)]
 
 
 class QcsRoiSet(C.Structure):
@@ -115,6 +117,7 @@ PROTOTYPES = {
     "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
+    "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
     "qcs_tuning_get": (_i, [_vp, _P(QcsTuning)]),
     "qcs_tuning_set": (_i, [_vp, _P(QcsTuning)]),
     "qcs_net_param_count": (_i, []),
@@ -326,9 +329,10 @@ class Context:
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
         return {"max_iter": list(t.max_iter), "tol": t.tol, "warm_pmin": t.warm_pmin, "queue": t.queue,
-                "waves_per_cu": t.waves_per_cu}
+                "waves_per_cu": t.waves_per_cu, "ftol": t.ftol, "active_set": t.active_set}
 
-    def set_tuning(self, max_iter=None, tol=None, warm_pmin=None, queue=None, waves_per_cu=None):
+    def set_tuning(self, max_iter=None, tol=None, warm_pmin=None, queue=None, waves_per_cu=None, ftol=None,
+                   active_set=None):
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
         if max_iter is not None:
@@ -342,6 +346,10 @@ class Context:
             t.queue = int(queue)
         if waves_per_cu is not None:
             t.waves_per_cu = int(waves_per_cu)
+        if ftol is not None:
+            t.ftol = ftol
+        if active_set is not None:
+            t.active_set = int(active_set)
         check(lib().qcs_tuning_set(self.h, C.byref(t)), "qcs_tuning_set")
 
     def label_dev(self, d_rois, n, d_labels, d_sel=None, stream=None):
@@ -353,9 +361,14 @@ class Context:
                                      st if stats else None, stream), "qcs_fit_warm_dev")
         if not stats:
             return None
-        keys = ("final_k1", "final_k2", "final_k3", "fallbacks", "warm_iters", "warm_evals", "pred_k1", "pred_k2",
-                "pred_k3")
-        return dict(zip(keys, list(st)))
+        return dict(zip(STAT_KEYS, list(st)))
+
+    def fit_plain_stats_dev(self, d_rois, n, kmax, d_loc, d_sel=None, stream=None):
+        """qcs_fit_mle_dev with statistics (waits for the result)."""
+        st = (C.c_longlong * 9)()
+        check(lib().qcs_fit_plain_stats_dev(self.h, self._dp(d_rois), n, kmax, self._dp(d_loc), self._dp(d_sel), st, stream),
+              "qcs_fit_plain_stats_dev")
+        return dict(zip(STAT_KEYS, list(st)))
 
     def wle_dev(self, d_rois, n, d_wle, stream=None):
         check(lib().qcs_wle_dev(self.h, self._dp(d_rois), n, self._dp(d_wle), stream), "qcs_wle_dev")

@@ -154,3 +154,32 @@ def test_tuning_rejects_bad_queue_settings(gpu):
         ctx.set_tuning(waves_per_cu=-1)
     t = ctx.tuning()
     assert t["queue"] == 1 and t["waves_per_cu"] == 0
+
+
+def _valid(t, K, R, n_min=50.0):
+    ok = np.ones(len(t), bool)
+    for k in range(K):
+        ok &= t[:, 3 * k] >= n_min
+        ok &= (t[:, 3 * k + 1] > 0) & (t[:, 3 * k + 1] < R) & (t[:, 3 * k + 2] > 0) & (t[:, 3 * k + 2] < R)
+    return ok & (t[:, 3 * K] > 0.45) & (t[:, 3 * K] < 0.45 * R)
+
+
+@pytest.mark.parametrize("R", [7, 11])
+def test_three_emitter_pruning_is_exact(gpu, R):
+    """qcs_fit_mle_dev fits K = 3 only for ROIs that the first stage leaves at K = 2; the chosen
+    counts must equal the full rule applied to K = 1, 2, 3 fits of every ROI."""
+    n = 1500
+    rois, _ = _sim(R, n, 7300 + R)
+    d_rois = torch.from_numpy(rois.view(np.int16)).cuda()
+    ctx = Q.Context(Q.default_params(QE=1.0, ROISize=R, PixelSize=100.0))
+    (t1, _, l1), (t2, _, l2), (t3, _, l3) = _chain(ctx, d_rois, n)
+    K = np.where(_valid(t1, 1, R), 1, 0)
+    K = np.where(_valid(t2, 2, R) & ((K == 0) | (l2 - l1 > np.float64(np.float32(8.14)))), 2, K)
+    K = np.where((K == 2) & _valid(t3, 3, R) & (l3 - l2 > np.float64(np.float32(8.14))), 3, K)
+    loc = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
+    sel = torch.zeros(n, dtype=torch.int32, device="cuda")
+    st = ctx.fit_plain_stats_dev(d_rois, n, 3, loc, sel)
+    got = sel.cpu().numpy()
+    print("R", R, "counts", np.bincount(K, minlength=4), "stats", st)
+    np.testing.assert_array_equal(got, K)
+    assert st["final_k3"] == (K == 3).sum() and st["final_k2"] == (K == 2).sum()
