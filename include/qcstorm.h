@@ -329,6 +329,48 @@ int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_labels, 
  * whose count the network capped, ROIs the network put at 2 and at 3 emitters. */
 int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
                      long long *h_stats, void *stream);
+
+/* ========================================================================================
+ * Per-batch HIP graphs (no counterpart in the reference).  One call of qcs_fit_mle_dev (net
+ * NULL) or qcs_fit_warm_dev (statistics off) over fixed device buffers is captured into a HIP
+ * graph; each qcs_graph_launch refits whatever d_rois holds at that time into d_loc / d_sel.  The
+ * graph owns a private context with the caller's parameters and tuning (copied at creation), so
+ * the scratch addresses it uses stay valid; create runs the path once eagerly (after waiting for
+ * `stream`) to size that scratch.  Replays of one graph must not overlap.
+ * ====================================================================================== */
+typedef struct qcs_graph qcs_graph;
+int qcs_graph_create_mle(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel,
+                         void *stream, qcs_graph **out);
+int qcs_graph_launch(qcs_graph *g, void *stream);
+int qcs_graph_destroy(qcs_graph *g);
+
+/* ========================================================================================
+ * Autotuner of the multi-emitter paths (no counterpart in the reference).  On n sample ROIs it
+ * runs the plain path fully converged (caps 100, ftol 0) as the reference, then times candidate
+ * settings -- iteration caps, ftol, persistent waves per CU, persistent vs one-pass kernels, and
+ * with a network warm_pmin -- with HIP events (trials runs each, after one warm-up run) and
+ * scores each against the reference: the fraction of ROIs with the same emitter count and the
+ * 99th percentile of max(|dx|, |dy|) (pixels) over the emitters of those ROIs.  The fastest plain
+ * candidate with select_agree >= min_select_agree and dx_p99 <= max_dx_p99 sets the fitting
+ * knobs, the fastest such warm candidate warm_pmin; the result becomes the context's tuning
+ * (qcs_tuning_get).  d_loc (3n rows) and d_sel (n) are scratch for the runs.  table (may be NULL
+ * with table_cap 0) receives up to table_cap candidates, *n_entries their number.
+ * ====================================================================================== */
+typedef struct qcs_autotune_opts {
+    int   trials;             /* timed runs per candidate, 1..100 */
+    float min_select_agree;   /* e.g. 0.99 */
+    float max_dx_p99;         /* pixels, e.g. 0.02 */
+} qcs_autotune_opts;
+typedef struct qcs_autotune_entry {
+    qcs_tuning tuning;
+    float ms;                 /* mean time per call of n ROIs */
+    float select_agree, dx_p99;
+    int   warm;               /* 1: warm path (network) */
+    int   within;             /* met both tolerances */
+    int   chosen;
+} qcs_autotune_entry;
+int qcs_autotune_mle_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
+                         const qcs_autotune_opts *opts, qcs_autotune_entry *table, int table_cap, int *n_entries, void *stream);
 /* building block: C[M][N] (fp32) = A[M][K] B[N][K]^T, bf16 inputs, K a multiple of 16 */
 int qcs_dev_gemm_nt_bf16(const uint16_t *d_a, const uint16_t *d_b, float *d_c, int M, int N, int K, void *stream);
 

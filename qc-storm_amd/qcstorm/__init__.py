@@ -75,6 +75,36 @@ class QcsTuning(C.Structure):
                 ("waves_per_cu", C.c_int), ("ftol", C.c_float), ("active_set", C.c_int)]
 
 
+class QcsAutotuneOpts(C.Structure):
+    _fields_ = [("trials", C.c_int), ("min_select_agree", C.c_float), ("max_dx_p99", C.c_float)]
+
+
+class QcsAutotuneEntry(C.Structure):
+    _fields_ = [("tuning", QcsTuning), ("ms", C.c_float), ("select_agree", C.c_float), ("dx_p99", C.c_float),
+                ("warm", C.c_int), ("within", C.c_int), ("chosen", C.c_int)]
+
+
+class Graph:
+    """A replayable HIP graph of one multi-emitter batch (qcs_graph_*)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self, stream=None):
+        check(lib().qcs_graph_launch(self.h, stream), "qcs_graph_launch")
+
+    def close(self):
+        if self.h:
+            lib().qcs_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class QcsRoiSet(C.Structure):
     _fields_ = [("roi_size", C.c_int), ("n", C.c_int), ("d_rois", C.c_void_p), ("d_loc", C.c_void_p),
                 ("d_sel", C.c_void_p)]
@@ -118,6 +148,10 @@ PROTOTYPES = {
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
     "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
+    "qcs_graph_create_mle": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _P(_vp)]),
+    "qcs_graph_launch": (_i, [_vp, _vp]),
+    "qcs_autotune_mle_dev": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _P(QcsAutotuneOpts), _P(QcsAutotuneEntry), _i, _P(_i), _vp]),
+    "qcs_graph_destroy": (_i, [_vp]),
     "qcs_tuning_get": (_i, [_vp, _P(QcsTuning)]),
     "qcs_tuning_set": (_i, [_vp, _P(QcsTuning)]),
     "qcs_net_param_count": (_i, []),
@@ -362,6 +396,32 @@ class Context:
         if not stats:
             return None
         return dict(zip(STAT_KEYS, list(st)))
+
+    def autotune_dev(self, d_rois, n, net=None, trials=2, min_select_agree=0.99, max_dx_p99=0.02, stream=None):
+        """Run the autotuner on n sample ROIs; returns the candidate table (list of dicts)."""
+        import torch
+        loc = torch.zeros((3 * n, 12), dtype=torch.float32, device=d_rois.device)
+        sel = torch.zeros(n, dtype=torch.int32, device=d_rois.device)
+        opts = QcsAutotuneOpts(trials, min_select_agree, max_dx_p99)
+        cap = 64
+        tab = (QcsAutotuneEntry * cap)()
+        cnt = C.c_int()
+        check(lib().qcs_autotune_mle_dev(self.h, net.h if net is not None else None, self._dp(d_rois), n, self._dp(loc),
+                                         self._dp(sel), C.byref(opts), tab, cap, C.byref(cnt), stream), "qcs_autotune_mle_dev")
+        out = []
+        for e in tab[:min(cnt.value, cap)]:
+            t = e.tuning
+            out.append({"max_iter": list(t.max_iter), "ftol": t.ftol, "queue": t.queue, "waves_per_cu": t.waves_per_cu,
+                        "warm_pmin": t.warm_pmin, "ms": e.ms, "select_agree": e.select_agree, "dx_p99": e.dx_p99,
+                        "warm": bool(e.warm), "within": bool(e.within), "chosen": bool(e.chosen)})
+        return out
+
+    def graph_mle(self, d_rois, n, kmax, d_loc, d_sel=None, net=None, stream=None):
+        """A captured multi-emitter batch (plain, or warm with `net`); returns a Graph."""
+        h = C.c_void_p()
+        check(lib().qcs_graph_create_mle(self.h, net.h if net is not None else None, self._dp(d_rois), n, kmax,
+                                         self._dp(d_loc), self._dp(d_sel), stream, C.byref(h)), "qcs_graph_create_mle")
+        return Graph(h)
 
     def fit_plain_stats_dev(self, d_rois, n, kmax, d_loc, d_sel=None, stream=None):
         """qcs_fit_mle_dev with statistics (waits for the result)."""
