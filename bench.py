@@ -166,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--multi-rois", type=int, default=1 << 20,
                     help="ROIs of the multi-emitter plain vs warm-start leg (0 skips it)")
+    ap.add_argument("--no-fast-leg", action="store_true", help="skip the fast-math pipeline leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -272,6 +273,53 @@ def main():
               "fit_kernel_fits_per_s": tm["fit_1e_rois"] / (tm["fit_1e_ms"] * 1e-3),
               "extract_frame_GBps": F * W * H * 2 * args.steps / (tm["extract_ms"] * 1e-3) / 1e9}
 
+    # the same pipeline with the transcendental-unit exp/log (the reference's kernels use the
+    # hardware approximations too); its rows are compared with the exact ones on the same ROIs
+    fast = None
+    if args.math == "exact" and not args.no_fast_leg:
+        log("bench: fast-math leg (%d step(s))" % args.steps)
+        exact_rows = loc[:n_last].clone()
+        params_f = Q.QcsParams.from_buffer_copy(params)
+        params_f.FitMath = Q.MATH_FAST
+        ctx_f = Q.Context(params_f)
+        loc_f = torch.empty_like(loc)
+
+        def step_f():
+            n, _ = ctx_f.extract_dev(frames, F, 1, rois, wle, cap, sh)
+            rows = ctx_f.fit_dev(rois, wle, n, loc_f, sh)
+            m = ctx_f.compact_sorted_dev(loc_f, rows, 1, F, out, sh)
+            with torch.cuda.stream(stream):
+                img.zero_()
+            ctx_f.render_dev(out, m, img, None, sh)
+            return n, m
+
+        for _ in range(args.warmup):
+            step_f()
+        torch.cuda.synchronize()
+        ctx_f.timing_enable(True)
+        t0 = time.perf_counter()
+        fits_f = 0
+        for _ in range(args.steps):
+            fits_f += step_f()[0]
+        torch.cuda.synchronize()
+        dt_f = time.perf_counter() - t0
+        tf = ctx_f.timing()
+        fl = max(tf["fit_1e_launches"], 1)
+        f_avg_ms = tf["fit_1e_ms"] / fl
+        f_achieved = FLOP_PER_FIT_7X7 * (tf["fit_1e_rois"] / fl) / (f_avg_ms * 1e-3) / 1e12
+        a, b = exact_rows.cpu().numpy(), loc_f[:n_last].cpu().numpy()
+        both = (a[:, 0] > 0) & (b[:, 0] > 0)
+        d = np.maximum(np.abs(a[both, 1] - b[both, 1]), np.abs(a[both, 2] - b[both, 2]))
+        fast = {"value": fits_f / dt_f, "unit": "fits/s", "ms_per_step": dt_f * 1e3 / args.steps,
+                "frames_per_s": F * args.steps / dt_f,
+                "fit_kernel_fits_per_s": tf["fit_1e_rois"] / (tf["fit_1e_ms"] * 1e-3),
+                "roofline": {"achieved": f_achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                             "frac": f_achieved / PEAK_FP32_TFLOPS, "avg_launch_ms": f_avg_ms},
+                "vs_exact_rows": {"rows": int(n_last), "acceptance_agree": float(np.mean((a[:, 0] > 0) == (b[:, 0] > 0))),
+                                  "dxy_px_median": float(np.median(d)) if d.size else 0.0,
+                                  "dxy_px_p99": float(np.percentile(d, 99)) if d.size else 0.0}}
+        ctx_f.close()
+
     multi = None
     if args.multi_rois > 0:
         log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
@@ -298,7 +346,7 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "multi_emitter": multi}
+            "fast_math": fast, "multi_emitter": multi}
     ctx.close()
     print(json.dumps(line), flush=True)
 

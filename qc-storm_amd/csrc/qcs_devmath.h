@@ -58,7 +58,7 @@ __device__ __forceinline__ float qlogf_normal(float x)
 }
 
 // true when x is a positive normal number (v_cmp_class)
-__device__ __forceinline__ bool is_pos_normal(float x) { return __builtin_amdgcn_class(x, 0x100); }
+__device__ __forceinline__ bool is_pos_normal(float x) { return __builtin_amdgcn_classf(x, 0x100); }
 
 // log(x) for every x: zero, negative, subnormal, infinite and NaN arguments included
 __device__ __forceinline__ float qlogf(float x)
