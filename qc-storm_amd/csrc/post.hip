@@ -10,8 +10,12 @@
 // Sorted compaction: the reference launches one kernel per frame and appends with an atomic
 // counter (within-frame order arbitrary, O(frames x rows) work).  Here the kept rows are compacted
 // in row order (scan), then stably radix-sorted by frame (hipCUB), so rows of one frame keep their
-// original order.  Rendering: one lane per localization splats a 7 x 7 Gaussian with float
-// atomics into the SR image, as the reference does; the PSF uses the deterministic exponential.
+// original order.  Rendering splats each localization's 7 x 7 Gaussian with float atomics, as the
+// reference does, but into LDS: localizations are binned by 64 x 64 tile of the SR image (a
+// footprint that straddles a tile edge is listed in each tile it touches), one work-group
+// accumulates a tile with LDS atomics and then adds it to the image with one coalesced
+// read-modify-write.  Scattered atomics into a 1.7 GB image would miss the caches on nearly
+// every update.  The PSF uses the deterministic exponential.
 #include <hipcub/hipcub.hpp>
 #include <string.h>
 #include "qcs_internal.h"
@@ -74,48 +78,123 @@ __global__ void __launch_bounds__(256) copy_rows_kernel(const float *__restrict_
 struct RenderConst {
     int srw, srh, mode, is3d;
     float zoom, ps, snr_th, fixed_prec, zth;
+    int tiles_x, tiles_y;
 };
+constexpr int RTILE = 64;   // SR tile edge, pixels
 
-__global__ void __launch_bounds__(256) render_kernel(const float *__restrict__ loc, int n, RenderConst c,
-                                                     float *__restrict__ img, float *__restrict__ zimg)
+// The drawing parameters of one localization row (FluoRenderWithPrec_2D/_3D,
+// ImageRender_2d.cu:55-155, ImageRender_3d.cu:60-147): false if it is not drawn.
+struct Splat {
+    float pp, rs, fx, fy, z;
+    int ix, iy;
+    bool one;   // a single pixel (peak photons, 2D)
+};
+__device__ __forceinline__ bool render_prep(const float *__restrict__ r, const RenderConst &c, Splat &o)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *r = loc + (size_t)i * QCS_REC_FLOATS;
     float pp = r[QCS_POS_PPHO];
     const float rx = r[QCS_POS_XPOS] * c.zoom, ry = r[QCS_POS_YPOS] * c.zoom;
     float snr = r[QCS_POS_PSNR];
     float lp = c.is3d ? (r[QCS_POS_CRBX] + r[QCS_POS_CRBY]) / 2.0f : r[QCS_POS_CRBX];
-    const float z = r[QCS_POS_ZPOS];
+    o.z = r[QCS_POS_ZPOS];
     if (lp != lp) { snr = 0.0f; pp = 0.0f; }
     if (lp <= 0.0f) pp = 0.0f;
     bool inside = snr >= c.snr_th && pp > 1.0f && rx > 7.0f && rx < (float)(c.srw - 1 - 7) && ry > 7.0f &&
                   ry < (float)(c.srh - 1 - 7);
-    if (c.is3d) inside = inside && fabsf(z) < c.zth;
-    if (!inside) return;
+    if (c.is3d) inside = inside && fabsf(o.z) < c.zth;
+    if (!inside) return false;
     if (c.mode == QCS_RENDER_1PHOTON_CRLB || c.mode == QCS_RENDER_1PHOTON_FIXED) pp = 1.0f;
     if (c.mode == QCS_RENDER_1PHOTON_FIXED) lp = c.fixed_prec;
     if (!c.is3d && lp > 30.0f) { lp = 30.0f; pp = 0.0f; }
     lp = fmaxf(lp, (c.ps / c.zoom) / 2.0f);
     float rs = lp / (c.ps / c.zoom);
-    rs = -0.5f / (rs * rs);
-    const int ix = (int)rx, iy = (int)ry;
-    const float fx = (rx - (float)ix) + 3.0f, fy = (ry - (float)iy) + 3.0f;
-    if (!c.is3d && c.mode == QCS_RENDER_PEAKPHOTON_1PIXEL) {
-        atomicAdd(&img[(size_t)c.srw * iy + ix], pp);
-        return;
-    }
-#pragma unroll
-    for (int a = 0; a < 7; a++) {
-        const size_t off = (size_t)c.srw * (iy - 3 + a) + ix - 3;
-        const float dy = (float)a - fy;
-#pragma unroll
-        for (int b = 0; b < 7; b++) {
-            const float dx = (float)b - fx;
-            const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * rs);
-            atomicAdd(&img[off + b], pp * psf);
-            if (c.is3d && zimg) atomicAdd(&zimg[off + b], (z * pp) * psf);
+    o.rs = -0.5f / (rs * rs);
+    o.pp = pp;
+    o.ix = (int)rx;
+    o.iy = (int)ry;
+    o.fx = (rx - (float)o.ix) + 3.0f;
+    o.fy = (ry - (float)o.iy) + 3.0f;
+    o.one = !c.is3d && c.mode == QCS_RENDER_PEAKPHOTON_1PIXEL;
+    return true;
+}
+
+// tiles touched by the footprint of a drawn localization
+__device__ __forceinline__ void splat_tiles(const Splat &q, int &tx0, int &tx1, int &ty0, int &ty1)
+{
+    const int h = q.one ? 0 : 3;
+    tx0 = (q.ix - h) / RTILE;
+    tx1 = (q.ix + h) / RTILE;
+    ty0 = (q.iy - h) / RTILE;
+    ty1 = (q.iy + h) / RTILE;
+}
+
+// pass 1 (cursor == null): localizations per tile; pass 2: the per-tile lists
+__global__ void __launch_bounds__(256) render_bin_kernel(const float *__restrict__ loc, int n, RenderConst c,
+                                                         int *__restrict__ tcnt, const int *__restrict__ toff,
+                                                         int *__restrict__ cursor, int *__restrict__ list)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Splat q;
+    if (!render_prep(loc + (size_t)i * QCS_REC_FLOATS, c, q)) return;
+    int tx0, tx1, ty0, ty1;
+    splat_tiles(q, tx0, tx1, ty0, ty1);
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            const int t = ty * c.tiles_x + tx;
+            if (!cursor) atomicAdd(&tcnt[t], 1);
+            else list[toff[t] + atomicAdd(&cursor[t], 1)] = i;
         }
+}
+
+// one work-group per tile: LDS accumulation, then img += tile (and zimg += z tile in 3D)
+__global__ void __launch_bounds__(256) render_tile_kernel(const float *__restrict__ loc, RenderConst c, const int *__restrict__ tcnt,
+                                                          const int *__restrict__ toff, const int *__restrict__ list,
+                                                          float *__restrict__ img, float *__restrict__ zimg)
+{
+    __shared__ float acc[RTILE * RTILE];
+    __shared__ float zacc[RTILE * RTILE];
+    const int t = blockIdx.x;
+    const int cnt = tcnt[t];
+    if (cnt == 0) return;   // uniform over the work-group
+    const bool zon = c.is3d && zimg;
+    const int x0 = (t % c.tiles_x) * RTILE, y0 = (t / c.tiles_x) * RTILE;
+    for (int p = threadIdx.x; p < RTILE * RTILE; p += blockDim.x) {
+        acc[p] = 0.0f;
+        if (zon) zacc[p] = 0.0f;
+    }
+    __syncthreads();
+    const int *lst = list + toff[t];
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        Splat q;
+        render_prep(loc + (size_t)lst[k] * QCS_REC_FLOATS, c, q);   // listed, hence drawn
+        if (q.one) {
+            atomicAdd(&acc[(q.iy - y0) * RTILE + (q.ix - x0)], q.pp);
+            continue;
+        }
+#pragma unroll
+        for (int a = 0; a < 7; a++) {
+            const int py = q.iy - 3 + a - y0;
+            if (py < 0 || py >= RTILE) continue;
+            const float dy = (float)a - q.fy;
+#pragma unroll
+            for (int b = 0; b < 7; b++) {
+                const int px = q.ix - 3 + b - x0;
+                if (px < 0 || px >= RTILE) continue;
+                const float dx = (float)b - q.fx;
+                const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * q.rs);
+                atomicAdd(&acc[py * RTILE + px], q.pp * psf);
+                if (zon) atomicAdd(&zacc[py * RTILE + px], (q.z * q.pp) * psf);
+            }
+        }
+    }
+    __syncthreads();
+    // the tile is owned by this work-group: plain coalesced read-modify-write
+    for (int p = threadIdx.x; p < RTILE * RTILE; p += blockDim.x) {
+        const int gy = y0 + p / RTILE, gx = x0 + p % RTILE;
+        if (gx >= c.srw || gy >= c.srh) continue;
+        const size_t g = (size_t)gy * c.srw + gx;
+        img[g] += acc[p];
+        if (zon) zimg[g] += zacc[p];
     }
 }
 
@@ -133,9 +212,10 @@ __global__ void __launch_bounds__(256) max_disp_kernel(const float *__restrict__
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-int render_launch(const qcs_params &p, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s)
+int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
+    const qcs_params &p = ctx->p;
     RenderConst c;
     c.srw = p.SRImageWidth;
     c.srh = p.SRImageHigh;
@@ -146,7 +226,24 @@ int render_launch(const qcs_params &p, const float *d_loc, int n, float *d_img, 
     c.snr_th = p.SNR_th;
     c.fixed_prec = p.FixedLocPrec;
     c.zth = 5.0f * fmaxf(fabsf(p.MinZDepth), fabsf(p.MaxZDepth));
-    hipLaunchKernelGGL(render_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_loc, n, c, d_img, d_zimg);
+    c.tiles_x = (c.srw + RTILE - 1) / RTILE;
+    c.tiles_y = (c.srh + RTILE - 1) / RTILE;
+    const size_t ntile = (size_t)c.tiles_x * c.tiles_y;
+    if (ntile > (size_t)INT32_MAX / 8 || (size_t)n * 4 > (size_t)INT32_MAX) return fail(QCS_ERR_ARG, "SR image or row count too large");
+    const int nst = scan_tile_count(ntile);
+    // tile counts, offsets (+ scan sums), cursors, then the lists (a footprint touches <= 4 tiles)
+    const size_t words = ntile * 3 + (size_t)nst + 1 + (size_t)n * 4;
+    int rc;
+    if ((rc = ctx->rn_buf.ensure(words * sizeof(int)))) return rc;
+    int *tcnt = ctx->rn_buf.as<int>(), *toff = tcnt + ntile, *tsum = toff + ntile, *cursor = tsum + nst + 1,
+        *list = cursor + ntile;
+    fill_async(tcnt, ntile, 0, s);
+    fill_async(cursor, ntile, 0, s);
+    const dim3 g1((n + 255) / 256), b1(256);
+    hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, nullptr, nullptr, nullptr);
+    if ((rc = scan_exclusive(tcnt, (int)ntile, toff, tsum, s))) return rc;
+    hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, toff, cursor, list);
+    hipLaunchKernelGGL(render_tile_kernel, dim3((unsigned)ntile), dim3(256), 0, s, d_loc, c, tcnt, toff, list, d_img, d_zimg);
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
@@ -245,7 +342,7 @@ extern "C" int qcs_render_dev(qcs_ctx *c, const float *d_loc, int n, float *d_im
     hipEvent_t ev;
     int rc = timer_begin(c->timer, (hipStream_t)stream, &ev);
     if (rc) return rc;
-    rc = render_launch(c->p, d_loc, n, d_img, d_zimg, (hipStream_t)stream);
+    rc = render_launch(c, d_loc, n, d_img, d_zimg, (hipStream_t)stream);
     if (rc) return rc;
     return timer_end(c->timer, TM_RENDER, (hipStream_t)stream, ev, n);
 }

@@ -75,7 +75,7 @@ int crlb_launch(const qcs_params &p, float *d_loc, int n, const int *d_n, hipStr
 
 // rendering helpers (post.hip): splat rows into the SR image(s); max of (int) image value into
 // *d_out (which must start at 0)
-int render_launch(const qcs_params &p, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);
+int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);
 int max_disp_launch(const float *d_img, size_t npx, int *d_out, hipStream_t s);
 
 // device utilities (extract.hip): exclusive scan of n ints (total at tile_sums[ntiles]),
@@ -99,6 +99,8 @@ struct qcs_ctx {
     qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;
     // converged fitting scratch
     qcs::DevBuf cf_theta, cf_ll, cf_list, cf_rows, cf_queue;
+    // rendering scratch (tile bins)
+    qcs::DevBuf rn_buf;
     qcs_tuning tune = {{100, 40, 40}, 1e-4f, 0.8f, 1, 0, 1e-3f, 0};
     // host staging for the host-pointer API
     qcs::DevBuf h_rois_dev, h_wle_dev, h_loc_dev;

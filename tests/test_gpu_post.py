@@ -70,3 +70,22 @@ def test_render_matches_oracle(gpu, lt, mode):
     if lt == 1:
         zg = zimg.cpu().numpy()
         assert np.allclose(zg, zref, rtol=1e-4, atol=1e-2)
+
+
+def test_render_accumulates_into_existing_image(gpu):
+    """Rendering adds to the image (tile read-modify-write): two calls give twice one call, and
+    pixels no localization touches keep their value."""
+    loc = _rows(20000, 11)
+    params = Q.default_params(ImageWidth=200, ImageHigh=150, PixelZoom=10.0)
+    ref, _ = O.render(params, loc, with_z=False)
+    ctx = Q.Context(params)
+    d = torch.from_numpy(loc).cuda()
+    img = torch.full((params.SRImageHigh, params.SRImageWidth), 0.25, dtype=torch.float32, device="cuda")
+    ctx.render_dev(d, loc.shape[0], img, None)
+    ctx.render_dev(d, loc.shape[0], img, None)
+    torch.cuda.synchronize()
+    got = img.cpu().numpy()
+    want = 0.25 + 2.0 * ref
+    scale = np.maximum(np.abs(want), 1e-3)
+    assert np.all(np.abs(got - want) <= 2e-5 * scale + 1e-6)
+    assert np.all(got[ref == 0] == np.float32(0.25))
