@@ -155,6 +155,74 @@ def multi_emitter_leg(n, seed, log):
             "weights": wsrc, "accuracy_sample_rois": m}
 
 
+class Pipeline:
+    """The full-frame pipeline over F resident frames.  With lanes > 1 the frames are split into
+    contiguous groups, each extracted, fitted and compacted by its own context on its own stream
+    from its own host thread (the GPU runs the lanes' extraction and fitting concurrently); the
+    frame-sorted groups are then rendered in order into the SR image."""
+
+    def __init__(self, params, frames, F, lanes, stream, torch):
+        import concurrent.futures as cf
+        self.torch, self.F, self.stream, self.lanes = torch, F, stream, []
+        dev = frames.device
+        R = params.ROISize
+        bounds = [round(F * k / lanes) for k in range(lanes + 1)]
+        for k in range(lanes):
+            f0, f1 = bounds[k], bounds[k + 1]
+            cap = 20000 * (f1 - f0)
+            st = stream if k == 0 else torch.cuda.Stream(device=dev)
+            self.lanes.append({
+                "f0": f0, "nf": f1 - f0, "cap": cap, "stream": st, "sh": C.c_void_p(st.cuda_stream),
+                "ctx": Q.Context(params),
+                "rois": torch.empty((cap, Q.roi_words(R)), dtype=torch.int16, device=dev),
+                "wle": torch.empty((cap, Q.WLE_FLOATS), dtype=torch.float32, device=dev),
+                "loc": torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev),
+                "out": torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev)})
+        self.frames = frames
+        self.img = torch.empty((params.SRImageHigh, params.SRImageWidth), dtype=torch.float32, device=dev)
+        self.ctx = self.lanes[0]["ctx"]
+        self.pool = cf.ThreadPoolExecutor(max_workers=lanes) if lanes > 1 else None
+
+    def _lane(self, L):
+        n, found = L["ctx"].extract_dev(self.frames[L["f0"]:L["f0"] + L["nf"]], L["nf"], L["f0"] + 1, L["rois"], L["wle"],
+                                        L["cap"], L["sh"])
+        if found != n:
+            raise RuntimeError("ROI buffer too small: %d found, %d kept" % (found, n))
+        rows = L["ctx"].fit_dev(L["rois"], L["wle"], n, L["loc"], L["sh"])
+        m = L["ctx"].compact_sorted_dev(L["loc"], rows, L["f0"] + 1, L["f0"] + L["nf"], L["out"], L["sh"])
+        L["n"], L["m"] = n, m
+        return n, m
+
+    def step(self):
+        if self.pool is None:
+            self._lane(self.lanes[0])
+        else:
+            list(self.pool.map(self._lane, self.lanes))   # each lane's compaction waits for its stream
+        with self.torch.cuda.stream(self.stream):
+            self.img.zero_()
+        for L in self.lanes:
+            self.ctx.render_dev(L["out"], L["m"], self.img, None, self.lanes[0]["sh"])
+        return sum(L["n"] for L in self.lanes), sum(L["m"] for L in self.lanes)
+
+    def timing_enable(self):
+        for L in self.lanes:
+            L["ctx"].timing_enable(True)
+
+    def timing(self):
+        """Stage totals summed over the lanes (each lane's intervals come from its own stream)."""
+        tot = None
+        for L in self.lanes:
+            t = L["ctx"].timing()
+            tot = t if tot is None else {k: tot[k] + t[k] for k in tot}
+        return tot
+
+    def close(self):
+        if self.pool is not None:
+            self.pool.shutdown()
+        for L in self.lanes:
+            L["ctx"].close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +235,8 @@ def main():
     ap.add_argument("--multi-rois", type=int, default=1 << 20,
                     help="ROIs of the multi-emitter plain vs warm-start leg (0 skips it)")
     ap.add_argument("--no-fast-leg", action="store_true", help="skip the fast-math pipeline leg")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="frame groups processed concurrently (own context, stream and host thread each)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -213,30 +283,14 @@ def main():
                                                C.c_void_p(frames[f0].data_ptr()), sh), "qcs_sim_frames_dev")
     stream.synchronize()
 
-    cap = 20000 * F
-    rois = torch.empty((cap, Q.roi_words(R)), dtype=torch.int16, device=dev)
-    wle = torch.empty((cap, Q.WLE_FLOATS), dtype=torch.float32, device=dev)
-    loc = torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev)
-    out = torch.empty((cap, Q.REC_FLOATS), dtype=torch.float32, device=dev)
-    img = torch.empty((params.SRImageHigh, params.SRImageWidth), dtype=torch.float32, device=dev)
-    ctx = Q.Context(params)
-
-    def step():
-        n, found = ctx.extract_dev(frames, F, 1, rois, wle, cap, sh)
-        if found != n:
-            raise RuntimeError("ROI buffer too small: %d found, %d kept" % (found, n))
-        rows = ctx.fit_dev(rois, wle, n, loc, sh)
-        m = ctx.compact_sorted_dev(loc, rows, 1, F, out, sh)
-        with torch.cuda.stream(stream):
-            img.zero_()
-        ctx.render_dev(out, m, img, None, sh)
-        return n, m
+    pipe = Pipeline(params, frames, F, args.lanes, stream, torch)
+    step = pipe.step
 
     log("bench: warmup %d step(s)" % args.warmup)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ctx.timing_enable(True)
+    pipe.timing_enable()
     log("bench: timing %d step(s)" % args.steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -247,7 +301,7 @@ def main():
         kept += m
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    tm = ctx.timing()
+    tm = pipe.timing()
     log("bench: stage timing %s" % json.dumps(tm))
 
     value = fits / dt
@@ -278,36 +332,26 @@ def main():
     fast = None
     if args.math == "exact" and not args.no_fast_leg:
         log("bench: fast-math leg (%d step(s))" % args.steps)
-        exact_rows = loc[:n_last].clone()
+        exact_rows = [L["loc"][:L["n"]].clone() for L in pipe.lanes]
         params_f = Q.QcsParams.from_buffer_copy(params)
         params_f.FitMath = Q.MATH_FAST
-        ctx_f = Q.Context(params_f)
-        loc_f = torch.empty_like(loc)
-
-        def step_f():
-            n, _ = ctx_f.extract_dev(frames, F, 1, rois, wle, cap, sh)
-            rows = ctx_f.fit_dev(rois, wle, n, loc_f, sh)
-            m = ctx_f.compact_sorted_dev(loc_f, rows, 1, F, out, sh)
-            with torch.cuda.stream(stream):
-                img.zero_()
-            ctx_f.render_dev(out, m, img, None, sh)
-            return n, m
-
+        pipe_f = Pipeline(params_f, frames, F, args.lanes, stream, torch)
         for _ in range(args.warmup):
-            step_f()
+            pipe_f.step()
         torch.cuda.synchronize()
-        ctx_f.timing_enable(True)
+        pipe_f.timing_enable()
         t0 = time.perf_counter()
         fits_f = 0
         for _ in range(args.steps):
-            fits_f += step_f()[0]
+            fits_f += pipe_f.step()[0]
         torch.cuda.synchronize()
         dt_f = time.perf_counter() - t0
-        tf = ctx_f.timing()
+        tf = pipe_f.timing()
         fl = max(tf["fit_1e_launches"], 1)
         f_avg_ms = tf["fit_1e_ms"] / fl
         f_achieved = FLOP_PER_FIT_7X7 * (tf["fit_1e_rois"] / fl) / (f_avg_ms * 1e-3) / 1e12
-        a, b = exact_rows.cpu().numpy(), loc_f[:n_last].cpu().numpy()
+        a = np.concatenate([r.cpu().numpy() for r in exact_rows])
+        b = np.concatenate([L["loc"][:L["n"]].cpu().numpy() for L in pipe_f.lanes])
         both = (a[:, 0] > 0) & (b[:, 0] > 0)
         d = np.maximum(np.abs(a[both, 1] - b[both, 1]), np.abs(a[both, 2] - b[both, 2]))
         fast = {"value": fits_f / dt_f, "unit": "fits/s", "ms_per_step": dt_f * 1e3 / args.steps,
@@ -318,7 +362,7 @@ def main():
                 "vs_exact_rows": {"rows": int(n_last), "acceptance_agree": float(np.mean((a[:, 0] > 0) == (b[:, 0] > 0))),
                                   "dxy_px_median": float(np.median(d)) if d.size else 0.0,
                                   "dxy_px_p99": float(np.percentile(d, 99)) if d.size else 0.0}}
-        ctx_f.close()
+        pipe_f.close()
 
     multi = None
     if args.multi_rois > 0:
@@ -329,11 +373,12 @@ def main():
     parity = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle, bounded sample)")
-        h_rois = rois[:n_last].cpu().numpy().view(np.uint16)
-        h_wle = wle[:n_last].cpu().numpy()
+        L0 = pipe.lanes[0]   # its ROIs are the first frames' (raster order): the workload's prefix
+        h_rois = L0["rois"][:L0["n"]].cpu().numpy().view(np.uint16)
+        h_wle = L0["wle"][:L0["n"]].cpu().numpy()
         cpu, ref = cpu_baseline(params, h_rois, h_wle, args.cpu_seconds)
         if args.math == "exact":
-            got = loc[:ref.shape[0]].cpu().numpy()
+            got = L0["loc"][:ref.shape[0]].cpu().numpy()
             same = int(np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1).sum())
             parity = {"rows": int(ref.shape[0]), "bit_exact_rows": same}
 
@@ -344,10 +389,10 @@ def main():
                                    "extraction + 7x7 single-emitter BFGS fit + CRLB + sorted compaction + "
                                    "10 nm rendering" % F,
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
-                       "rois_per_step": n_last},
+                       "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
             "fast_math": fast, "multi_emitter": multi}
-    ctx.close()
+    pipe.close()
     print(json.dumps(line), flush=True)
 
 
