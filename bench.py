@@ -364,6 +364,22 @@ def main():
                                   "dxy_px_p99": float(np.percentile(d, 99)) if d.size else 0.0}}
         pipe_f.close()
 
+    # statistics of the step's localizations (histograms, means, activation density), one call over
+    # the frame-sorted rows of lane 0 (all frames when lanes == 1)
+    log("bench: statistics leg")
+    L0 = pipe.lanes[0]
+    st_rows = L0["m"]
+    pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
+    st_ms = (time.perf_counter() - t0) * 1e3 / reps
+    statistics = {"rows": int(st_rows), "ms": st_ms, "rows_per_s": st_rows / (st_ms * 1e-3),
+                  "valid": st["valid"], "density_total": st["density_total"],
+                  "loc_density_2d_per_um2": st["loc_density_2d"], "mean_total_photon": st["mean_total_photon"],
+                  "mean_loc_prec_nm": st["mean_loc_prec"]}
+
     multi = None
     if args.multi_rois > 0:
         log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
@@ -381,6 +397,23 @@ def main():
             got = L0["loc"][:ref.shape[0]].cpu().numpy()
             same = int(np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1).sum())
             parity = {"rows": int(ref.shape[0]), "bit_exact_rows": same}
+        # statistics: the oracle (per-frame all-pairs neighbour scan, one thread) on the rows of the
+        # first 6 frames, against the device call on the same rows
+        import oracle_lib as O   # checker only
+        rows_h = L0["out"][:L0["m"]].cpu().numpy()
+        sub = rows_h[rows_h[:, 11] <= 6]
+        t0 = time.perf_counter()
+        st_ref = O.loc_stats(params, sub)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        d_sub = torch.from_numpy(np.ascontiguousarray(sub)).to(dev)
+        st_dev = pipe.ctx.loc_stats_dev(d_sub, sub.shape[0], sh)
+        statistics["cpu_baseline"] = {"value": sub.shape[0] / (cpu_ms * 1e-3), "unit": "rows/s", "cores": 1,
+                                      "kind": "port", "sample": "rows of frames 1-6 (%d rows)" % sub.shape[0]}
+        statistics["parity_sample"] = {
+            "rows": int(sub.shape[0]),
+            "counts_equal": all(st_dev[k] == st_ref[k] for k in ("valid", "density_total", "density_n0", "density_n1")),
+            "histograms_equal": all(np.array_equal(st_dev[k], st_ref[k]) for k in
+                                    ("hist_total_photon", "hist_loc_prec", "hist_peak_snr", "hist_psf_width"))}
 
     line = {"metric": METRIC, "value": value, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
@@ -391,7 +424,7 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "fast_math": fast, "multi_emitter": multi}
+            "fast_math": fast, "multi_emitter": multi, "statistics": statistics}
     pipe.close()
     print(json.dumps(line), flush=True)
 

@@ -181,7 +181,8 @@ int qcs_wle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_wle, void 
  * Row i of d_loc (12 floats) belongs to ROI i; in multi-emitter mode accepted second
  * emitters are appended at rows >= n.  `loc_cap` rows must be available (>= n; 2n covers
  * every append).  *n_rows receives the number of rows written (oValidFluoNum).
- * d_wle may be NULL when WLEEn == 0 and MultiEmitterFitEn == 0. */
+ * d_wle (n WLE records, qcs_wle_dev) is required: the fitter reads each ROI's type and PSF
+ * width estimate from it. */
 int qcs_fit_dev(qcs_ctx *ctx, const uint16_t *d_rois, const float *d_wle, int n,
                 float *d_loc, int loc_cap, int *n_rows, void *stream);
 
@@ -331,6 +332,35 @@ int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, 
                      long long *h_stats, void *stream);
 
 /* ========================================================================================
+ * Statistics of a batch of localization rows (FluoStatisticData_TypeDef::GetStatisticalInf,
+ * StatisticsInfo/statisticsInfo_top.cu:61-128; row kernel statisticsInfo.cu:19-91; activation
+ * density statisticsInfo_top.cu:130-205, statisticsInfo_densityCalc.cu:17-107; helpers
+ * statisticsInfo_rc.cu:17-152).  Rows: TotPho > 30, Bkg > 2, PSNR >= 3, PSF width
+ * (SigX + SigY) / 2 >= 0.01 px and 0.01 < sqrt(CRLBx^2 + CRLBy^2) < 55 nm count as valid and
+ * enter four histograms and the sums of the means.  The activation density comes from the
+ * fractions of molecules with no and with exactly one neighbour within 10 px (20 px for 3D) in
+ * the same frame, over the frames between the first and last row's frames (both ends excluded
+ * when more than 3 frames), counting molecules 5 px or more inside the image.  The sums are FP64
+ * (the reference sums with float atomics in arbitrary order); the call waits for its stream.
+ * ====================================================================================== */
+#define QCS_STAT_BINS 1600   /* StatInf_Hist_DatLenMax */
+typedef struct qcs_loc_stats {
+    int    hist_total_photon[QCS_STAT_BINS];   /* bins of 25 photons */
+    int    hist_loc_prec[QCS_STAT_BINS];       /* sqrt(CRLBx^2 + CRLBy^2), bins of 0.15 nm */
+    int    hist_peak_snr[QCS_STAT_BINS];       /* bins of 0.125 */
+    int    hist_psf_width[QCS_STAT_BINS];      /* (SigX + SigY) / 2, bins of 0.0125 px */
+    int    valid;                              /* rows that passed the filter */
+    double sum_loc_prec, sum_peak_snr, sum_psf_width, sum_background;
+    float  mean_total_photon;                  /* centre of mass of the photon histogram over bins >= 10% of its maximum */
+    float  mean_loc_prec, mean_peak_snr, mean_background, mean_psf_width;
+    float  mean_psf_width_ctl;                 /* mean_psf_width * -mean_peak_snr (lower is better) */
+    int    density_first_frame, density_last_frame;
+    int    density_total, density_n0, density_n1;   /* molecules considered; with 0 and with 1 neighbour */
+    float  loc_density_2d;                     /* activation density, 1/um^2; 0 unless density_total > 50 */
+} qcs_loc_stats;
+int qcs_loc_stats_dev(qcs_ctx *ctx, const float *d_loc, int n, qcs_loc_stats *out, void *stream);
+
+/* ========================================================================================
  * Per-batch HIP graphs (no counterpart in the reference).  One call of qcs_fit_mle_dev (net
  * NULL) or qcs_fit_warm_dev (statistics off) over fixed device buffers is captured into a HIP
  * graph; each qcs_graph_launch refits whatever d_rois holds at that time into d_loc / d_sel.  The
@@ -414,6 +444,13 @@ int qcs_session_destroy(qcs_session *s);
  * `cap` of them into out12.  Frames are numbered from `start_frame`. */
 long long qcs_loc_batch_sync(qcs_ctx *ctx, const uint16_t *frames, int nframes, int start_frame,
                              float *out12, long long cap);
+/* lm_LocBatchedImg with its return array (MM/QC_STORM_CPPDLL.cpp:857-924; the flow of
+ * LocResource_ZDriftMeasure.h:42-70, statistics reset per call): as qcs_loc_batch_sync, then the
+ * statistics of the batch's rows (qcs_loc_stats_dev) into `stats` (may be NULL) and the 8-float
+ * summary into `info8` (may be NULL): [0] mean peak SNR, [1] -mean PSF width, [2] [0] + [1],
+ * [3] [0] * [1], [4] rows, [5] activation density (0 when 50 or fewer molecules), [6], [7] 0. */
+long long qcs_loc_batch_sync_info(qcs_ctx *ctx, const uint16_t *frames, int nframes, int start_frame,
+                                  float *out12, long long cap, float *info8, qcs_loc_stats *stats);
 
 /* ========================================================================================
  * Synthetic data (deterministic simulator; no reference counterpart).  Philox4x32-10 keyed

@@ -118,6 +118,11 @@ int   orc_mle_f64(const double *img, int R, int K, int astig, double *theta, int
 /* Full-Fisher CRLB (standard deviations) at theta for the same models. */
 int   orc_fisher_crlb_f64(int R, int K, int astig, const double *theta, double *crlb);
 
+/* statistics of a batch of rows (orc_stats.c) */
+int   orc_loc_stats(const qcs_params *p, const float *loc, int n, qcs_loc_stats *out);
+float orc_activation_density(float r0, float r1, float radius_um);
+float orc_hist_mean(const int *h, int len, float pct);
+
 #ifdef __cplusplus
 }
 #endif

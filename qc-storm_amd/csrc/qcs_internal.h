@@ -99,8 +99,8 @@ struct qcs_ctx {
     qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;
     // converged fitting scratch
     qcs::DevBuf cf_theta, cf_ll, cf_list, cf_rows, cf_queue;
-    // rendering scratch (tile bins)
-    qcs::DevBuf rn_buf;
+    // rendering scratch (tile bins), statistics scratch
+    qcs::DevBuf rn_buf, st_buf;
     qcs_tuning tune = {{100, 40, 40}, 1e-4f, 0.8f, 1, 0, 1e-3f, 0};
     // host staging for the host-pointer API
     qcs::DevBuf h_rois_dev, h_wle_dev, h_loc_dev;

@@ -367,7 +367,15 @@ int qcs_session_destroy(qcs_session *s)
 long long qcs_loc_batch_sync(qcs_ctx *c, const uint16_t *frames, int nframes, int start_frame, float *out12,
                              long long cap)
 {
+    return qcs_loc_batch_sync_info(c, frames, nframes, start_frame, out12, cap, nullptr, nullptr);
+}
+
+long long qcs_loc_batch_sync_info(qcs_ctx *c, const uint16_t *frames, int nframes, int start_frame, float *out12,
+                                  long long cap, float *info8, qcs_loc_stats *stats)
+{
     if (!c || nframes < 0 || cap < 0 || (nframes > 0 && !frames)) return -fail(QCS_ERR_ARG, "invalid argument");
+    if (info8) memset(info8, 0, 8 * sizeof(float));
+    if (stats) memset(stats, 0, sizeof(*stats));
     if (nframes == 0) return 0;
     const qcs_params saved = c->p;
     if (c->p.ImageWidth <= 0 || c->p.ImageHigh <= 0) return -fail(QCS_ERR_ARG, "ImageWidth/ImageHigh not set");
@@ -396,6 +404,20 @@ long long qcs_loc_batch_sync(qcs_ctx *c, const uint16_t *frames, int nframes, in
         const long long k = m < cap ? m : cap;
         if (hipMemcpy(out12, out, (size_t)k * QCS_REC_FLOATS * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             return -fail(QCS_ERR_HIP, "result download failed");
+    }
+    if (info8 || stats) {
+        // statistics of the batch's rows and the MM summary (MM/QC_STORM_CPPDLL.cpp:895-913)
+        qcs_loc_stats st;
+        if ((rc = qcs_loc_stats_dev(c, out, m, &st, nullptr))) return -rc;
+        if (stats) *stats = st;
+        if (info8) {
+            info8[0] = st.mean_peak_snr;
+            info8[1] = -st.mean_psf_width;
+            info8[2] = st.mean_peak_snr - st.mean_psf_width;
+            info8[3] = st.mean_peak_snr * (-st.mean_psf_width);
+            info8[4] = (float)m;
+            info8[5] = st.loc_density_2d;
+        }
     }
     return m;
 }

@@ -75,6 +75,29 @@ class QcsTuning(C.Structure):
                 ("waves_per_cu", C.c_int), ("ftol", C.c_float), ("active_set", C.c_int)]
 
 
+STAT_BINS = 1600
+
+
+class QcsLocStats(C.Structure):
+    _fields_ = [("hist_total_photon", C.c_int * STAT_BINS), ("hist_loc_prec", C.c_int * STAT_BINS),
+                ("hist_peak_snr", C.c_int * STAT_BINS), ("hist_psf_width", C.c_int * STAT_BINS),
+                ("valid", C.c_int), ("sum_loc_prec", C.c_double), ("sum_peak_snr", C.c_double),
+                ("sum_psf_width", C.c_double), ("sum_background", C.c_double), ("mean_total_photon", C.c_float),
+                ("mean_loc_prec", C.c_float), ("mean_peak_snr", C.c_float), ("mean_background", C.c_float),
+                ("mean_psf_width", C.c_float), ("mean_psf_width_ctl", C.c_float), ("density_first_frame", C.c_int),
+                ("density_last_frame", C.c_int), ("density_total", C.c_int), ("density_n0", C.c_int),
+                ("density_n1", C.c_int), ("loc_density_2d", C.c_float)]
+
+
+def loc_stats_dict(st):
+    """qcs_loc_stats as a dict (histograms as numpy arrays)."""
+    out = {}
+    for k, _ in QcsLocStats._fields_:
+        v = getattr(st, k)
+        out[k] = np.frombuffer(v, dtype=np.int32).copy() if k.startswith("hist_") else v
+    return out
+
+
 class QcsAutotuneOpts(C.Structure):
     _fields_ = [("trials", C.c_int), ("min_select_agree", C.c_float), ("max_dx_p99", C.c_float)]
 
@@ -150,6 +173,7 @@ PROTOTYPES = {
     "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
     "qcs_graph_create_mle": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _P(_vp)]),
     "qcs_graph_launch": (_i, [_vp, _vp]),
+    "qcs_loc_stats_dev": (_i, [_vp, _vp, _i, _P(QcsLocStats), _vp]),
     "qcs_autotune_mle_dev": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _P(QcsAutotuneOpts), _P(QcsAutotuneEntry), _i, _P(_i), _vp]),
     "qcs_graph_destroy": (_i, [_vp]),
     "qcs_tuning_get": (_i, [_vp, _P(QcsTuning)]),
@@ -176,6 +200,7 @@ PROTOTYPES = {
     "qcs_session_get_max_disp_val": (_i, [_vp, _P(_i)]),
     "qcs_session_destroy": (_i, [_vp]),
     "qcs_loc_batch_sync": (_ll, [_vp, _vp, _i, _i, _vp, _ll]),
+    "qcs_loc_batch_sync_info": (_ll, [_vp, _vp, _i, _i, _vp, _ll, _vp, _P(QcsLocStats)]),
     "qcs_sim_params_default": (None, [_P(QcsSimParams)]),
     "qcs_sim_rois_host": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp, _vp]),
     "qcs_sim_rois_dev": (_i, [_P(QcsSimParams), _u64, _i, _i, _vp, _vp, _vp]),
@@ -416,6 +441,12 @@ class Context:
                         "warm": bool(e.warm), "within": bool(e.within), "chosen": bool(e.chosen)})
         return out
 
+    def loc_stats_dev(self, d_loc, n, stream=None):
+        """Statistics of n localization rows on the device (waits for the result)."""
+        st = QcsLocStats()
+        check(lib().qcs_loc_stats_dev(self.h, self._dp(d_loc), n, C.byref(st), stream), "qcs_loc_stats_dev")
+        return loc_stats_dict(st)
+
     def graph_mle(self, d_rois, n, kmax, d_loc, d_sel=None, net=None, stream=None):
         """A captured multi-emitter batch (plain, or warm with `net`); returns a Graph."""
         h = C.c_void_p()
@@ -469,6 +500,19 @@ class Context:
         if n < 0:
             check(int(-n), "qcs_loc_batch_sync")
         return out[:min(n, cap)].copy()
+
+    def loc_batch_sync_info(self, frames, start_frame=1, cap=None):
+        """lm_LocBatchedImg with its 8-float return array: (rows, info8, statistics dict)."""
+        frames = np.ascontiguousarray(frames, np.uint16)
+        cap = cap or 2 * MAX_POINT_NUM
+        out = np.zeros((cap, REC_FLOATS), np.float32)
+        info = np.zeros(8, np.float32)
+        st = QcsLocStats()
+        n = lib().qcs_loc_batch_sync_info(self.h, _ptr(frames), frames.shape[0], start_frame, _ptr(out), cap,
+                                          _ptr(info), C.byref(st))
+        if n < 0:
+            check(int(-n), "qcs_loc_batch_sync_info")
+        return out[:min(n, cap)].copy(), info, loc_stats_dict(st)
 
 
 class Session:

@@ -63,6 +63,9 @@ PROTOS = {
     "orc_render": (None, [_PP, _vp, _i, _vp, _vp]),
     "orc_mle_f64": (_i, [_vp, _i, _i, _i, _vp, _i, _P(C.c_double)]),
     "orc_fisher_crlb_f64": (_i, [_i, _i, _i, _vp, _vp]),
+    "orc_loc_stats": (_i, [_PP, _vp, _i, _P(Q.QcsLocStats)]),
+    "orc_activation_density": (_f, [_f, _f, _f]),
+    "orc_hist_mean": (_f, [_vp, _i, _f]),
 }
 
 _lib = None
@@ -206,3 +209,19 @@ def fisher_crlb(R, theta, K=1, astig=False):
     out = np.zeros_like(th)
     lib().orc_fisher_crlb_f64(R, K, int(astig), p(th), p(out))
     return out
+
+
+def loc_stats(params, loc):
+    loc = np.ascontiguousarray(loc, np.float32)
+    st = Q.QcsLocStats()
+    lib().orc_loc_stats(C.byref(params), p(loc), loc.shape[0], C.byref(st))
+    return Q.loc_stats_dict(st)
+
+
+def activation_density(r0, r1, radius_um):
+    return lib().orc_activation_density(r0, r1, radius_um)
+
+
+def hist_mean(h, pct):
+    h = np.ascontiguousarray(h, np.int32)
+    return lib().orc_hist_mean(p(h), h.shape[0], pct)

@@ -68,6 +68,27 @@ def test_loc_batch_sync_matches_oracle(gpu):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_loc_batch_sync_info_matches_oracle(gpu):
+    """lm_LocBatchedImg's return array: statistics of the batch's rows, per call."""
+    W, H = 256, 256
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.006)
+    frames = Q.sim_frames_host(sp, 72, 1, 6)
+    params = Q.default_params(QE=1.0, ImageWidth=W, ImageHigh=H)
+    ctx = Q.Context(params)
+    for f0, nf in ((1, 6), (3, 2)):
+        rows, info, st = ctx.loc_batch_sync_info(frames[f0 - 1:f0 - 1 + nf], start_frame=f0)
+        ref = _oracle_pipeline(params, [frames[f0 - 1:f0 - 1 + nf]], first_frame=f0)
+        assert np.array_equal(rows.view(np.uint32), ref.view(np.uint32))
+        rst = O.loc_stats(params, ref)
+        for k in ("valid", "density_total", "density_n0", "density_n1", "density_first_frame", "density_last_frame"):
+            assert st[k] == rst[k], k
+        np.testing.assert_array_equal(st["hist_peak_snr"], rst["hist_peak_snr"])
+        snr, psf = rst["mean_peak_snr"], rst["mean_psf_width"]
+        want = [snr, -psf, snr - psf, snr * -psf, ref.shape[0], rst["loc_density_2d"], 0, 0]
+        np.testing.assert_allclose(info, np.array(want, np.float32), rtol=1e-6, atol=1e-30)
+    assert info[5] == 0 or st["density_total"] > 50
+
+
 def test_session_rejects_unsupported_modes(gpu):
     with pytest.raises(Q.QcsError):
         Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, ConsecFitEn=1))
