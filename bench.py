@@ -380,6 +380,27 @@ def main():
                   "loc_density_2d_per_um2": st["loc_density_2d"], "mean_total_photon": st["mean_total_photon"],
                   "mean_loc_prec_nm": st["mean_loc_prec"]}
 
+    # consecutive-frame merging of the step's localizations (one call that gives everything back,
+    # 100 nm threshold), then removal of the merged-away rows
+    log("bench: consecutive-frame merging leg")
+    params_c = Q.QcsParams.from_buffer_copy(params)
+    params_c.ConsecFitEn = 1
+    params_c.ConsecFit_DistanceTh_nm = 100.0
+    ctx_c = Q.Context(params_c)
+    cons = Q.Consec(ctx_c)
+    c_out = torch.empty((st_rows, Q.REC_FLOATS), dtype=torch.float32, device=dev)
+    c_kept = torch.empty_like(c_out)
+    cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cm, ratio = cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
+    c_ms = (time.perf_counter() - t0) * 1e3 / reps
+    ck = ctx_c.compact_dev(c_out, cm, c_kept, sh)
+    torch.cuda.synchronize()
+    consec = {"rows": int(st_rows), "ms": c_ms, "rows_per_s": st_rows / (c_ms * 1e-3), "threshold_nm": 100.0,
+              "rows_after_merge": int(ck), "ontime_ratio": [float(x) for x in ratio]}
+
     multi = None
     if args.multi_rois > 0:
         log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
@@ -414,6 +435,23 @@ def main():
             "counts_equal": all(st_dev[k] == st_ref[k] for k in ("valid", "density_total", "density_n0", "density_n1")),
             "histograms_equal": all(np.array_equal(st_dev[k], st_ref[k]) for k in
                                     ("hist_total_photon", "hist_loc_prec", "hist_peak_snr", "hist_psf_width"))}
+        # consecutive-frame merging: the oracle (the reference's per-row scan, one thread) on the
+        # rows of the first 4 frames, against the device call on the same rows
+        sub = rows_h[rows_h[:, 11] <= 4]
+        ref_c = O.Consec(params_c)
+        t0 = time.perf_counter()
+        want, _ = ref_c.fit(sub, True)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        d_sub = torch.from_numpy(np.ascontiguousarray(sub)).to(dev)
+        d_o = torch.empty_like(d_sub)
+        gm, _ = cons.fit_dev(d_sub, sub.shape[0], True, d_o, sh)
+        got = d_o[:gm].cpu().numpy()
+        nan = np.isnan(got)
+        consec["cpu_baseline"] = {"value": sub.shape[0] / (cpu_ms * 1e-3), "unit": "rows/s", "cores": 1, "kind": "port",
+                                  "sample": "rows of frames 1-4 (%d rows)" % sub.shape[0]}
+        consec["parity_sample"] = {"rows": int(sub.shape[0]), "equal": bool(
+            got.shape == want.shape and np.array_equal(nan, np.isnan(want)) and
+            np.array_equal(got.view(np.uint32)[~nan], want.view(np.uint32)[~nan]))}
 
     line = {"metric": METRIC, "value": value, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
@@ -424,7 +462,9 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "fast_math": fast, "multi_emitter": multi, "statistics": statistics}
+            "fast_math": fast, "multi_emitter": multi, "statistics": statistics, "consecutive_merge": consec}
+    cons.close()
+    ctx_c.close()
     pipe.close()
     print(json.dumps(line), flush=True)
 

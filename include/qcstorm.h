@@ -35,6 +35,7 @@ extern "C" {
 #define QCS_ERR_STATE        4  /* call not valid in the current handle state */
 #define QCS_ERR_NOMEM        5  /* allocation failed */
 #define QCS_ERR_NODEVICE     6  /* no usable gfx950 device */
+#define QCS_ERR_IO           7  /* result file could not be created or written */
 
 /* ---- localization types (DLL/bfgsMLE_Loc_LD/bfgs_CommonPara.h:274-277) */
 #define QCS_LOCTYPE_GS2D     0
@@ -106,7 +107,7 @@ typedef struct qcs_params {
     int   LocType;           /* QCS_LOCTYPE_* */
     int   MultiEmitterFitEn;
     int   WLEEn;
-    int   ConsecFitEn;       /* accepted for interface parity; consecutive merging is not on this path */
+    int   ConsecFitEn;       /* session: merge molecules over consecutive frames (qcs_consec_*) */
     float ConsecFit_DistanceTh_nm;
     /* rendering */
     float PixelSize;         /* nm */
@@ -132,6 +133,7 @@ typedef struct qcs_params {
     int   FitMath;           /* QCS_MATH_EXACT (default) or QCS_MATH_FAST */
     int   MaxROIPerBatch;    /* ROI list capacity per extraction batch; default QCS_MAX_POINT_NUM */
     int   BatchedImageNum;   /* frames stacked per extraction batch (IJJ QC_STORM_.java:138-142); default 1 */
+    int   FlushROINum;       /* session: ROIs accumulated before a fit; 0 = PointNumTh (20480; /4 for AS3D) */
 } qcs_params;
 
 /* Fill `p` with LocalizationPara() defaults (bfgs_base.cu:18-86) and the library controls. */
@@ -361,6 +363,34 @@ typedef struct qcs_loc_stats {
 int qcs_loc_stats_dev(qcs_ctx *ctx, const float *d_loc, int n, qcs_loc_stats *out, void *stream);
 
 /* ========================================================================================
+ * Consecutive-frame merging (ConsecutiveFit_TypeDef::ConsecutiveFit_WeightedAvg,
+ * ConsecutiveFit/ConsecutiveFit_top.cu:21-101; kernels ConsecutiveFit.cu:18-220).  A qcs_consec
+ * holds back the rows of each call until the next one.  A call links every row of [held rows,
+ * d_rows] to the closest row of the next frame within ConsecFit_DistanceTh_nm / PixelSize pixels
+ * (positions truncated to integers, squared distance strictly below the squared threshold, ties
+ * to the lower row index; the reference's scan starts at the row's 32-row block, which matters
+ * only for rows of frame 0), replaces each chain's first row by the 10/CRLBx-weighted average of
+ * X, Y, Z, SigX, SigY with summed PPho, TotPho, Bkg and PSNR = sqrt(QE) PPho / sqrt(PPho + Bkg),
+ * and zeroes the chain's other rows.  The CRLB (qcs_crlb_dev) of the held rows is recomputed.
+ * d_out receives the held rows (is_end = 0; the call's rows are then held) or all rows
+ * (is_end = 1; nothing is held), zeroed rows included: remove them with qcs_compact_dev, as the
+ * reference does.  *n_out = number of rows written; out_cap must cover it.  d_rows must be sorted
+ * by frame (rows of frame 0 may appear anywhere), as qcs_compact_sorted_dev writes them; other
+ * orders fail with QCS_ERR_ARG and leave the held rows unchanged.  ontime_ratio (10 floats, may
+ * be NULL): fraction of this call's chains (heads among d_rows) lasting 1, 2, ... >= 10 frames.
+ * The parameters are the context's at call time; the context must outlive the qcs_consec.  The
+ * call waits for `stream`.
+ * ====================================================================================== */
+#define QCS_ONTIME_BINS 10   /* MaxOnTimeConsecutiveNum */
+typedef struct qcs_consec qcs_consec;
+int qcs_consec_create(qcs_ctx *ctx, qcs_consec **out);
+int qcs_consec_fit_dev(qcs_consec *cs, const float *d_rows, int n, int is_end, float *d_out, int out_cap, int *n_out,
+                       float *ontime_ratio, void *stream);
+int qcs_consec_held(const qcs_consec *cs);   /* rows held back (the reference's FluoNum_LastGroup) */
+int qcs_consec_reset(qcs_consec *cs);        /* drop the held rows */
+int qcs_consec_destroy(qcs_consec *cs);
+
+/* ========================================================================================
  * Per-batch HIP graphs (no counterpart in the reference).  One call of qcs_fit_mle_dev (net
  * NULL) or qcs_fit_warm_dev (statistics off) over fixed device buffers is captured into a HIP
  * graph; each qcs_graph_launch refits whatever d_rois holds at that time into d_loc / d_sel.  The
@@ -438,6 +468,21 @@ int qcs_session_get_sr_image(qcs_session *s, float *img);
 int qcs_session_get_max_disp_val(qcs_session *s, int *val);
 /* lm_ReleaseResource. */
 int qcs_session_destroy(qcs_session *s);
+/* The reference's LocFile (IJ/OnlineLocalizationLD.cpp:70-71, 283): every flush's final rows --
+ * frame-sorted, consecutive-frame merged when ConsecFitEn -- are appended to `path` as raw
+ * little-endian 12-float records (48 bytes each), by a writer thread of the session.  Call before
+ * the first qcs_session_feed; the file is complete when qcs_session_is_finished returns 1 (a
+ * write failure is reported as QCS_ERR_IO by the session calls). */
+int qcs_session_set_output(qcs_session *s, const char *path);
+/* FluoStatisticData_TypeDef after the last flush (GetStatisticalInf on the flush's rows,
+ * statisticsInfo_top.cu:61-128) and, when ConsecFitEn, the flush's on-time ratios
+ * (UpdateOntimeRatio).  Either pointer may be NULL. */
+int qcs_session_get_stats(qcs_session *s, qcs_loc_stats *out, float *ontime_ratio);
+/* SetLocDataFileName (IJ/LocResource.cpp:160-179): image name with the trailing '.', 't', 'i',
+ * 'f' characters removed (CString::TrimRight(".tif")), then "_result<LocType+2>D<ROISize>_<M|S>"
+ * and "_Consec<ConsecFit_DistanceTh_nm>nm" when ConsecFitEn, then ".txt".  Copies at most cap-1
+ * characters and a terminating 0 into out; returns the full length (<0 on error). */
+int qcs_result_file_name(const qcs_params *p, const char *image_name, char *out, int cap);
 
 /* lm_LocBatchedImg (MM/QC_STORM_CPPDLL.cpp:857-924): synchronous extract + fit + frame-sorted
  * compaction of `nframes` frames; returns the number of rows (<0 on error) and copies up to

@@ -123,6 +123,15 @@ int   orc_loc_stats(const qcs_params *p, const float *loc, int n, qcs_loc_stats 
 float orc_activation_density(float r0, float r1, float radius_um);
 float orc_hist_mean(const int *h, int len, float pct);
 
+/* consecutive-frame merging (orc_consec.c) */
+typedef struct orc_consec orc_consec;
+void  orc_consec_links(const float *loc, int n, float th_px, int *back, int *fwd);
+orc_consec *orc_consec_create(void);
+void  orc_consec_destroy(orc_consec *c);
+int   orc_consec_held(const orc_consec *c);
+int   orc_consec_fit(const qcs_params *p, orc_consec *c, const float *rows, int n, int is_end, float *out, int out_cap,
+                     float *ontime_ratio);
+
 #ifdef __cplusplus
 }
 #endif

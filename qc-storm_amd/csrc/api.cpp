@@ -79,6 +79,7 @@ int validate_params(const qcs_params &p)
     if (!(p.KAdc > 0.0f) || !(p.QE > 0.0f)) return fail(QCS_ERR_ARG, "KAdc and QE must be positive");
     if (p.FitMath != QCS_MATH_EXACT && p.FitMath != QCS_MATH_FAST) return fail(QCS_ERR_ARG, "FitMath %d invalid", p.FitMath);
     if (p.ImageWidth < 0 || p.ImageHigh < 0) return fail(QCS_ERR_ARG, "negative image size");
+    if (p.FlushROINum < 0) return fail(QCS_ERR_ARG, "FlushROINum must be >= 0 (got %d)", p.FlushROINum);
     return QCS_OK;
 }
 
@@ -165,6 +166,7 @@ void qcs_params_default(qcs_params *p)
     p->FitMath = QCS_MATH_EXACT;
     p->MaxROIPerBatch = QCS_MAX_POINT_NUM;
     p->BatchedImageNum = 1;
+    p->FlushROINum = 0;
     qcs_params_update_sr_size(p);
 }
 

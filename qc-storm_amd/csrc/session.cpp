@@ -3,13 +3,17 @@
 // The reference's online loop (IJ/OnlineLocalizationLD.cpp:52-360, th_OnlineLocalizationLD)
 // pops fed frame groups from a queue, extracts ROIs per group (frames numbered from 1), and once
 // PointNumTh (20480; /4 for AS3D) ROIs have accumulated -- or the run ends -- fits them, removes
-// rejected rows with frame sorting over [FirstFrame, EndFrame], appends the rows to the result
-// stream and renders them.  This file keeps that flow with HBM-resident accumulation: the ROI
-// records of successive groups are appended on the device, so the only host traffic is the frame
-// upload and the download of the compacted rows.
+// rejected rows with frame sorting over [FirstFrame, EndFrame], merges molecules over consecutive
+// frames when ConsecFitEn (the merged-away rows removed again), writes the rows to the result file,
+// renders them and computes their statistics.  This file keeps that flow with HBM-resident
+// accumulation: the ROI records of successive groups are appended on the device, so the only host
+// traffic is the frame upload and the download of the final rows.
 //
 // Threading: one worker thread per session owns the context and the HIP stream; the API threads
-// only touch the queue (mutex + condition variables) and read results under the same mutex.
+// only touch the queue (mutex + condition variables) and read results under the same mutex.  A
+// writer thread (when an output file is set) appends each flush's rows to the file, so the worker
+// never waits for the disk; the session reports finished only after the file is complete.
+#include <stdio.h>
 #include <string.h>
 #include <condition_variable>
 #include <deque>
@@ -52,13 +56,29 @@ struct qcs_session {
     std::string err_msg;
     std::vector<float> locs;       // frame-sorted rows, 12 floats each
 
+    qcs_loc_stats stats{};         // statistics of the last flush's rows (under mu)
+    float ontime[QCS_ONTIME_BINS] = {0};
+    long long flushes = 0;
+    long long fed = 0;             // frames accepted by qcs_session_feed (under mu)
+
     std::mutex dev_mu;             // device image access vs the worker's flush
 
     // device state (worker-owned)
-    DevBuf frames, rois, wle, loc, sorted, img, zimg;
-    HostBuf pin;                   // staging for the compacted rows
+    DevBuf frames, rois, wle, loc, sorted, img, zimg, merged, kept;
+    HostBuf pin;                   // staging for the final rows
+    qcs_consec *consec = nullptr;  // ConsecFitEn
     int acc_n = 0, first_frame = 0, end_frame = 0, cur_frame = 0, threshold = kPointNumTh, acc_cap = 0;
     long long dropped = 0;
+
+    // result file (the reference's LocFile): chunks handed to the writer thread
+    FILE *file = nullptr;
+    std::thread writer;
+    std::mutex wmu;
+    std::condition_variable wcv;
+    std::deque<std::vector<float>> wq;
+    bool wdone = false;
+    int werr = 0;
+    std::string wpath;
 };
 
 namespace {
@@ -81,31 +101,94 @@ int session_status(qcs_session *s)
     return QCS_OK;
 }
 
-// Fit the accumulated ROIs, compact/sort by frame, render and append the rows.
-int flush(qcs_session *s)
+void writer_main(qcs_session *s)
 {
-    if (s->acc_n == 0) return QCS_OK;
+    for (;;) {
+        std::vector<float> chunk;
+        {
+            std::unique_lock<std::mutex> g(s->wmu);
+            s->wcv.wait(g, [s] { return !s->wq.empty() || s->wdone; });
+            if (s->wq.empty()) return;   // done and drained
+            chunk.swap(s->wq.front());
+            s->wq.pop_front();
+        }
+        if (!s->werr && fwrite(chunk.data(), sizeof(float), chunk.size(), s->file) != chunk.size()) {
+            std::lock_guard<std::mutex> g(s->wmu);
+            s->werr = 1;
+        }
+    }
+}
+
+// ends the writer after it has written everything queued; closes the file (0 or an error code)
+int close_writer(qcs_session *s)
+{
+    if (!s->file) return QCS_OK;
+    {
+        std::lock_guard<std::mutex> g(s->wmu);
+        s->wdone = true;
+        s->wcv.notify_all();
+    }
+    if (s->writer.joinable()) s->writer.join();
+    const bool bad = s->werr || fclose(s->file) != 0;
+    s->file = nullptr;
+    if (bad) return fail(QCS_ERR_IO, "writing the result file %s failed", s->wpath.c_str());
+    return QCS_OK;
+}
+
+// Fit the accumulated ROIs, compact/sort by frame, merge consecutive frames, write, render, compute
+// statistics and append the rows.  is_end: the last flush (gives back the rows the merger holds).
+int flush(qcs_session *s, bool is_end)
+{
+    const bool consec = s->consec != nullptr;
+    if (s->acc_n == 0 && !(is_end && consec && qcs_consec_held(s->consec) > 0)) return QCS_OK;
     const int n = s->acc_n;
     std::lock_guard<std::mutex> dg(s->dev_mu);
     int rows = 0, m = 0, rc;
-    if ((rc = qcs_fit_dev(s->ctx, s->rois.as<uint16_t>(), s->wle.as<float>(), n, s->loc.as<float>(), 2 * s->acc_cap,
-                          &rows, s->s)))
-        return rc;
-    if ((rc = qcs_compact_sorted_dev(s->ctx, s->loc.as<float>(), rows, s->first_frame, s->end_frame,
-                                     s->sorted.as<float>(), &m, s->s)))
-        return rc;
-    if (m > 0) {
-        if ((rc = qcs_render_dev(s->ctx, s->sorted.as<float>(), m, s->img.as<float>(), s->zimg.as<float>(), s->s)))
+    if (n > 0) {
+        if ((rc = qcs_fit_dev(s->ctx, s->rois.as<uint16_t>(), s->wle.as<float>(), n, s->loc.as<float>(), 2 * s->acc_cap,
+                              &rows, s->s)))
             return rc;
+        if ((rc = qcs_compact_sorted_dev(s->ctx, s->loc.as<float>(), rows, s->first_frame, s->end_frame,
+                                         s->sorted.as<float>(), &m, s->s)))
+            return rc;
+    }
+    const float *res = s->sorted.as<float>();
+    float ontime[QCS_ONTIME_BINS] = {0};
+    if (consec) {
+        // ConsecutiveFit_WeightedAvg, then RemoveZeroLocalizations without sorting
+        // (IJ/OnlineLocalizationLD.cpp:240-256)
+        const int held = qcs_consec_held(s->consec);
+        const size_t cap = (size_t)held + (size_t)m;
+        const size_t rb = (size_t)QCS_REC_FLOATS * sizeof(float);
+        if ((rc = s->merged.ensure((cap > 0 ? cap : 1) * rb)) || (rc = s->kept.ensure((cap > 0 ? cap : 1) * rb))) return rc;
+        int k = 0;
+        if ((rc = qcs_consec_fit_dev(s->consec, s->sorted.as<float>(), m, is_end ? 1 : 0, s->merged.as<float>(), (int)cap, &k,
+                                     ontime, s->s)) ||
+            (rc = qcs_compact_dev(s->ctx, s->merged.as<float>(), k, s->kept.as<float>(), &m, s->s)))
+            return rc;
+        res = s->kept.as<float>();
+    }
+    qcs_loc_stats st;
+    if ((rc = qcs_loc_stats_dev(s->ctx, res, m, &st, s->s))) return rc;   // GetStatisticalInf per flush
+    if (m > 0) {
+        if ((rc = qcs_render_dev(s->ctx, res, m, s->img.as<float>(), s->zimg.as<float>(), s->s))) return rc;
         const size_t b = (size_t)m * QCS_REC_FLOATS * sizeof(float);
         if ((rc = s->pin.ensure(b))) return rc;
-        QCS_HIP(hipMemcpyAsync(s->pin.ptr, s->sorted.ptr, b, hipMemcpyDeviceToHost, s->s));
+        QCS_HIP(hipMemcpyAsync(s->pin.ptr, res, b, hipMemcpyDeviceToHost, s->s));
     }
     QCS_HIP(hipStreamSynchronize(s->s));
-    if (m > 0) {
+    const float *src = s->pin.as<float>();
+    if (m > 0 && s->file) {
+        std::lock_guard<std::mutex> g(s->wmu);
+        s->wq.emplace_back(src, src + (size_t)m * QCS_REC_FLOATS);
+        s->wcv.notify_one();
+    }
+    {
         std::lock_guard<std::mutex> g(s->mu);
-        const float *src = s->pin.as<float>();
-        s->locs.insert(s->locs.end(), src, src + (size_t)m * QCS_REC_FLOATS);
+        if (m > 0) s->locs.insert(s->locs.end(), src, src + (size_t)m * QCS_REC_FLOATS);
+        s->stats = st;
+        if (consec) memcpy(s->ontime, ontime, sizeof(ontime));
+        s->flushes++;
     }
     s->acc_n = 0;
     return QCS_OK;
@@ -130,7 +213,7 @@ int process(qcs_session *s, const FeedItem &it)
     s->cur_frame += it.nframes;
     s->end_frame = s->cur_frame;
     s->acc_n += no;
-    if (s->acc_n >= s->threshold) return flush(s);
+    if (s->acc_n >= s->threshold) return flush(s, false);
     return QCS_OK;
 }
 
@@ -163,7 +246,8 @@ void worker_main(qcs_session *s)
             return;
         }
     }
-    int rc = flush(s);
+    int rc = flush(s, true);
+    if (!rc) rc = close_writer(s);   // the file is complete before the session reports finished
     if (rc) {
         set_error(s, rc);
         return;
@@ -191,18 +275,18 @@ int qcs_session_create(const qcs_params *pin, qcs_session **out)
     int rc = validate_params(p);
     if (rc) return rc;
     if (p.ImageWidth <= 0 || p.ImageHigh <= 0) return fail(QCS_ERR_ARG, "ImageWidth/ImageHigh not set");
-    if (p.ConsecFitEn) return fail(QCS_ERR_ARG, "ConsecFitEn (consecutive-frame merging) is not supported");
     if (p.LocType == QCS_LOCTYPE_DH3D) return fail(QCS_ERR_ARG, "LocType DH3D is not supported");
     if (p.SRImageWidth <= 16 || p.SRImageHigh <= 16) qcs_params_update_sr_size(&p);
     qcs_session *s = new (std::nothrow) qcs_session();
     if (!s) return fail(QCS_ERR_NOMEM, "session allocation failed");
     s->p = p;
-    s->threshold = p.LocType == QCS_LOCTYPE_AS3D ? kPointNumTh / 4 : kPointNumTh;
+    s->threshold = p.FlushROINum > 0 ? p.FlushROINum : (p.LocType == QCS_LOCTYPE_AS3D ? kPointNumTh / 4 : kPointNumTh);
     const int batch_cap = p.MaxROIPerBatch > 0 ? p.MaxROIPerBatch : QCS_MAX_POINT_NUM;
     s->acc_cap = s->threshold + batch_cap;
     const int R = p.ROISize;
     const size_t npx = (size_t)p.SRImageWidth * p.SRImageHigh;
     auto cleanup = [s](int code) {
+        if (s->consec) qcs_consec_destroy(s->consec);
         if (s->ctx) qcs_ctx_destroy(s->ctx);
         if (s->s) (void)hipStreamDestroy(s->s);
         DevBuf *b[] = {&s->frames, &s->rois, &s->wle, &s->loc, &s->sorted, &s->img, &s->zimg};
@@ -213,6 +297,7 @@ int qcs_session_create(const qcs_params *pin, qcs_session **out)
     };
     if (hipGetDevice(&s->device) != hipSuccess) return cleanup(fail(QCS_ERR_NODEVICE, "no HIP device"));
     if ((rc = qcs_ctx_create(&p, &s->ctx))) return cleanup(rc);
+    if (p.ConsecFitEn && (rc = qcs_consec_create(s->ctx, &s->consec))) return cleanup(rc);
     if (hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(QCS_ERR_HIP, "hipStreamCreate failed"));
     if ((rc = s->rois.ensure((size_t)s->acc_cap * QCS_ROI_WORDS(R) * sizeof(uint16_t))) ||
@@ -269,6 +354,7 @@ int qcs_session_feed(qcs_session *s, const uint16_t *frames, int nframes)
     memcpy(it.h, frames, bytes);
     it.nframes = nframes;
     std::lock_guard<std::mutex> g(s->mu);
+    s->fed += nframes;
     s->queue.push_back(it);
     s->cv_work.notify_one();
     return QCS_OK;
@@ -350,18 +436,74 @@ int qcs_session_destroy(qcs_session *s)
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    (void)close_writer(s);
     (void)hipSetDevice(s->device);
     for (auto &it : s->queue) (void)hipHostFree(it.h);
     s->queue.clear();
     free_items(s->spare);
     if (s->s) (void)hipStreamSynchronize(s->s);
-    DevBuf *b[] = {&s->frames, &s->rois, &s->wle, &s->loc, &s->sorted, &s->img, &s->zimg};
+    DevBuf *b[] = {&s->frames, &s->rois, &s->wle, &s->loc, &s->sorted, &s->img, &s->zimg, &s->merged, &s->kept};
     for (auto *x : b) x->release();
     s->pin.release();
+    if (s->consec) qcs_consec_destroy(s->consec);
     if (s->ctx) qcs_ctx_destroy(s->ctx);
     if (s->s) (void)hipStreamDestroy(s->s);
     delete s;
     return QCS_OK;
+}
+
+int qcs_session_set_output(qcs_session *s, const char *path)
+{
+    if (!s || !path || !*path) return fail(QCS_ERR_ARG, "invalid argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->finishing || s->fed > 0)
+            return fail(QCS_ERR_STATE, "the result file must be set before the first frames are fed");
+    }
+    if (s->file) return fail(QCS_ERR_STATE, "the result file is already set");
+    FILE *f = fopen(path, "wb");
+    if (!f) return fail(QCS_ERR_IO, "cannot create the result file %s", path);
+    s->wpath = path;
+    s->file = f;
+    try {
+        s->writer = std::thread(writer_main, s);
+    } catch (...) {
+        fclose(f);
+        s->file = nullptr;
+        return fail(QCS_ERR_NOMEM, "could not start the result writer");
+    }
+    return QCS_OK;
+}
+
+int qcs_session_get_stats(qcs_session *s, qcs_loc_stats *out, float *ontime_ratio)
+{
+    if (!s || (!out && !ontime_ratio)) return fail(QCS_ERR_ARG, "invalid argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->err) return fail(s->err, "session worker failed: %s", s->err_msg.c_str());
+    if (out) *out = s->stats;
+    if (ontime_ratio) memcpy(ontime_ratio, s->ontime, sizeof(s->ontime));
+    return QCS_OK;
+}
+
+int qcs_result_file_name(const qcs_params *p, const char *image_name, char *out, int cap)
+{
+    if (!p || !image_name || cap < 0 || (cap > 0 && !out)) return -fail(QCS_ERR_ARG, "invalid argument");
+    // SetLocDataFileName (IJ/LocResource.cpp:160-179): CString::TrimRight(".tif") drops every
+    // trailing character found in the set {'.', 't', 'i', 'f'}, then the suffix is appended
+    std::string base = image_name;
+    while (!base.empty() && strchr(".tif", base.back()) != nullptr) base.pop_back();
+    char consec[64] = "";
+    if (p->ConsecFitEn) snprintf(consec, sizeof(consec), "_Consec%.0fnm", p->ConsecFit_DistanceTh_nm);
+    char suffix[128];
+    snprintf(suffix, sizeof(suffix), "_result%dD%d%s%s.txt", p->LocType + 2, p->ROISize, p->MultiEmitterFitEn ? "_M" : "_S",
+             consec);
+    const std::string name = base + suffix;
+    if (out && cap > 0) {
+        const size_t k = name.size() < (size_t)cap - 1 ? name.size() : (size_t)cap - 1;
+        memcpy(out, name.data(), k);
+        out[k] = 0;
+    }
+    return (int)name.size();
 }
 
 long long qcs_loc_batch_sync(qcs_ctx *c, const uint16_t *frames, int nframes, int start_frame, float *out12,

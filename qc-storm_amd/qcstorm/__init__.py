@@ -46,7 +46,7 @@ class QcsParams(C.Structure):
         ("p4_XLY", C.c_float), ("p3_XLY", C.c_float), ("p2_XLY", C.c_float), ("p1_XLY", C.c_float),
         ("p0_XLY", C.c_float),
         ("RenderingMode", C.c_int), ("FixedLocPrec", C.c_float),
-        ("FitMath", C.c_int), ("MaxROIPerBatch", C.c_int), ("BatchedImageNum", C.c_int),
+        ("FitMath", C.c_int), ("MaxROIPerBatch", C.c_int), ("BatchedImageNum", C.c_int), ("FlushROINum", C.c_int),
     ]
 
 
@@ -128,6 +128,45 @@ class Graph:
             pass
 
 
+ONTIME_BINS = 10
+
+
+class Consec:
+    """Consecutive-frame merger (qcs_consec_*): rows of each call are held back until the next."""
+
+    def __init__(self, ctx):
+        h = C.c_void_p()
+        check(lib().qcs_consec_create(ctx.h, C.byref(h)), "qcs_consec_create")
+        self.h = h
+        self.ctx = ctx   # the context must outlive the merger
+
+    def fit_dev(self, d_rows, n, is_end, d_out, stream=None):
+        """Returns (rows written to d_out, on-time ratios of the call's chains)."""
+        m = C.c_int()
+        ratio = np.zeros(ONTIME_BINS, np.float32)
+        cap = d_out.shape[0] if d_out is not None else 0
+        check(lib().qcs_consec_fit_dev(self.h, Context._dp(d_rows), n, int(is_end), Context._dp(d_out), cap,
+                                       C.byref(m), C.c_void_p(ratio.ctypes.data), stream), "qcs_consec_fit_dev")
+        return m.value, ratio
+
+    def held(self):
+        return lib().qcs_consec_held(self.h)
+
+    def reset(self):
+        check(lib().qcs_consec_reset(self.h), "qcs_consec_reset")
+
+    def close(self):
+        if self.h:
+            lib().qcs_consec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class QcsRoiSet(C.Structure):
     _fields_ = [("roi_size", C.c_int), ("n", C.c_int), ("d_rois", C.c_void_p), ("d_loc", C.c_void_p),
                 ("d_sel", C.c_void_p)]
@@ -172,6 +211,11 @@ PROTOTYPES = {
     "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
     "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
     "qcs_graph_create_mle": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _P(_vp)]),
+    "qcs_consec_create": (_i, [_vp, _P(_vp)]),
+    "qcs_consec_fit_dev": (_i, [_vp, _vp, _i, _i, _vp, _i, _P(_i), _vp, _vp]),
+    "qcs_consec_held": (_i, [_vp]),
+    "qcs_consec_reset": (_i, [_vp]),
+    "qcs_consec_destroy": (_i, [_vp]),
     "qcs_graph_launch": (_i, [_vp, _vp]),
     "qcs_loc_stats_dev": (_i, [_vp, _vp, _i, _P(QcsLocStats), _vp]),
     "qcs_autotune_mle_dev": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _P(QcsAutotuneOpts), _P(QcsAutotuneEntry), _i, _P(_i), _vp]),
@@ -199,6 +243,9 @@ PROTOTYPES = {
     "qcs_session_get_sr_image": (_i, [_vp, _vp]),
     "qcs_session_get_max_disp_val": (_i, [_vp, _P(_i)]),
     "qcs_session_destroy": (_i, [_vp]),
+    "qcs_session_set_output": (_i, [_vp, C.c_char_p]),
+    "qcs_session_get_stats": (_i, [_vp, _P(QcsLocStats), _vp]),
+    "qcs_result_file_name": (_i, [_P(QcsParams), C.c_char_p, C.c_char_p, _i]),
     "qcs_loc_batch_sync": (_ll, [_vp, _vp, _i, _i, _vp, _ll]),
     "qcs_loc_batch_sync_info": (_ll, [_vp, _vp, _i, _i, _vp, _ll, _vp, _P(QcsLocStats)]),
     "qcs_sim_params_default": (None, [_P(QcsSimParams)]),
@@ -257,6 +304,16 @@ def default_params(**kw):
         setattr(p, k, v)
     lib().qcs_params_update_sr_size(C.byref(p))
     return p
+
+
+def result_file_name(params, image_name):
+    """SetLocDataFileName: the result file name the reference derives from the image name."""
+    n = lib().qcs_result_file_name(C.byref(params), image_name.encode(), None, 0)
+    if n < 0:
+        check(int(-n), "qcs_result_file_name")
+    buf = C.create_string_buffer(n + 1)
+    lib().qcs_result_file_name(C.byref(params), image_name.encode(), buf, n + 1)
+    return buf.value.decode()
 
 
 def default_sim(**kw):
@@ -564,6 +621,17 @@ class Session:
         v = C.c_int()
         check(lib().qcs_session_get_max_disp_val(self.h, C.byref(v)), "qcs_session_get_max_disp_val")
         return v.value
+
+    def set_output(self, path):
+        """Append every flush's final rows to `path` (raw 12-float records); before the first feed."""
+        check(lib().qcs_session_set_output(self.h, os.fsencode(path)), "qcs_session_set_output")
+
+    def stats(self):
+        """(statistics dict of the last flush's rows, on-time ratios of its chains)."""
+        st = QcsLocStats()
+        ratio = np.zeros(ONTIME_BINS, np.float32)
+        check(lib().qcs_session_get_stats(self.h, C.byref(st), _ptr(ratio)), "qcs_session_get_stats")
+        return loc_stats_dict(st), ratio
 
     def close(self):
         if self.h:

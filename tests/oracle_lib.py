@@ -66,6 +66,11 @@ PROTOS = {
     "orc_loc_stats": (_i, [_PP, _vp, _i, _P(Q.QcsLocStats)]),
     "orc_activation_density": (_f, [_f, _f, _f]),
     "orc_hist_mean": (_f, [_vp, _i, _f]),
+    "orc_consec_links": (None, [_vp, _i, _f, _vp, _vp]),
+    "orc_consec_create": (_vp, []),
+    "orc_consec_destroy": (None, [_vp]),
+    "orc_consec_held": (_i, [_vp]),
+    "orc_consec_fit": (_i, [_PP, _vp, _vp, _i, _i, _vp, _i, _vp]),
 }
 
 _lib = None
@@ -225,3 +230,40 @@ def activation_density(r0, r1, radius_um):
 def hist_mean(h, pct):
     h = np.ascontiguousarray(h, np.int32)
     return lib().orc_hist_mean(p(h), h.shape[0], pct)
+
+
+def consec_links(loc, th_px):
+    loc = np.ascontiguousarray(loc, np.float32)
+    n = loc.shape[0]
+    back = np.zeros(max(n, 1), np.int32)
+    fwd = np.zeros(max(n, 1), np.int32)
+    lib().orc_consec_links(p(loc), n, th_px, p(back), p(fwd))
+    return back[:n], fwd[:n]
+
+
+class Consec:
+    """The oracle's consecutive-frame merger (held-back group between calls)."""
+
+    def __init__(self, params):
+        self.params = params
+        self.h = lib().orc_consec_create()
+
+    def held(self):
+        return lib().orc_consec_held(self.h)
+
+    def fit(self, rows, is_end):
+        rows = np.ascontiguousarray(rows, np.float32).reshape(-1, 12)
+        cap = self.held() + rows.shape[0]
+        out = np.zeros((max(cap, 1), 12), np.float32)
+        ratio = np.zeros(10, np.float32)
+        m = lib().orc_consec_fit(C.byref(self.params), self.h, p(rows), rows.shape[0], int(is_end), p(out), cap, p(ratio))
+        assert m >= 0
+        return out[:m].copy(), ratio
+
+    def close(self):
+        if self.h:
+            lib().orc_consec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -56,6 +56,91 @@ def test_session_matches_oracle_pipeline(gpu, lt, mu, R, group):
     assert mx == max(int(img.max()), 1)
 
 
+def _oracle_online(params, groups, threshold):
+    """The online loop with flushes (IJ/OnlineLocalizationLD.cpp:195-300): ROIs of fed groups
+    accumulate until `threshold` (or the end); each flush fits them, removes rejected rows with
+    frame sorting, merges consecutive frames (held-back group; the end flush gives everything
+    back) and removes the merged-away rows.  Returns the rows of every flush."""
+    cs = O.Consec(params) if params.ConsecFitEn else None
+    out, rois, wles = [], [], []
+    f, first = 1, 1
+
+    def flush(end):
+        nonlocal rois, wles, first
+        if rois:
+            loc, _ = O.fit(params, np.concatenate(rois), np.concatenate(wles))
+            rows = O.compact_sorted(loc, first, f - 1)
+        else:
+            rows = np.zeros((0, 12), np.float32)
+        if cs is not None:
+            if not rois and not (end and cs.held()):
+                return
+            merged, ratio = cs.fit(rows, end)
+            rows = O.compact(merged)
+        elif not rois:
+            return
+        out.append(rows)
+        rois, wles = [], []
+
+    for g in groups:
+        p = Q.default_params(**{k: getattr(params, k) for k, _ in Q.QcsParams._fields_})
+        p.BatchedImageNum = g.shape[0]
+        r, w, _, _ = O.extract(p, g, start_frame=f, cap=200000)
+        if not rois:
+            first = f
+        rois.append(r)
+        wles.append(w)
+        f += g.shape[0]
+        if sum(x.shape[0] for x in rois) >= threshold:
+            flush(False)
+    flush(True)
+    return out
+
+
+@pytest.mark.parametrize("consec,th_nm", [(1, 100.0), (1, 160.0), (0, 100.0)])
+def test_session_flushes_consec_and_result_file(gpu, tmp_path, consec, th_nm):
+    W, H, nf = 128, 128, 30
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.004)
+    frames = Q.sim_frames_host(sp, 900 + consec, 1, nf)
+    params = Q.default_params(QE=1.0, ImageWidth=W, ImageHigh=H, ConsecFitEn=consec, ConsecFit_DistanceTh_nm=th_nm,
+                              FlushROINum=300)
+    groups = [frames[i:i + 2] for i in range(0, nf, 2)]
+    s = Q.Session(params)
+    path = tmp_path / Q.result_file_name(params, "movie.tif")
+    s.set_output(str(path))
+    for g in groups:
+        s.feed(g)
+    s.finish()
+    s.wait()
+    got = s.locs()
+    st, ratio = s.stats()
+    s.close()
+    flushes = _oracle_online(params, groups, 300)
+    ref = np.concatenate(flushes)
+    print("consec", consec, th_nm, "flushes", len(flushes), "rows", got.shape[0], ref.shape[0])
+    assert len(flushes) >= 3
+    assert got.shape == ref.shape and ref.shape[0] > 100
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    on_disk = np.fromfile(path, dtype="<f4").reshape(-1, 12)
+    assert np.array_equal(on_disk.view(np.uint32), got.view(np.uint32))
+    rst = O.loc_stats(params, flushes[-1])
+    for k in ("valid", "density_total", "density_n0", "density_n1"):
+        assert st[k] == rst[k], k
+    if consec:
+        assert ratio.sum() == pytest.approx(1.0, abs=1e-6) or ratio.sum() == 0.0
+
+
+def test_session_output_errors(gpu, tmp_path):
+    params = Q.default_params(QE=1.0, ImageWidth=64, ImageHigh=64)
+    s = Q.Session(params)
+    with pytest.raises(Q.QcsError):
+        s.set_output(str(tmp_path / "no_such_dir" / "x.txt"))
+    s.feed(np.zeros((1, 64, 64), np.uint16) + 100)
+    with pytest.raises(Q.QcsError):
+        s.set_output(str(tmp_path / "late.txt"))
+    s.close()
+
+
 def test_loc_batch_sync_matches_oracle(gpu):
     W, H = 256, 256
     sp = Q.default_sim(width=W, height=H, density_per_px=0.003)
@@ -91,6 +176,6 @@ def test_loc_batch_sync_info_matches_oracle(gpu):
 
 def test_session_rejects_unsupported_modes(gpu):
     with pytest.raises(Q.QcsError):
-        Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, ConsecFitEn=1))
+        Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, FlushROINum=-1))
     with pytest.raises(Q.QcsError):
         Q.Session(Q.default_params(ImageWidth=64, ImageHigh=64, LocType=Q.LOCTYPE_DH3D))
