@@ -81,6 +81,33 @@ def cpu_baseline(params, rois, wle, seconds):
                        % (n, dt, threads)}, loc)
 
 
+def drift_workload(W, H, nframes, npts, seed, ax=0.005, ay=-0.003, wobble=0.1):
+    """Frame-sorted rows of a fixed structure (npts points, each on in half of the frames)
+    localized under a known drift: (rows, drift x per frame, drift y per frame)."""
+    rng = np.random.default_rng(seed)
+    pts = np.stack([rng.uniform(8, W - 8, npts), rng.uniform(8, H - 8, npts)], 1).astype(np.float32)
+    f = np.arange(1, nframes + 1, dtype=np.float64)
+    dx = (ax * f + wobble * np.sin(f / 25.0)).astype(np.float32)
+    dy = (ay * f + wobble * (np.cos(f / 19.0) - 1.0)).astype(np.float32)
+    out = []
+    for k in range(nframes):
+        on = rng.random(npts) < 0.5
+        m = int(on.sum())
+        r = np.zeros((m, 12), np.float32)
+        r[:, 0] = rng.uniform(200, 2000, m)
+        r[:, 1] = pts[on, 0] + dx[k] + rng.normal(0, 0.03, m).astype(np.float32)
+        r[:, 2] = pts[on, 1] + dy[k] + rng.normal(0, 0.03, m).astype(np.float32)
+        r[:, 4:6] = 1.3
+        r[:, 6] = r[:, 0] * 6
+        r[:, 7] = 20
+        r[:, 8] = rng.uniform(8, 40, m)
+        r[:, 9] = rng.uniform(6, 14, m)
+        r[:, 10] = r[:, 9]
+        r[:, 11] = k + 1
+        out.append(r)
+    return np.concatenate(out), dx, dy
+
+
 def match_rmse(loc, truth, pixel_size):
     """Greedy nearest matching (within 1 px) of fitted rows [n][3][12] to simulated emitters
     [n][3][8] of 7x7 ROIs (header Xc = Yc = 100); returns rmse (px), recall, precision and the
@@ -401,6 +428,41 @@ def main():
     consec = {"rows": int(st_rows), "ms": c_ms, "rows_per_s": st_rows / (c_ms * 1e-3), "threshold_nm": 100.0,
               "rows_after_merge": int(ck), "ontime_ratio": [float(x) for x in ratio]}
 
+    # drift correction by SR-image cross-correlation: a fixed structure (25,000 points, 12,500
+    # localizations per frame) drifting over 200 frames of 2048 x 2048 px, groups of 50 frames
+    log("bench: drift-correction leg")
+    d_rows_h, d_dx, d_dy = drift_workload(W, H, F, 25000, 77)
+    params_d = Q.QcsParams.from_buffer_copy(params)
+    ctx_d = Q.Context(params_d)
+    d_rows = torch.from_numpy(d_rows_h).to(dev)
+    corr = 50
+    ctx_d.drift_correct_dev(d_rows.clone(), d_rows_h.shape[0], corr, stream=sh)   # warm-up (allocates the SR images)
+    ctx_d.timing_enable(True)
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        work = d_rows.clone()
+        torch.cuda.synchronize()
+        total_d, dfx, dfy, dgxy = ctx_d.drift_correct_dev(work, d_rows_h.shape[0], corr, stream=sh)
+    dr_ms = (time.perf_counter() - t0) * 1e3 / reps
+    td = ctx_d.timing()
+    cc_launches = max(td["drift_cc_launches"], 1)
+    cc_ms = td["drift_cc_ms"] / cc_launches
+    cc_flop = 2.0 * 47 * 47 * td["drift_cc_pixels"] / cc_launches
+    cc_tflops = cc_flop / (cc_ms * 1e-3) / 1e12
+    from_frames = [(1 + k * (corr // 2), k * (corr // 2) + corr) for k in range(dgxy.shape[0])]
+    mdx = np.array([d_dx[a - 1:b].mean() for a, b in from_frames])
+    mdy = np.array([d_dy[a - 1:b].mean() for a, b in from_frames])
+    drift = {"rows": int(d_rows_h.shape[0]), "frames": int(total_d), "corr_frames": corr, "groups": int(dgxy.shape[0]),
+             "ms": dr_ms, "rows_per_s": d_rows_h.shape[0] / (dr_ms * 1e-3),
+             "cc_kernel": {"avg_launch_ms": cc_ms, "launches_per_call": cc_launches // reps,
+                           "sr_pixels": td["drift_cc_pixels"] // cc_launches,
+                           "roofline": {"bound": "mfma", "achieved": cc_tflops, "peak": 157.3, "unit": "TFLOP/s",
+                                        "frac": cc_tflops / 157.3, "flop_per_launch": cc_flop}},
+             "group_shift_err_px": float(max(np.abs(dgxy[:, 0] - (mdx - mdx[0])).max(),
+                                             np.abs(dgxy[:, 1] - (mdy - mdy[0])).max()))}
+    ctx_d.timing_enable(False)
+
     multi = None
     if args.multi_rois > 0:
         log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
@@ -452,6 +514,26 @@ def main():
         consec["parity_sample"] = {"rows": int(sub.shape[0]), "equal": bool(
             got.shape == want.shape and np.array_equal(nan, np.isnan(want)) and
             np.array_equal(got.view(np.uint32)[~nan], want.view(np.uint32)[~nan]))}
+        # drift correction: the oracle (the reference's per-shift passes, one thread) on a 96 x 96 px
+        # crop of the drifting structure, 60 frames in groups of 20 (4 correlations of 768^2 SR px)
+        p_crop = Q.QcsParams.from_buffer_copy(params)
+        p_crop.ImageWidth = p_crop.ImageHigh = 96
+        crop, _, _ = drift_workload(96, 96, 60, 1200, 78)
+        t0 = time.perf_counter()
+        O.drift_correct(p_crop, crop, 20)
+        cpu_ms = (time.perf_counter() - t0) * 1e3
+        drift["cpu_baseline"] = {"value": 4 * 768 * 768 * 47 * 47 / (cpu_ms * 1e-3), "unit": "SR pixel-shift products/s",
+                                 "cores": 1, "kind": "port",
+                                 "sample": "96x96 px crop, 60 frames, 4 correlations of 768x768 SR px (%.1f s)" % (cpu_ms / 1e3)}
+        drift["pixel_shift_products_per_s"] = td["drift_cc_pixels"] * 47 * 47 / (td["drift_cc_ms"] * 1e-3)
+        d_crop = torch.from_numpy(crop.copy()).to(dev)
+        ctx_crop = Q.Context(p_crop)
+        _, _, _, gxy_crop = ctx_crop.drift_correct_dev(d_crop, crop.shape[0], 20, stream=sh)
+        _, _, _, _, rgx, rgy = O.drift_correct(p_crop, crop, 20)
+        drift["parity_sample"] = {"groups": int(rgx.shape[0]),
+                                  "max_group_shift_diff_px": float(max(np.abs(gxy_crop[:, 0] - rgx).max(),
+                                                                       np.abs(gxy_crop[:, 1] - rgy).max()))}
+        ctx_crop.close()
 
     line = {"metric": METRIC, "value": value, "unit": "fits/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps, "higher_is_better": True,
@@ -462,9 +544,11 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "fast_math": fast, "multi_emitter": multi, "statistics": statistics, "consecutive_merge": consec}
+            "fast_math": fast, "multi_emitter": multi, "statistics": statistics, "consecutive_merge": consec,
+            "drift_correction": drift}
     cons.close()
     ctx_c.close()
+    ctx_d.close()
     pipe.close()
     print(json.dumps(line), flush=True)
 

@@ -205,6 +205,8 @@ typedef struct qcs_timing {
     long long fit_1e_launches, fit_1e_rois, fit_me_launches;
     double cfit_ms, nn_ms;               /* converged fitting (qcs_cfit_dev, qcs_fit_mle_dev, warm path), network */
     long long cfit_launches, cfit_rois, nn_launches;
+    double drift_cc_ms;                  /* drift correction: the cross-correlation kernel */
+    long long drift_cc_launches, drift_cc_pixels;   /* launches; SR pixels correlated (47 x 47 shifts each) */
 } qcs_timing;
 int qcs_timing_enable(qcs_ctx *ctx, int on);
 int qcs_timing_get(qcs_ctx *ctx, qcs_timing *out);
@@ -389,6 +391,28 @@ int qcs_consec_fit_dev(qcs_consec *cs, const float *d_rows, int n, int is_end, f
 int qcs_consec_held(const qcs_consec *cs);   /* rows held back (the reference's FluoNum_LastGroup) */
 int qcs_consec_reset(qcs_consec *cs);        /* drop the held rows */
 int qcs_consec_destroy(qcs_consec *cs);
+
+/* ========================================================================================
+ * Drift correction by cross-correlation of super-resolution images
+ * (SRDriftCorrData_TypeDef::CorrectSampleShift, DriftCorrection_SRImageXCorr_top.cu:18-74; group
+ * bookkeeping and interpolation DriftCorrection_SRImageXCorr_rc.cu:20-395; kernels
+ * DriftCorrection_SRImageXCorr.cu:21-212).  d_rows: n frame-sorted rows (a result file's
+ * contents), corrected in place.  The frame count is the last non-zero frame among the last 400
+ * rows; the frames form groups of corr_frames overlapping by half (frames / corr_frames >= 1, at
+ * most QCS_DRIFT_MAX_GROUPS groups).  Each group's rows with PSNR >= SNR_th and PPho > 1 are
+ * rendered at 8x zoom of ImageWidth x ImageHigh as unit-weight Gaussians of sigma CRLBx (35 nm
+ * outside (0.01, 35) nm); each group is correlated with group 0 over 47 x 47 SR-pixel shifts
+ * centred on the previous group's integer shift, and the centre of mass of the peak region
+ * (radius 12) gives its shift.  Shifts are interpolated linearly between group centres (from 0 at
+ * frame 1) and subtracted from X and Y of rows of frames 1..frames (other rows are left as they
+ * are).  shift_x / shift_y (may be NULL; shift_cap entries) receive the per-frame shifts in
+ * pixels (index frame - 1), group_shift_xy (may be NULL; 2 x QCS_DRIFT_MAX_GROUPS floats) the
+ * group shifts (x, y), *total_frames the frame count.  Uses 2 x 64 x ImageWidth x ImageHigh
+ * floats of device scratch; the call waits for `stream`.
+ * ====================================================================================== */
+#define QCS_DRIFT_MAX_GROUPS 512   /* MaxCorrGroupNum */
+int qcs_drift_correct_dev(qcs_ctx *ctx, float *d_rows, long long n, int corr_frames, float *shift_x, float *shift_y,
+                          int shift_cap, int *total_frames, float *group_shift_xy, void *stream);
 
 /* ========================================================================================
  * Per-batch HIP graphs (no counterpart in the reference).  One call of qcs_fit_mle_dev (net

@@ -132,6 +132,18 @@ int   orc_consec_held(const orc_consec *c);
 int   orc_consec_fit(const qcs_params *p, orc_consec *c, const float *rows, int n, int is_end, float *out, int out_cap,
                      float *ontime_ratio);
 
+/* drift correction by SR-image cross-correlation (orc_drift.c) */
+int    orc_drift_total_frame(const float *loc, long long n);
+int    orc_drift_groups(int total, int corr, int *gs, int *ge);
+void   orc_drift_group_rows(const float *loc, long long n, int ng, const int *gs, const int *ge, long long *first,
+                            long long *last);
+void   orc_drift_render(const qcs_params *p, const float *loc, long long n, int srw, int srh, float *img);
+double orc_drift_cc(const float *img1, const float *img2, int w, int h, int sx, int sy);
+void   orc_drift_peak(const double *cc, int bias_x, int bias_y, float *shift_x, float *shift_y);
+void   orc_drift_interp(int total, int ng, const int *gs, const int *ge, const float *gx, const float *gy, float *fx, float *fy);
+int    orc_drift_correct(const qcs_params *p, float *loc, long long n, int corr, float *fx, float *fy, int fcap,
+                         int *total_out, float *gx_out, float *gy_out);
+
 #ifdef __cplusplus
 }
 #endif

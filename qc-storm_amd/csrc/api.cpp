@@ -226,7 +226,8 @@ int qcs_ctx_destroy(qcs_ctx *c)
     qcs::DevBuf *bufs[] = {&c->ex_bkg, &c->ex_tmp, &c->ex_det, &c->ex_mark, &c->ex_cnt, &c->ex_xy, &c->ex_var,
                            &c->ex_frames, &c->fit_idx1, &c->fit_idx2, &c->fit_idx3, &c->fit_flag, &c->fit_cnt,
                            &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev, &c->cf_theta, &c->cf_ll,
-                           &c->cf_list, &c->cf_rows, &c->cf_queue, &c->rn_buf, &c->st_buf};
+                           &c->cf_list, &c->cf_rows, &c->cf_queue, &c->rn_buf, &c->st_buf,
+                           &c->dr_img, &c->dr_cc, &c->dr_shift, &c->dr_frames};
     for (auto *b : bufs) b->release();
     c->pin_cnt.release();
     (void)timer_resolve(c->timer);
@@ -271,6 +272,9 @@ int qcs_timing_get(qcs_ctx *c, qcs_timing *out)
     out->cfit_launches = t.launches[TM_CFIT];
     out->cfit_rois = t.items[TM_CFIT];
     out->nn_launches = t.launches[TM_NN];
+    out->drift_cc_ms = t.ms[TM_DRIFTCC];
+    out->drift_cc_launches = t.launches[TM_DRIFTCC];
+    out->drift_cc_pixels = t.items[TM_DRIFTCC];
     return QCS_OK;
 }
 

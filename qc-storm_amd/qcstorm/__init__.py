@@ -179,6 +179,7 @@ class QcsTiming(C.Structure):
         ("fit_1e_launches", C.c_longlong), ("fit_1e_rois", C.c_longlong), ("fit_me_launches", C.c_longlong),
         ("cfit_ms", C.c_double), ("nn_ms", C.c_double),
         ("cfit_launches", C.c_longlong), ("cfit_rois", C.c_longlong), ("nn_launches", C.c_longlong),
+        ("drift_cc_ms", C.c_double), ("drift_cc_launches", C.c_longlong), ("drift_cc_pixels", C.c_longlong),
     ]
 
 
@@ -211,6 +212,7 @@ PROTOTYPES = {
     "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
     "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
     "qcs_graph_create_mle": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _P(_vp)]),
+    "qcs_drift_correct_dev": (_i, [_vp, _vp, _ll, _i, _vp, _vp, _i, _P(_i), _vp, _vp]),
     "qcs_consec_create": (_i, [_vp, _P(_vp)]),
     "qcs_consec_fit_dev": (_i, [_vp, _vp, _i, _i, _vp, _i, _P(_i), _vp, _vp]),
     "qcs_consec_held": (_i, [_vp]),
@@ -547,6 +549,20 @@ class Context:
     def render_dev(self, d_loc, n, d_img, d_zimg=None, stream=None):
         check(lib().qcs_render_dev(self.h, self._dp(d_loc), n, self._dp(d_img), self._dp(d_zimg), stream),
               "qcs_render_dev")
+
+    def drift_correct_dev(self, d_rows, n, corr_frames, shift_cap=None, stream=None):
+        """Drift correction of n frame-sorted device rows in place.  Returns (frame count, per-frame x
+        shifts, y shifts, group shifts as an (groups, 2) array)."""
+        cap = shift_cap if shift_cap is not None else int(2 ** 20)
+        fx = np.zeros(cap, np.float32)
+        fy = np.zeros(cap, np.float32)
+        gxy = np.full((512, 2), np.nan, np.float32)
+        total = C.c_int()
+        check(lib().qcs_drift_correct_dev(self.h, self._dp(d_rows), n, corr_frames, _ptr(fx), _ptr(fy), cap,
+                                          C.byref(total), _ptr(gxy), stream), "qcs_drift_correct_dev")
+        t = total.value
+        g = int(np.count_nonzero(~np.isnan(gxy[:, 0])))
+        return t, fx[:t].copy(), fy[:t].copy(), gxy[:g].copy()
 
     def loc_batch_sync(self, frames, start_frame=1, cap=None):
         """lm_LocBatchedImg: extract + fit + frame-sorted compaction of a group of frames (host arrays)."""

@@ -71,6 +71,13 @@ PROTOS = {
     "orc_consec_destroy": (None, [_vp]),
     "orc_consec_held": (_i, [_vp]),
     "orc_consec_fit": (_i, [_PP, _vp, _vp, _i, _i, _vp, _i, _vp]),
+    "orc_drift_total_frame": (_i, [_vp, C.c_longlong]),
+    "orc_drift_groups": (_i, [_i, _i, _vp, _vp]),
+    "orc_drift_render": (None, [_PP, _vp, C.c_longlong, _i, _i, _vp]),
+    "orc_drift_cc": (C.c_double, [_vp, _vp, _i, _i, _i, _i]),
+    "orc_drift_peak": (None, [_vp, _i, _i, _P(_f), _P(_f)]),
+    "orc_drift_interp": (None, [_i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "orc_drift_correct": (_i, [_PP, _vp, C.c_longlong, _i, _vp, _vp, _i, _P(_i), _vp, _vp]),
 }
 
 _lib = None
@@ -267,3 +274,54 @@ class Consec:
 
     def __del__(self):
         self.close()
+
+
+def drift_groups(total, corr):
+    gs = np.zeros(512, np.int32)
+    ge = np.zeros(512, np.int32)
+    g = lib().orc_drift_groups(total, corr, p(gs), p(ge))
+    return gs[:max(g, 0)].copy(), ge[:max(g, 0)].copy()
+
+
+def drift_render(params, rows, srw, srh):
+    rows = np.ascontiguousarray(rows, np.float32)
+    img = np.zeros((srh, srw), np.float32)
+    lib().orc_drift_render(C.byref(params), p(rows), rows.shape[0], srw, srh, p(img))
+    return img
+
+
+def drift_cc(img1, img2, sx, sy):
+    img1 = np.ascontiguousarray(img1, np.float32)
+    img2 = np.ascontiguousarray(img2, np.float32)
+    return lib().orc_drift_cc(p(img1), p(img2), img1.shape[1], img1.shape[0], sx, sy)
+
+
+def drift_peak(cc47, bias_x, bias_y):
+    cc = np.ascontiguousarray(cc47, np.float64)
+    sx, sy = C.c_float(), C.c_float()
+    lib().orc_drift_peak(p(cc), bias_x, bias_y, C.byref(sx), C.byref(sy))
+    return sx.value, sy.value
+
+
+def drift_interp(total, gs, ge, gx, gy):
+    gs, ge = np.ascontiguousarray(gs, np.int32), np.ascontiguousarray(ge, np.int32)
+    gx, gy = np.ascontiguousarray(gx, np.float32), np.ascontiguousarray(gy, np.float32)
+    fx = np.zeros(total, np.float32)
+    fy = np.zeros(total, np.float32)
+    lib().orc_drift_interp(total, gs.shape[0], p(gs), p(ge), p(gx), p(gy), p(fx), p(fy))
+    return fx, fy
+
+
+def drift_correct(params, rows, corr):
+    """Returns (corrected rows, per-frame x shifts, y shifts, frame count, group x shifts, y shifts)."""
+    rows = np.ascontiguousarray(rows, np.float32).copy()
+    cap = int(rows[:, 11].max()) + 1 if rows.shape[0] else 1
+    fx = np.zeros(cap, np.float32)
+    fy = np.zeros(cap, np.float32)
+    gx = np.zeros(512, np.float32)
+    gy = np.zeros(512, np.float32)
+    total = C.c_int()
+    g = lib().orc_drift_correct(C.byref(params), p(rows), rows.shape[0], corr, p(fx), p(fy), cap, C.byref(total), p(gx), p(gy))
+    assert g > 0, g
+    t = total.value
+    return rows, fx[:t].copy(), fy[:t].copy(), t, gx[:g].copy(), gy[:g].copy()
