@@ -228,6 +228,21 @@ int qcs_compact_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_out, int *
  * (ImageRender_2d.cu:55-155, ImageRender_3d.cu:60-147). */
 int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream);
 
+/* The colour-coded 3D image (ImgRender_GetSaveImg_3D / RenderSaveImg_3D,
+ * ImgRender/ImageRender_3d.cu:238-403, 424-590).  d_img / d_zimg: the SRImageWidth x SRImageHigh
+ * intensity and z planes of qcs_render_dev.  The brightness scale is the intensity below which 97%
+ * of the pixels with intensity in (0, 12000) lie (2048-bin histogram) times bright_ratio
+ * (*max_intensity, may be NULL); a pixel's value is min(intensity * 120 / scale, 100), its hue
+ * (z plane / intensity) mapped over [MinZDepth, MaxZDepth] to 240..0 (ColorMode_3D 0, blue to
+ * red) or 0..240 (1), saturation 100; zero-intensity pixels are black.  d_out: 3 (BGR, RGB) or 4
+ * (BGRA, RGBA; alpha 0xff) bytes per pixel. */
+#define QCS_RGB_BGR  0   /* RGBImage_EncodeMode_3B_BGR */
+#define QCS_RGB_RGB  1
+#define QCS_RGB_BGRA 2   /* the ImageJ plug-in's 3D display image */
+#define QCS_RGB_RGBA 3
+int qcs_render_rgb_dev(qcs_ctx *ctx, const float *d_img, const float *d_zimg, int mode, float bright_ratio, uint8_t *d_out,
+                       float *max_intensity, void *stream);
+
 /* ========================================================================================
  * Converged multi-emitter fitting (no counterpart in the reference, whose fitters run a fixed
  * iteration schedule): the 2D model mu = sum_k N_k/(2 pi s^2) exp(-d_k^2/(2 s^2)) + b with K = 1..3
@@ -488,6 +503,10 @@ int qcs_session_is_finished(qcs_session *s);
 long long qcs_session_fetch_locs(qcs_session *s, float *out12, long long cap);
 /* lm_GetSMLMImage: copies the SRImageWidth*SRImageHigh float image. */
 int qcs_session_get_sr_image(qcs_session *s, float *img);
+/* lm_GetSMLMImage3D (IJ/QC_STORM_CPPDLL.cpp:283-309; the image of OnlineRendDisp.cpp:35-43): the
+ * colour-coded 3D image of the accumulated rows, 4 bytes BGRA per SR pixel (qcs_render_rgb_dev,
+ * bright ratio 1).  AS3D sessions only. */
+int qcs_session_get_sr_image3d(qcs_session *s, uint8_t *bgra);
 /* lm_GetMaxDispVal: max of (int) SR image value. */
 int qcs_session_get_max_disp_val(qcs_session *s, int *val);
 /* lm_ReleaseResource. */

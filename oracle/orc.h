@@ -132,6 +132,11 @@ int   orc_consec_held(const orc_consec *c);
 int   orc_consec_fit(const qcs_params *p, orc_consec *c, const float *rows, int n, int is_end, float *out, int out_cap,
                      float *ontime_ratio);
 
+/* colour-coded 3D image (orc_display.c) */
+float  orc_display_max_intensity(const float *img, long long npx, const int *hist_in, int *hist_out);
+void   orc_display_rgb(const qcs_params *p, const float *img, const float *zimg, long long npx, float max_intensity, int mode,
+                       uint8_t *out);
+
 /* drift correction by SR-image cross-correlation (orc_drift.c) */
 int    orc_drift_total_frame(const float *loc, long long n);
 int    orc_drift_groups(int total, int corr, int *gs, int *ge);

@@ -227,7 +227,7 @@ int qcs_ctx_destroy(qcs_ctx *c)
                            &c->ex_frames, &c->fit_idx1, &c->fit_idx2, &c->fit_idx3, &c->fit_flag, &c->fit_cnt,
                            &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev, &c->cf_theta, &c->cf_ll,
                            &c->cf_list, &c->cf_rows, &c->cf_queue, &c->rn_buf, &c->st_buf,
-                           &c->dr_img, &c->dr_cc, &c->dr_shift, &c->dr_frames};
+                           &c->dr_img, &c->dr_cc, &c->dr_shift, &c->dr_frames, &c->dp_hist};
     for (auto *b : bufs) b->release();
     c->pin_cnt.release();
     (void)timer_resolve(c->timer);

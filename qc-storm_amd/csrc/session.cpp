@@ -403,6 +403,27 @@ int qcs_session_get_sr_image(qcs_session *s, float *img)
     return QCS_OK;
 }
 
+int qcs_session_get_sr_image3d(qcs_session *s, uint8_t *bgra)
+{
+    if (!s || !bgra) return fail(QCS_ERR_ARG, "invalid argument");
+    if (s->p.LocType != QCS_LOCTYPE_AS3D || !s->zimg.ptr) return fail(QCS_ERR_STATE, "the 3D image needs an AS3D session");
+    int rc = session_status(s);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> dg(s->dev_mu);
+    QCS_HIP(hipSetDevice(s->device));
+    const size_t npx = (size_t)s->p.SRImageWidth * s->p.SRImageHigh;
+    DevBuf d;
+    if ((rc = d.ensure(npx * 4))) return rc;
+    rc = qcs_render_rgb_dev(s->ctx, s->img.as<float>(), s->zimg.as<float>(), QCS_RGB_BGRA, 1.0f, d.as<uint8_t>(), nullptr, s->s);
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMemcpyAsync(bgra, d.ptr, npx * 4, hipMemcpyDeviceToHost, s->s);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(s->s);
+    d.release();
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "3D image", __FILE__, __LINE__);
+    return QCS_OK;
+}
+
 int qcs_session_get_max_disp_val(qcs_session *s, int *val)
 {
     if (!s || !val) return fail(QCS_ERR_ARG, "invalid argument");

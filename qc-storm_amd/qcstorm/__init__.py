@@ -129,6 +129,7 @@ class Graph:
 
 
 ONTIME_BINS = 10
+RGB_BGR, RGB_RGB, RGB_BGRA, RGB_RGBA = 0, 1, 2, 3
 
 
 class Consec:
@@ -246,6 +247,8 @@ PROTOTYPES = {
     "qcs_session_get_max_disp_val": (_i, [_vp, _P(_i)]),
     "qcs_session_destroy": (_i, [_vp]),
     "qcs_session_set_output": (_i, [_vp, C.c_char_p]),
+    "qcs_session_get_sr_image3d": (_i, [_vp, _vp]),
+    "qcs_render_rgb_dev": (_i, [_vp, _vp, _vp, _i, _f, _vp, _P(_f), _vp]),
     "qcs_session_get_stats": (_i, [_vp, _P(QcsLocStats), _vp]),
     "qcs_result_file_name": (_i, [_P(QcsParams), C.c_char_p, C.c_char_p, _i]),
     "qcs_loc_batch_sync": (_ll, [_vp, _vp, _i, _i, _vp, _ll]),
@@ -564,6 +567,13 @@ class Context:
         g = int(np.count_nonzero(~np.isnan(gxy[:, 0])))
         return t, fx[:t].copy(), fy[:t].copy(), gxy[:g].copy()
 
+    def render_rgb_dev(self, d_img, d_zimg, mode, d_out, bright_ratio=1.0, stream=None):
+        """Colour-coded 3D image of the intensity / z planes into d_out; returns the brightness scale."""
+        mx = C.c_float()
+        check(lib().qcs_render_rgb_dev(self.h, self._dp(d_img), self._dp(d_zimg), mode, bright_ratio, self._dp(d_out),
+                                       C.byref(mx), stream), "qcs_render_rgb_dev")
+        return mx.value
+
     def loc_batch_sync(self, frames, start_frame=1, cap=None):
         """lm_LocBatchedImg: extract + fit + frame-sorted compaction of a group of frames (host arrays)."""
         frames = np.ascontiguousarray(frames, np.uint16)
@@ -637,6 +647,12 @@ class Session:
         v = C.c_int()
         check(lib().qcs_session_get_max_disp_val(self.h, C.byref(v)), "qcs_session_get_max_disp_val")
         return v.value
+
+    def sr_image3d(self):
+        """lm_GetSMLMImage3D: the colour-coded 3D image, (SRImageHigh, SRImageWidth, 4) BGRA bytes."""
+        img = np.zeros((self.params.SRImageHigh, self.params.SRImageWidth, 4), np.uint8)
+        check(lib().qcs_session_get_sr_image3d(self.h, _ptr(img)), "qcs_session_get_sr_image3d")
+        return img
 
     def set_output(self, path):
         """Append every flush's final rows to `path` (raw 12-float records); before the first feed."""

@@ -71,6 +71,8 @@ PROTOS = {
     "orc_consec_destroy": (None, [_vp]),
     "orc_consec_held": (_i, [_vp]),
     "orc_consec_fit": (_i, [_PP, _vp, _vp, _i, _i, _vp, _i, _vp]),
+    "orc_display_max_intensity": (_f, [_vp, C.c_longlong, _vp, _vp]),
+    "orc_display_rgb": (None, [_PP, _vp, _vp, C.c_longlong, _f, _i, _vp]),
     "orc_drift_total_frame": (_i, [_vp, C.c_longlong]),
     "orc_drift_groups": (_i, [_i, _i, _vp, _vp]),
     "orc_drift_render": (None, [_PP, _vp, C.c_longlong, _i, _i, _vp]),
@@ -325,3 +327,22 @@ def drift_correct(params, rows, corr):
     assert g > 0, g
     t = total.value
     return rows, fx[:t].copy(), fy[:t].copy(), t, gx[:g].copy(), gy[:g].copy()
+
+
+def display_max_intensity(img=None, hist=None):
+    """(97% brightness level, 2048-bin histogram) of an SR image, or of given counts."""
+    out = np.zeros(2048, np.int32)
+    if hist is not None:
+        h = np.ascontiguousarray(hist, np.int32)
+        return lib().orc_display_max_intensity(None, 0, p(h), p(out)), out
+    img = np.ascontiguousarray(img, np.float32)
+    return lib().orc_display_max_intensity(p(img), img.size, None, p(out)), out
+
+
+def display_rgb(params, img, zimg, scale, mode):
+    img = np.ascontiguousarray(img, np.float32)
+    zimg = np.ascontiguousarray(zimg, np.float32)
+    nb = 3 if mode < 2 else 4
+    out = np.zeros(img.size * nb, np.uint8)
+    lib().orc_display_rgb(C.byref(params), p(img), p(zimg), img.size, scale, mode, p(out))
+    return out.reshape(img.shape + (nb,))
