@@ -42,3 +42,18 @@ def test_host_helpers_without_gpu():
     assert (p.SRImageWidth, p.SRImageHigh) == (5108, 2568)   # multiples of 4 (bfgs_base.cu:89-98)
     assert Q.lib().qcs_version().startswith(b"qcstorm")
     assert Q.roi_words(7) == 56
+
+
+def test_result_file_name_without_gpu():
+    """SetLocDataFileName (IJ/LocResource.cpp:160-179): CString::TrimRight(".tif") drops every
+    trailing '.', 't', 'i', 'f', then the suffix follows the mode."""
+    p = Q.default_params()
+    assert Q.result_file_name(p, "D:/data/cells.tif") == "D:/data/cells_result2D7_S.txt"
+    assert Q.result_file_name(p, "stif.tif") == "s_result2D7_S.txt"          # trailing set characters go too
+    assert Q.result_file_name(p, "movie") == "movie_result2D7_S.txt"
+    q = Q.default_params(LocType=Q.LOCTYPE_AS3D, ROISize=11, MultiEmitterFitEn=1, ConsecFitEn=1,
+                         ConsecFit_DistanceTh_nm=79.6)
+    assert Q.result_file_name(q, "a.tif") == "a_result3D11_M_Consec80nm.txt"
+    buf = C.create_string_buffer(8)
+    n = Q.lib().qcs_result_file_name(C.byref(p), b"cells.tif", buf, 8)
+    assert n == len("cells_result2D7_S.txt") and buf.value == b"cells_r"   # truncated, terminated
