@@ -560,8 +560,12 @@ __device__ __forceinline__ int mark_count(const unsigned long long *mark, int ns
     return cnt;
 }
 
-// one wave per ROI record (grid-stride over the kept count): lanes write consecutive uint16
-// words of the record, so the stores are contiguous
+// one wave per group of CUT_U consecutive ROI records (grid-stride over the kept count): lanes
+// write consecutive uint16 words of each record, so the stores are contiguous; the CUT_U records'
+// list entries and pixels are loaded before any is stored, so a wave keeps CUT_U independent
+// memory round trips in flight instead of one
+constexpr int CUT_U = 8;
+
 __global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
                                                  const unsigned long long *__restrict__ mark_mask,
                                                  const int *__restrict__ info, const int3 *__restrict__ xyb,
@@ -570,40 +574,53 @@ __global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ fr
     const int n = info[0];
     const int R = c.R, half = R / 2, WR = R * (R + 1), RR = R * R;
     const int l = threadIdx.x;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        const int3 e = xyb[i];
-        const int x = e.x, y = e.y, b = e.z;
-        const size_t rbase = (size_t)b * c.B * c.H;   // stacked row of the batch's first row
-        int ft = QCS_ROITYPE_SINGLE;
-        if (c.multi) {
-            // lanes 0..R-1 count the marks of one window row each
-            int all = 0, inner = 0;
-            if (l < R) {
-                const size_t row = rbase + y - half + l;
-                all = mark_count(mark_mask, c.nseg, row, x - half, x + half);
-                if (l >= half - 1 && l <= half + 1) inner = mark_count(mark_mask, c.nseg, row, x - 1, x + 1);
-            }
-            int d = all - inner;
-            for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-            ft = (d > 0) ? QCS_ROITYPE_MULTI : QCS_ROITYPE_SINGLE;
+    for (int i0 = blockIdx.x * CUT_U; i0 < n; i0 += gridDim.x * CUT_U) {
+        int3 e[CUT_U];
+        int ft[CUT_U];
+#pragma unroll
+        for (int j = 0; j < CUT_U; j++) {
+            e[j] = (i0 + j < n) ? xyb[i0 + j] : make_int3(half, half, 0);
+            ft[j] = QCS_ROITYPE_SINGLE;
         }
-        const uint16_t *img = frames + rbase * c.W;
-        uint16_t *o = rois + (size_t)i * WR;
-        const int fr = c.start_frame + b * c.B + y / c.H;
-        for (int k = l; k < WR; k += 64) {
-            uint16_t v;
-            if (k < RR) {
-                const int r = k / R, cc = k - r * R;
-                v = img[(size_t)(y - half + r) * c.W + (x - half + cc)];
-            } else {
-                const int h = k - RR;
-                v = h == 0 ? (uint16_t)x
-                  : h == 1 ? (uint16_t)(y % c.H)
-                  : h == 2 ? (uint16_t)(fr % 65536)
-                  : h == 3 ? (uint16_t)(fr / 65536)
-                  : h == 4 ? (uint16_t)ft : (uint16_t)0;
+        if (c.multi) {
+#pragma unroll
+            for (int j = 0; j < CUT_U; j++) {
+                // lanes 0..R-1 count the marks of one window row each
+                const int x = e[j].x, y = e[j].y;
+                const size_t rbase = (size_t)e[j].z * c.B * c.H;
+                int all = 0, inner = 0;
+                if (l < R && i0 + j < n) {
+                    const size_t row = rbase + y - half + l;
+                    all = mark_count(mark_mask, c.nseg, row, x - half, x + half);
+                    if (l >= half - 1 && l <= half + 1) inner = mark_count(mark_mask, c.nseg, row, x - 1, x + 1);
+                }
+                int d = all - inner;
+                for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+                ft[j] = (d > 0) ? QCS_ROITYPE_MULTI : QCS_ROITYPE_SINGLE;
             }
-            o[k] = v;
+        }
+        for (int k = l; k < WR; k += 64) {
+            uint16_t v[CUT_U];
+#pragma unroll
+            for (int j = 0; j < CUT_U; j++) {
+                const int x = e[j].x, y = e[j].y, b = e[j].z;
+                if (k < RR) {
+                    const int r = k / R, cc = k - r * R;
+                    const uint16_t *img = frames + (size_t)b * c.B * c.H * c.W;
+                    v[j] = (i0 + j < n) ? img[(size_t)(y - half + r) * c.W + (x - half + cc)] : (uint16_t)0;
+                } else {
+                    const int h = k - RR;
+                    const int fr = c.start_frame + b * c.B + y / c.H;
+                    v[j] = h == 0 ? (uint16_t)x
+                         : h == 1 ? (uint16_t)(y % c.H)
+                         : h == 2 ? (uint16_t)(fr % 65536)
+                         : h == 3 ? (uint16_t)(fr / 65536)
+                         : h == 4 ? (uint16_t)ft[j] : (uint16_t)0;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < CUT_U; j++)
+                if (i0 + j < n) rois[(size_t)(i0 + j) * WR + k] = v[j];
         }
     }
 }
