@@ -245,20 +245,25 @@ int qcs_render_rgb_dev(qcs_ctx *ctx, const float *d_img, const float *d_zimg, in
 
 /* ========================================================================================
  * Converged multi-emitter fitting (no counterpart in the reference, whose fitters run a fixed
- * iteration schedule): the 2D model mu = sum_k N_k/(2 pi s^2) exp(-d_k^2/(2 s^2)) + b with K = 1..3
- * emitters sharing s, Poisson MLE by BFGS with analytic gradient run to convergence (step below
- * 1e-4 in scaled units, at most 100 iterations).  ROISize 7, 9 or 11, LocType 0.
+ * iteration schedule), K = 1..3 emitters, Poisson MLE by BFGS with analytic gradient run to
+ * convergence (step below 1e-4 in scaled units, at most 100 iterations), ROISize 7, 9 or 11:
+ *   LocType 0 (2D):      mu = sum_k N_k/(2 pi s^2) exp(-d_k^2/(2 s^2)) + b, emitters sharing s,
+ *                        theta = [N_k, x_k, y_k]_k, s, b (P = 3K + 2);
+ *   LocType 1 (AS3D):    mu = sum_k N_k/(2 pi sx_k sy_k) exp(-(dx^2/(2 sx_k^2) + dy^2/(2 sy_k^2))) + b,
+ *                        theta = [N_k, x_k, y_k, sx_k, sy_k]_k, b (P = 5K + 1, 16 at K = 3); rows
+ *                        carry SigX/SigY = sx/sy and z from the p*_XGY / p*_XLY polynomial of
+ *                        sx^2 - sy^2 times ZDepthCorrFactor, as the reference-schedule fitter does.
  * ====================================================================================== */
 #define QCS_CFIT_INIT_HEUR  0   /* centroid / edge-background heuristic */
-#define QCS_CFIT_INIT_GIVEN 1   /* d_init holds n x (3K+2) starting vectors */
+#define QCS_CFIT_INIT_GIVEN 1   /* d_init holds n x P starting vectors */
 #define QCS_CFIT_INIT_ADD   2   /* d_init holds the (K-1)-emitter fits; add one at the largest residual */
 #define QCS_CFIT_CONVERGED  1   /* info[1] flags */
 #define QCS_CFIT_MAXITER    2
 #define QCS_CFIT_STALLED    4
 #define QCS_CFIT_FLAT       8   /* converged by ftol (set together with QCS_CFIT_CONVERGED) */
-/* Fit model K to every ROI.  theta (n x (3K+2)): [N_k photons, x_k, y_k px (pixel edges)]_k, s px,
- * b photons/px.  info (n x 4, may be NULL): iterations, flags, loss evaluations, 0.  ll (n, may be
- * NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
+/* Fit model K to every ROI.  theta (n x P): N_k photons, x_k, y_k px (pixel edges), widths px,
+ * b photons/px in the order above.  info (n x 4, may be NULL): iterations, flags, loss
+ * evaluations, 0.  ll (n, may be NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
 int qcs_cfit_dev(qcs_ctx *ctx, int K, const uint16_t *d_rois, int n, int init_mode, const float *d_init,
                  float *d_theta, int *d_info, double *d_ll, void *stream);
 /* Tuning of the converged fits and of the warm start (per context). */
@@ -274,6 +279,9 @@ typedef struct qcs_tuning {
     int   active_set;       /* 1: parameters held at a box bound with an outward gradient are left
                              * out of the quasi-Newton step (default 0) */
 } qcs_tuning;
+/* ftol, active_set and max_iter apply as set to the 2D model.  Fits of the elliptical model
+ * (LocType AS3D: theta = [N, x, y, sx, sy]_k, b) always run with the active set, without the
+ * ftol stop and with at least 100 iterations. */
 int qcs_tuning_get(qcs_ctx *ctx, qcs_tuning *out);
 int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t);
 /* The plain multi-emitter path: the emitter count follows the FP64 log-likelihood-ratio rule

@@ -1,12 +1,16 @@
 // cfit_dev.h -- device building blocks of the converged multi-emitter fitter (gfx950), shared by
 // the one-pass kernels (cfit.hip) and the persistent work-queue kernels (cfq.hip).
 //
-// Model, pixel (r, c) centred at (c + 0.5, r + 0.5):
+// Models, pixel (r, c) centred at (c + 0.5, r + 0.5), K = 1..3 emitters:
 //
-//   mu = sum_k N_k / (2 pi s^2) exp(-((c + .5 - x_k)^2 + (r + .5 - y_k)^2) / (2 s^2)) + b
+//   2D (KK = K):          mu = sum_k N_k / (2 pi s^2) exp(-((c + .5 - x_k)^2 + (r + .5 - y_k)^2) / (2 s^2)) + b
+//                         theta = [N_k, x_k, y_k]_k, s, b              (3K + 2 parameters)
+//   elliptical (KK = K + 10, astigmatic 3D):
+//                         mu = sum_k N_k / (2 pi sx_k sy_k) exp(-(dx^2 / (2 sx_k^2) + dy^2 / (2 sy_k^2))) + b
+//                         theta = [N_k, x_k, y_k, sx_k, sy_k]_k, b     (5K + 1 parameters, 16 at K = 3)
 //
-// with K = 1..3 emitters sharing the PSF width s; theta = [N_k, x_k, y_k]_k, s, b.  Per-lane
-// arrays in LDS are lane-interleaved (element i of lane l at [i * 64 + l]).
+// The template parameter KK selects model and emitter count.  Per-lane arrays in LDS are
+// lane-interleaved (element i of lane l at [i * 64 + l]).
 #pragma once
 #include <math.h>
 #include "qcs_internal.h"
@@ -24,13 +28,32 @@ struct CFitConst {
 };
 enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
 enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4, CF_FLAT = 8 };
+constexpr int CF_ELLIPTIC = 10;   // KK = K + CF_ELLIPTIC selects the elliptical model
 
-template <int K> struct CP { static constexpr int P = 3 * K + 2, NS = P * (P + 1) / 2; };
+// model traits: parameter count and indices
+template <int KK> struct CM {
+    static constexpr bool AS = KK > CF_ELLIPTIC;
+    static constexpr int K = AS ? KK - CF_ELLIPTIC : KK;
+    static constexpr int E = AS ? 5 : 3;                  // parameters per emitter
+    static constexpr int P = AS ? 5 * K + 1 : 3 * K + 2;
+    static constexpr int NS = P * (P + 1) / 2;
+    static constexpr int IB = P - 1;                      // background
+    __host__ __device__ static constexpr int iN(int k) { return E * k; }
+    __host__ __device__ static constexpr int ix(int k) { return E * k + 1; }
+    __host__ __device__ static constexpr int iy(int k) { return E * k + 2; }
+    __host__ __device__ static constexpr int isx(int k) { return AS ? E * k + 3 : 3 * K; }
+    __host__ __device__ static constexpr int isy(int k) { return AS ? E * k + 4 : 3 * K; }
+};
+template <int KK> struct CP { static constexpr int P = CM<KK>::P, NS = CM<KK>::NS; };
 
-__device__ __forceinline__ float cf_scale(int i, int P)
+// N -> kilo-photons, b -> hundreds of photons, positions and widths in pixels
+template <int KK>
+__device__ __forceinline__ float cf_scale(int i)
 {
-    // N -> kilo-photons, b -> hundreds of photons, positions and width in pixels
-    return (i == P - 1) ? 100.0f : ((i < P - 2 && i % 3 == 0) ? 1000.0f : 1.0f);
+    constexpr int P = CM<KK>::P;
+    if (i == P - 1) return 100.0f;
+    if constexpr (CM<KK>::AS) return (i % 5 == 0) ? 1000.0f : 1.0f;
+    else return (i < P - 2 && i % 3 == 0) ? 1000.0f : 1.0f;
 }
 
 // packed upper-triangular index of (i, j), i <= j
@@ -38,159 +61,269 @@ __device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P
 
 // loss sum((mu - q) - q ln(mu / q)) (the Poisson negative log-likelihood up to a constant) and,
 // if G, its gradient with respect to theta (physical units)
-template <int K, int R, bool G>
-__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<K>::P], float (&g)[CP<K>::P])
+template <int KK, int R, bool G>
+__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<KK>::P], float (&g)[CP<KK>::P])
 {
-    constexpr int P = CP<K>::P;
-    const float s = th[3 * K], b = th[3 * K + 1];
-    const float is2 = 1.0f / (s * s);
-    const float h = -0.5f * is2;
-    const float nrm = 0.15915494309189535f * is2;   // 1 / (2 pi s^2)
-    float ex[K][R], dx[K][R];
+    using M = CM<KK>;
+    constexpr int K = M::K;
+    if constexpr (!M::AS) {
+        const float s = th[3 * K], b = th[3 * K + 1];
+        const float is2 = 1.0f / (s * s);
+        const float h = -0.5f * is2;
+        const float nrm = 0.15915494309189535f * is2;   // 1 / (2 pi s^2)
+        float ex[K][R], dx[K][R];
 #pragma unroll
-    for (int k = 0; k < K; k++)
+        for (int k = 0; k < K; k++)
 #pragma unroll
-        for (int c = 0; c < R; c++) {
-            dx[k][c] = ((float)c + 0.5f) - th[3 * k + 1];
-            ex[k][c] = __expf(h * dx[k][c] * dx[k][c]);
-        }
-    float L = 0.0f;
-    float gN[K], gx[K], gy[K], gd2 = 0.0f, gw = 0.0f, gb = 0.0f;
+            for (int c = 0; c < R; c++) {
+                dx[k][c] = ((float)c + 0.5f) - th[3 * k + 1];
+                ex[k][c] = __expf(h * dx[k][c] * dx[k][c]);
+            }
+        float L = 0.0f;
+        float gN[K], gx[K], gy[K], gd2 = 0.0f, gw = 0.0f, gb = 0.0f;
 #pragma unroll
-    for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
-    // rows stay a loop: a full unroll lets the scheduler hoist every row's LDS loads and
-    // exponentials, which costs registers (occupancy) for no gain
+        for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
+        // rows stay a loop: a full unroll lets the scheduler hoist every row's LDS loads and
+        // exponentials, which costs registers (occupancy) for no gain
 #pragma unroll 1
-    for (int r = 0; r < R; r++) {
-        float ay[K], dy[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
-            ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
-        }
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            float gk[K];
-            float mu = b;
+        for (int r = 0; r < R; r++) {
+            float ay[K], dy[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                gk[k] = ay[k] * ex[k][c];
-                mu += gk[k];
+                dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
+                ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
             }
-            mu = fmaxf(mu, 1e-6f);
-            const float q = qs[(r * R + c) * 64];
-            // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
-            L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
-            if (G) {
-                const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
-                gb += w;
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                float gk[K];
+                float mu = b;
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const float wg = w * gk[k];
-                    gN[k] += wg;
-                    gx[k] = fmaf(wg, dx[k][c], gx[k]);
-                    gy[k] = fmaf(wg, dy[k], gy[k]);
-                    gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
-                    gw += wg;
+                    gk[k] = ay[k] * ex[k][c];
+                    mu += gk[k];
+                }
+                mu = fmaxf(mu, 1e-6f);
+                const float q = qs[(r * R + c) * 64];
+                // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
+                L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
+                if (G) {
+                    const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
+                    gb += w;
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const float wg = w * gk[k];
+                        gN[k] += wg;
+                        gx[k] = fmaf(wg, dx[k][c], gx[k]);
+                        gy[k] = fmaf(wg, dy[k], gy[k]);
+                        gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
+                        gw += wg;
+                    }
                 }
             }
         }
-    }
-    if (G) {
+        if (G) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                g[3 * k] = gN[k] / fmaxf(th[3 * k], 1e-3f);
+                g[3 * k + 1] = gx[k] * is2;
+                g[3 * k + 2] = gy[k] * is2;
+            }
+            g[3 * K] = (gd2 * is2 - 2.0f * gw) / s;
+            g[3 * K + 1] = gb;
+        }
+        return L;
+    } else {
+        const float b = th[M::IB];
+        float ex[K][R], dx[K][R], isx2[K], isy2[K], hy[K], nk[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            g[3 * k] = gN[k] / fmaxf(th[3 * k], 1e-3f);
-            g[3 * k + 1] = gx[k] * is2;
-            g[3 * k + 2] = gy[k] * is2;
+            const float sx = th[M::isx(k)], sy = th[M::isy(k)];
+            isx2[k] = 1.0f / (sx * sx);
+            isy2[k] = 1.0f / (sy * sy);
+            hy[k] = -0.5f * isy2[k];
+            nk[k] = th[M::iN(k)] * (0.15915494309189535f / (sx * sy));
+            const float hx = -0.5f * isx2[k];
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                dx[k][c] = ((float)c + 0.5f) - th[M::ix(k)];
+                ex[k][c] = __expf(hx * dx[k][c] * dx[k][c]);
+            }
         }
-        g[3 * K] = (gd2 * is2 - 2.0f * gw) / s;
-        g[3 * K + 1] = gb;
-        (void)P;
+        float L = 0.0f;
+        float gN[K], gx[K], gy[K], gxx[K], gyy[K], gb = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = gxx[k] = gyy[k] = 0.0f;
+#pragma unroll 1
+        for (int r = 0; r < R; r++) {
+            float ay[K], dy[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                dy[k] = ((float)r + 0.5f) - th[M::iy(k)];
+                ay[k] = nk[k] * __expf(hy[k] * dy[k] * dy[k]);
+            }
+#pragma unroll
+            for (int c = 0; c < R; c++) {
+                float gk[K];
+                float mu = b;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    gk[k] = ay[k] * ex[k][c];
+                    mu += gk[k];
+                }
+                mu = fmaxf(mu, 1e-6f);
+                const float q = qs[(r * R + c) * 64];
+                L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
+                if (G) {
+                    const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
+                    gb += w;
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const float wg = w * gk[k];
+                        gN[k] += wg;
+                        gx[k] = fmaf(wg, dx[k][c], gx[k]);
+                        gy[k] = fmaf(wg, dy[k], gy[k]);
+                        gxx[k] = fmaf(wg, dx[k][c] * dx[k][c], gxx[k]);
+                        gyy[k] = fmaf(wg, dy[k] * dy[k], gyy[k]);
+                    }
+                }
+            }
+        }
+        if (G) {
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                g[M::iN(k)] = gN[k] / fmaxf(th[M::iN(k)], 1e-3f);
+                g[M::ix(k)] = gx[k] * isx2[k];
+                g[M::iy(k)] = gy[k] * isy2[k];
+                g[M::isx(k)] = fmaf(gxx[k], isx2[k], -gN[k]) / th[M::isx(k)];
+                g[M::isy(k)] = fmaf(gyy[k], isy2[k], -gN[k]) / th[M::isy(k)];
+            }
+            g[M::IB] = gb;
+        }
+        return L;
     }
-    return L;
 }
 
-template <int K, int R>
-__device__ __forceinline__ void cf_project(float (&th)[CP<K>::P])
+template <int KK, int R>
+__device__ __forceinline__ void cf_project(float (&th)[CP<KK>::P])
 {
+    using M = CM<KK>;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        th[3 * k] = fmaxf(th[3 * k], 1.0f);
-        th[3 * k + 1] = fminf(fmaxf(th[3 * k + 1], -0.5f), (float)R + 0.5f);
-        th[3 * k + 2] = fminf(fmaxf(th[3 * k + 2], -0.5f), (float)R + 0.5f);
+    for (int k = 0; k < M::K; k++) {
+        th[M::iN(k)] = fmaxf(th[M::iN(k)], 1.0f);
+        th[M::ix(k)] = fminf(fmaxf(th[M::ix(k)], -0.5f), (float)R + 0.5f);
+        th[M::iy(k)] = fminf(fmaxf(th[M::iy(k)], -0.5f), (float)R + 0.5f);
+        if constexpr (M::AS) {
+            th[M::isx(k)] = fminf(fmaxf(th[M::isx(k)], 0.4f), 0.5f * (float)R);
+            th[M::isy(k)] = fminf(fmaxf(th[M::isy(k)], 0.4f), 0.5f * (float)R);
+        }
     }
-    th[3 * K] = fminf(fmaxf(th[3 * K], 0.4f), 0.5f * (float)R);
-    th[3 * K + 1] = fmaxf(th[3 * K + 1], 0.01f);
+    if constexpr (!M::AS) th[3 * M::K] = fminf(fmaxf(th[3 * M::K], 0.4f), 0.5f * (float)R);
+    th[M::IB] = fmaxf(th[M::IB], 0.01f);
+}
+
+// d mu / d theta (scaled units) of one pixel; returns mu
+template <int KK>
+__device__ __forceinline__ float cf_jacobian(const float (&th)[CP<KK>::P], float cpx, float rpx, float (&J)[CP<KK>::P])
+{
+    using M = CM<KK>;
+    constexpr int K = M::K, P = M::P;
+    float mu = th[M::IB];
+    if constexpr (!M::AS) {
+        const float s = th[3 * K], is2 = 1.0f / (s * s), h = -0.5f * is2, nrm = 0.15915494309189535f * is2;
+        float jw = 0.0f;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const float dx = cpx - th[3 * k + 1], dy = rpx - th[3 * k + 2];
+            const float d2 = fmaf(dx, dx, dy * dy);
+            const float mk = th[3 * k] * nrm * __expf(h * d2);
+            mu += mk;
+            J[3 * k] = mk / fmaxf(th[3 * k], 1e-3f) * cf_scale<KK>(3 * k);
+            J[3 * k + 1] = mk * dx * is2;
+            J[3 * k + 2] = mk * dy * is2;
+            jw = fmaf(mk, fmaf(d2, is2, -2.0f), jw);
+        }
+        J[3 * K] = jw / s;
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const float sx = th[M::isx(k)], sy = th[M::isy(k)];
+            const float isx2 = 1.0f / (sx * sx), isy2 = 1.0f / (sy * sy);
+            const float dx = cpx - th[M::ix(k)], dy = rpx - th[M::iy(k)];
+            const float mk = th[M::iN(k)] * (0.15915494309189535f / (sx * sy)) *
+                             __expf(fmaf(-0.5f * isx2, dx * dx, -0.5f * isy2 * (dy * dy)));
+            mu += mk;
+            J[M::iN(k)] = mk / fmaxf(th[M::iN(k)], 1e-3f) * cf_scale<KK>(M::iN(k));
+            J[M::ix(k)] = mk * dx * isx2;
+            J[M::iy(k)] = mk * dy * isy2;
+            J[M::isx(k)] = mk * fmaf(dx * dx, isx2, -1.0f) / sx;
+            J[M::isy(k)] = mk * fmaf(dy * dy, isy2, -1.0f) / sy;
+        }
+    }
+    J[P - 1] = cf_scale<KK>(P - 1);
+    return mu;
 }
 
 // Inverse of the Fisher information sum grad(mu) grad(mu)^T / mu at theta, in scaled units,
 // into the packed symmetric D (the BFGS starting inverse Hessian).  Returns false when the
-// Cholesky factorisation fails (D untouched).
-template <int K, int R>
-__device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<K>::P], float *D)
+// Cholesky factorisation fails (D then holds no usable matrix).  Up to 66 packed entries (the
+// 2D models) the factorisation runs in registers, larger ones (the elliptical models with two or
+// three emitters) in the lane's LDS column D.
+template <int KK, int R>
+__device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<KK>::P], float *D)
 {
-    constexpr int P = CP<K>::P, NS = CP<K>::NS;
-    float F[NS];
+    constexpr int P = CP<KK>::P, NS = CP<KK>::NS;
+    constexpr bool IN_LDS = NS > 66;
+    float Freg[IN_LDS ? 1 : NS];
+    auto F = [&](int i) -> float & {
+        if constexpr (IN_LDS) return D[i * 64];
+        else return Freg[i];
+    };
 #pragma unroll
-    for (int i = 0; i < NS; i++) F[i] = 0.0f;
-    const float s = th[3 * K], is2 = 1.0f / (s * s), h = -0.5f * is2, nrm = 0.15915494309189535f * is2;
+    for (int i = 0; i < NS; i++) F(i) = 0.0f;
 #pragma unroll 1
     for (int r = 0; r < R; r++) {
 #pragma unroll
         for (int c = 0; c < R; c++) {
             float J[P];
-            float mu = th[3 * K + 1], jw = 0.0f;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const float dx = ((float)c + 0.5f) - th[3 * k + 1], dy = ((float)r + 0.5f) - th[3 * k + 2];
-                const float d2 = fmaf(dx, dx, dy * dy);
-                const float mk = th[3 * k] * nrm * __expf(h * d2);
-                mu += mk;
-                J[3 * k] = mk / fmaxf(th[3 * k], 1e-3f) * cf_scale(3 * k, P);
-                J[3 * k + 1] = mk * dx * is2;
-                J[3 * k + 2] = mk * dy * is2;
-                jw = fmaf(mk, fmaf(d2, is2, -2.0f), jw);
-            }
-            J[3 * K] = jw / s;
-            J[3 * K + 1] = cf_scale(3 * K + 1, P);
+            const float mu = cf_jacobian<KK>(th, (float)c + 0.5f, (float)r + 0.5f, J);
             const float im = __builtin_amdgcn_rcpf(fmaxf(mu, 1e-6f));
 #pragma unroll
             for (int a = 0; a < P; a++) {
                 const float ja = J[a] * im;
 #pragma unroll
-                for (int b2 = a; b2 < P; b2++) F[tri(a, b2, P)] = fmaf(ja, J[b2], F[tri(a, b2, P)]);
+                for (int b2 = a; b2 < P; b2++) F(tri(a, b2, P)) = fmaf(ja, J[b2], F(tri(a, b2, P)));
             }
         }
     }
     // F = U^T U (U upper, in place)
 #pragma unroll
     for (int j = 0; j < P; j++) {
-        float d = F[tri(j, j, P)];
+        float d = F(tri(j, j, P));
 #pragma unroll
-        for (int k = 0; k < j; k++) d = fmaf(-F[tri(k, j, P)], F[tri(k, j, P)], d);
+        for (int k = 0; k < j; k++) d = fmaf(-F(tri(k, j, P)), F(tri(k, j, P)), d);
         if (!(d > 0.0f)) return false;
         const float u = sqrtf(d), iu = 1.0f / u;
-        F[tri(j, j, P)] = u;
+        F(tri(j, j, P)) = u;
 #pragma unroll
         for (int i = j + 1; i < P; i++) {
-            float v = F[tri(j, i, P)];
+            float v = F(tri(j, i, P));
 #pragma unroll
-            for (int k = 0; k < j; k++) v = fmaf(-F[tri(k, j, P)], F[tri(k, i, P)], v);
-            F[tri(j, i, P)] = v * iu;
+            for (int k = 0; k < j; k++) v = fmaf(-F(tri(k, j, P)), F(tri(k, i, P)), v);
+            F(tri(j, i, P)) = v * iu;
         }
     }
     // U^-1 in place, rows bottom-up, each row right to left
 #pragma unroll
     for (int i = P - 1; i >= 0; i--) {
-        const float iu = 1.0f / F[tri(i, i, P)];
+        const float iu = 1.0f / F(tri(i, i, P));
 #pragma unroll
         for (int j = P - 1; j > i; j--) {
             float v = 0.0f;
 #pragma unroll
-            for (int k = i + 1; k <= j; k++) v = fmaf(F[tri(i, k, P)], F[tri(k, j, P)], v);
-            F[tri(i, j, P)] = -v * iu;
+            for (int k = i + 1; k <= j; k++) v = fmaf(F(tri(i, k, P)), F(tri(k, j, P)), v);
+            F(tri(i, j, P)) = -v * iu;
         }
-        F[tri(i, i, P)] = iu;
+        F(tri(i, i, P)) = iu;
     }
     // F^-1 = U^-1 U^-T, in place row by row (entries (i, j), j >= i, use row i and row j from k >= j)
 #pragma unroll
@@ -199,41 +332,97 @@ __device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<K>::P], f
         for (int j = i; j < P; j++) {
             float v = 0.0f;
 #pragma unroll
-            for (int k = j; k < P; k++) v = fmaf(F[tri(i, k, P)], F[tri(j, k, P)], v);
-            F[tri(i, j, P)] = v;
+            for (int k = j; k < P; k++) v = fmaf(F(tri(i, k, P)), F(tri(j, k, P)), v);
+            F(tri(i, j, P)) = v;
         }
+    if constexpr (!IN_LDS) {
 #pragma unroll
-    for (int i = 0; i < NS; i++) D[i * 64] = F[i];
+        for (int i = 0; i < NS; i++) D[i * 64] = Freg[i];
+    }
     return true;
 }
 
-// FP64 log-likelihood sum(q ln mu - mu) (without the ln q! term, common to every model)
-template <int K, int R>
-__device__ double cf_loglik64(const float *qs, const float (&th)[CP<K>::P])
+// mu of one pixel in double (the FP64 likelihood and CRLB)
+template <int KK>
+__device__ __forceinline__ double cf_mu64(const float (&th)[CP<KK>::P], double cpx, double rpx, double *J)
 {
-    const double s = th[3 * K], b = th[3 * K + 1];
-    const double h = -0.5 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
-    double ll = 0.0;
-    for (int r = 0; r < R; r++)
-        for (int c = 0; c < R; c++) {
-            double mu = b;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
-                mu += (double)th[3 * k] * nrm * exp(h * (dx * dx + dy * dy));
+    using M = CM<KK>;
+    constexpr int K = M::K, P = M::P;
+    double mu = th[M::IB];
+    if constexpr (!M::AS) {
+        const double s = th[3 * K], is2 = 1.0 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+        double jw = 0.0;
+        for (int k = 0; k < K; k++) {
+            const double dx = cpx - (double)th[3 * k + 1], dy = rpx - (double)th[3 * k + 2];
+            const double d2 = dx * dx + dy * dy;
+            const double mk = (double)th[3 * k] * nrm * exp(-0.5 * d2 * is2);
+            mu += mk;
+            if (J) {
+                J[3 * k] = mk / (double)th[3 * k];
+                J[3 * k + 1] = mk * dx * is2;
+                J[3 * k + 2] = mk * dy * is2;
+                jw += mk * (d2 * is2 - 2.0) / s;
             }
-            const double q = qs[(r * R + c) * 64];
-            ll += q * log(mu) - mu;
         }
+        if (J) J[3 * K] = jw;
+    } else {
+        for (int k = 0; k < K; k++) {
+            const double sx = th[M::isx(k)], sy = th[M::isy(k)];
+            const double isx2 = 1.0 / (sx * sx), isy2 = 1.0 / (sy * sy);
+            const double dx = cpx - (double)th[M::ix(k)], dy = rpx - (double)th[M::iy(k)];
+            const double mk = (double)th[M::iN(k)] / (6.283185307179586 * sx * sy) * exp(-0.5 * (dx * dx * isx2 + dy * dy * isy2));
+            mu += mk;
+            if (J) {
+                J[M::iN(k)] = mk / (double)th[M::iN(k)];
+                J[M::ix(k)] = mk * dx * isx2;
+                J[M::iy(k)] = mk * dy * isy2;
+                J[M::isx(k)] = mk * (dx * dx * isx2 - 1.0) / sx;
+                J[M::isy(k)] = mk * (dy * dy * isy2 - 1.0) / sy;
+            }
+        }
+    }
+    if (J) J[P - 1] = 1.0;
+    return mu;
+}
+
+// FP64 log-likelihood sum(q ln mu - mu) (without the ln q! term, common to every model)
+template <int KK, int R>
+__device__ double cf_loglik64(const float *qs, const float (&th)[CP<KK>::P])
+{
+    double ll = 0.0;
+    if constexpr (!CM<KK>::AS) {
+        constexpr int K = CM<KK>::K;
+        const double s = th[3 * K], b = th[3 * K + 1];
+        const double h = -0.5 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+        for (int r = 0; r < R; r++)
+            for (int c = 0; c < R; c++) {
+                double mu = b;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
+                    mu += (double)th[3 * k] * nrm * exp(h * (dx * dx + dy * dy));
+                }
+                const double q = qs[(r * R + c) * 64];
+                ll += q * log(mu) - mu;
+            }
+    } else {
+        for (int r = 0; r < R; r++)
+            for (int c = 0; c < R; c++) {
+                const double mu = cf_mu64<KK>(th, c + 0.5, r + 0.5, nullptr);
+                const double q = qs[(r * R + c) * 64];
+                ll += q * log(mu) - mu;
+            }
+    }
     return ll;
 }
 
 // initial vector: heuristic (K = 1), given, or the previous model plus an emitter at the largest
 // positive residual (K >= 2; init holds the K-1 model's theta)
-template <int K, int R>
-__device__ __forceinline__ void cf_init(const float *qs, const float *init, int mode, float sigma0, float (&th)[CP<K>::P])
+template <int KK, int R>
+__device__ __forceinline__ void cf_init(const float *qs, const float *init, int mode, float sigma0, float (&th)[CP<KK>::P])
 {
-    constexpr int P = CP<K>::P;
+    using M = CM<KK>;
+    constexpr int P = M::P, K = M::K;
     if (mode == CF_INIT_GIVEN) {
 #pragma unroll
         for (int i = 0; i < P; i++) th[i] = init[i];
@@ -241,36 +430,70 @@ __device__ __forceinline__ void cf_init(const float *qs, const float *init, int 
     }
     if constexpr (K > 1) {
       if (mode == CF_INIT_ADD) {
-        constexpr int P0 = CP<K - 1>::P;
+        using M0 = CM<KK - 1>;
+        constexpr int P0 = M0::P;
         float t0[P0];
 #pragma unroll
         for (int i = 0; i < P0; i++) t0[i] = init[i];
-        const float s = t0[3 * (K - 1)], b = t0[3 * (K - 1) + 1];
-        const float h = -0.5f / (s * s), nrm = 0.15915494309189535f / (s * s);
+        if constexpr (M::AS) {
+            // the previous emitters made round at their narrower width: a fit of fewer emitters
+            // than the ROI holds widens along the line to the missing one, and the residual of
+            // that wide model hides it
+#pragma unroll
+            for (int k = 0; k < K - 1; k++) {
+                const float w = fminf(t0[M0::isx(k)], t0[M0::isy(k)]);
+                t0[M0::isx(k)] = w;
+                t0[M0::isy(k)] = w;
+            }
+        }
+        const float b = t0[M0::IB];
         float best = -1e30f;
         int bi = (R / 2) * R + R / 2;
         for (int r = 0; r < R; r++)
             for (int c = 0; c < R; c++) {
-                float mu = b;
+                float J0[P0];
+                float mu;
+                if constexpr (!M::AS) {
+                    const float s = t0[3 * (K - 1)];
+                    const float h = -0.5f / (s * s), nrm = 0.15915494309189535f / (s * s);
+                    mu = b;
 #pragma unroll
-                for (int k = 0; k < K - 1; k++) {
-                    const float dx = ((float)c + 0.5f) - t0[3 * k + 1], dy = ((float)r + 0.5f) - t0[3 * k + 2];
-                    mu += t0[3 * k] * nrm * __expf(h * (dx * dx + dy * dy));
+                    for (int k = 0; k < K - 1; k++) {
+                        const float dx = ((float)c + 0.5f) - t0[3 * k + 1], dy = ((float)r + 0.5f) - t0[3 * k + 2];
+                        mu += t0[3 * k] * nrm * __expf(h * (dx * dx + dy * dy));
+                    }
+                    (void)J0;
+                } else {
+                    mu = cf_jacobian<KK - 1>(t0, (float)c + 0.5f, (float)r + 0.5f, J0);
                 }
                 const float res = qs[(r * R + c) * 64] - mu;
                 if (res > best) { best = res; bi = r * R + c; }
             }
 #pragma unroll
         for (int k = 0; k < K - 1; k++) {
-            th[3 * k] = t0[3 * k];
-            th[3 * k + 1] = t0[3 * k + 1];
-            th[3 * k + 2] = t0[3 * k + 2];
+#pragma unroll
+            for (int e = 0; e < M::E; e++) th[M::E * k + e] = t0[M0::E * k + e];
         }
-        th[3 * (K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * s * s;
-        th[3 * (K - 1) + 1] = (float)(bi % R) + 0.5f;
-        th[3 * (K - 1) + 2] = (float)(bi / R) + 0.5f;
-        th[3 * K] = s;
-        th[3 * K + 1] = b;
+        if constexpr (!M::AS) {
+            const float s = t0[3 * (K - 1)];
+            th[3 * (K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * s * s;
+            th[3 * (K - 1) + 1] = (float)(bi % R) + 0.5f;
+            th[3 * (K - 1) + 2] = (float)(bi / R) + 0.5f;
+            th[3 * K] = s;
+        } else {
+            // the new emitter takes the mean widths of the others
+            float msx = 0.0f, msy = 0.0f;
+#pragma unroll
+            for (int k = 0; k < K - 1; k++) { msx += t0[M0::isx(k)]; msy += t0[M0::isy(k)]; }
+            msx /= (float)(K - 1);
+            msy /= (float)(K - 1);
+            th[M::iN(K - 1)] = fmaxf(best, 5.0f) * 6.283185307179586f * msx * msy;
+            th[M::ix(K - 1)] = (float)(bi % R) + 0.5f;
+            th[M::iy(K - 1)] = (float)(bi / R) + 0.5f;
+            th[M::isx(K - 1)] = msx;
+            th[M::isy(K - 1)] = msy;
+        }
+        th[M::IB] = b;
         return;
       }
     }
@@ -295,40 +518,31 @@ __device__ __forceinline__ void cf_init(const float *qs, const float *init, int 
     const float cx = tot > 0.0f ? sx / tot : 0.5f * R, cy = tot > 0.0f ? sy / tot : 0.5f * R;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        th[3 * k] = fmaxf(tot, 50.0f) / (float)K;
-        th[3 * k + 1] = cx + 0.3f * (float)k;
-        th[3 * k + 2] = cy;
+        th[M::iN(k)] = fmaxf(tot, 50.0f) / (float)K;
+        th[M::ix(k)] = cx + 0.3f * (float)k;
+        th[M::iy(k)] = cy;
+        if constexpr (M::AS) {
+            th[M::isx(k)] = sigma0;
+            th[M::isy(k)] = sigma0;
+        }
     }
-    th[3 * K] = sigma0;
-    th[3 * K + 1] = fmaxf(b, 0.5f);
+    if constexpr (!M::AS) th[3 * K] = sigma0;
+    th[M::IB] = fmaxf(b, 0.5f);
 }
 
 // ---- FP64 full-Fisher CRLB of one model (standard deviations of x_k and y_k, px) ----------
 // I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed in LDS per lane; Cholesky; then
 // (I^-1)_ii = |L^-1 e_i|^2 for the position indices.
-template <int K, int R>
-__device__ void cf_crlb64(const float (&th)[CP<K>::P], double *I, float (&cx)[K], float (&cy)[K])
+template <int KK, int R>
+__device__ void cf_crlb64(const float (&th)[CP<KK>::P], double *I, float (&cx)[CM<KK>::K], float (&cy)[CM<KK>::K])
 {
-    constexpr int P = CP<K>::P;
-    for (int i = 0; i < CP<K>::NS; i++) I[i * 64] = 0.0;
-    const double s = th[3 * K], is2 = 1.0 / (s * s), nrm = 1.0 / (6.283185307179586 * s * s);
+    using M = CM<KK>;
+    constexpr int P = M::P, K = M::K;
+    for (int i = 0; i < CP<KK>::NS; i++) I[i * 64] = 0.0;
     for (int r = 0; r < R; r++)
         for (int c = 0; c < R; c++) {
             double J[P];
-            double mu = th[3 * K + 1];
-            double jw = 0.0;
-            for (int k = 0; k < K; k++) {
-                const double dx = (c + 0.5) - (double)th[3 * k + 1], dy = (r + 0.5) - (double)th[3 * k + 2];
-                const double d2 = dx * dx + dy * dy;
-                const double mk = (double)th[3 * k] * nrm * exp(-0.5 * d2 * is2);
-                mu += mk;
-                J[3 * k] = mk / (double)th[3 * k];
-                J[3 * k + 1] = mk * dx * is2;
-                J[3 * k + 2] = mk * dy * is2;
-                jw += mk * (d2 * is2 - 2.0) / s;
-            }
-            J[3 * K] = jw;
-            J[3 * K + 1] = 1.0;
+            const double mu = cf_mu64<KK>(th, c + 0.5, r + 0.5, J);
             const double im = 1.0 / mu;
             for (int a = 0; a < P; a++) {
                 const double ja = J[a] * im;
@@ -351,7 +565,7 @@ __device__ void cf_crlb64(const float (&th)[CP<K>::P], double *I, float (&cx)[K]
     }
     for (int k = 0; k < K; k++) {
         for (int w = 0; w < 2; w++) {
-            const int e = 3 * k + 1 + w;
+            const int e = w == 0 ? M::ix(k) : M::iy(k);
             double var = -1.0;
             if (ok) {
                 // (I^-1)_ee = |U^-T e_e|^2: forward substitution U^T z = e_e
@@ -375,39 +589,66 @@ struct CSelConst {
     float n_min;        // minimum photons of every emitter of an accepted model
     float pixel_size, qe;
     int kmax;
+    float pG[5], pL[5], zcorr;   // astigmatic z calibration (elliptical model)
 };
 
-template <int K, int R>
+template <int KK, int R>
 __device__ __forceinline__ bool cf_valid(const float *t, float n_min)
 {
-    for (int k = 0; k < K; k++) {
-        if (!(t[3 * k] >= n_min)) return false;
-        if (!(t[3 * k + 1] > 0.0f && t[3 * k + 1] < (float)R && t[3 * k + 2] > 0.0f && t[3 * k + 2] < (float)R)) return false;
+    using M = CM<KK>;
+    for (int k = 0; k < M::K; k++) {
+        if (!(t[M::iN(k)] >= n_min)) return false;
+        if (!(t[M::ix(k)] > 0.0f && t[M::ix(k)] < (float)R && t[M::iy(k)] > 0.0f && t[M::iy(k)] < (float)R)) return false;
+        if constexpr (M::AS)
+            if (!(t[M::isx(k)] > 0.45f && t[M::isx(k)] < 0.45f * (float)R && t[M::isy(k)] > 0.45f &&
+                  t[M::isy(k)] < 0.45f * (float)R))
+                return false;
     }
-    return t[3 * K] > 0.45f && t[3 * K] < 0.45f * (float)R;
+    if constexpr (M::AS) return true;
+    else return t[3 * M::K] > 0.45f && t[3 * M::K] < 0.45f * (float)R;
 }
 
-template <int K, int R>
+// z of an elliptical emitter (bfgsMLE_AS3D_core.h:266-279): d = sx^2 - sy^2, polynomial by sign
+__device__ __forceinline__ float cf_zpoly(const float (&pc)[5], float d, float corr)
+{
+    const float d2 = d * d, d3 = d2 * d, d4 = d2 * d2;
+    float z = pc[0] * d4;
+    z = fmaf(pc[1], d3, z);
+    z = fmaf(pc[2], d2, z);
+    z = fmaf(pc[3], d, z);
+    z = z + pc[4];
+    return z * corr;
+}
+
+template <int KK, int R>
 __device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, double *I, float *rows)
 {
-    float th[CP<K>::P];
-    for (int i = 0; i < CP<K>::P; i++) th[i] = t[i];
+    using M = CM<KK>;
+    constexpr int K = M::K;
+    float th[CP<KK>::P];
+    for (int i = 0; i < CP<KK>::P; i++) th[i] = t[i];
     float cx[K], cy[K];
-    cf_crlb64<K, R>(th, I, cx, cy);
-    const float s = th[3 * K], b = th[3 * K + 1];
+    cf_crlb64<KK, R>(th, I, cx, cy);
+    const float b = th[M::IB];
     const int half = R / 2;
     const float xc = (float)hdr[0], yc = (float)hdr[1];
     const int fr = (int)hdr[2] + (int)hdr[3] * 65536;
     for (int k = 0; k < K; k++) {
         float *o = rows + (size_t)k * QCS_REC_FLOATS;
-        const float peak = th[3 * k] * 0.15915494309189535f / (s * s);
+        const float sx = th[M::isx(k)], sy = th[M::isy(k)];
+        const float peak = th[M::iN(k)] * 0.15915494309189535f / (sx * sy);
         o[QCS_POS_PPHO] = peak;
-        o[QCS_POS_XPOS] = (th[3 * k + 1] - (float)half) + xc;
-        o[QCS_POS_YPOS] = (th[3 * k + 2] - (float)half) + yc;
-        o[QCS_POS_ZPOS] = 0.0f;
-        o[QCS_POS_SIGX] = s;
-        o[QCS_POS_SIGY] = s;
-        o[QCS_POS_TPHO] = th[3 * k];
+        o[QCS_POS_XPOS] = (th[M::ix(k)] - (float)half) + xc;
+        o[QCS_POS_YPOS] = (th[M::iy(k)] - (float)half) + yc;
+        if constexpr (M::AS) {
+            const float d = sx * sx - sy * sy;
+            o[QCS_POS_ZPOS] = d >= 0.0f ? cf_zpoly(c.pG, d, c.zcorr) : cf_zpoly(c.pL, d, c.zcorr);
+        } else {
+            o[QCS_POS_ZPOS] = 0.0f;
+        }
+        o[QCS_POS_SIGX] = sx;
+        o[QCS_POS_SIGY] = sy;
+        o[QCS_POS_TPHO] = th[M::iN(k)];
         o[QCS_POS_BAKG] = b;
         o[QCS_POS_PSNR] = sqrtf(c.qe) * peak / sqrtf(peak + b);
         o[QCS_POS_CRBX] = cx[k] > 0.0f ? cx[k] * c.pixel_size : -1.0f;
@@ -435,14 +676,15 @@ __device__ __forceinline__ void cf_load_q(const uint16_t *rois, int roi, const C
 // A parameter held at its box bound (cf_project) whose descent direction -gu points further out:
 // it is left out of the quasi-Newton step (active set), so the line search is not defeated by a
 // projection that cancels part of the step.
-template <int K, int R>
-__device__ __forceinline__ bool cf_pinned(const float (&th)[CP<K>::P], int i, float gu)
+template <int KK, int R>
+__device__ __forceinline__ bool cf_pinned(const float (&th)[CP<KK>::P], int i, float gu)
 {
-    constexpr int P = CP<K>::P;
+    constexpr int P = CP<KK>::P;
     const float v = th[i];
     if (i == P - 1) return v <= 0.01f && gu > 0.0f;                                          // b
-    if (i == P - 2) return (v <= 0.4f && gu > 0.0f) || (v >= 0.5f * (float)R && gu < 0.0f);  // s
-    if (i % 3 == 0) return v <= 1.0f && gu > 0.0f;                                           // N_k
+    const int e = CM<KK>::AS ? i % 5 : (i == P - 2 ? 3 : i % 3);
+    if (e >= 3) return (v <= 0.4f && gu > 0.0f) || (v >= 0.5f * (float)R && gu < 0.0f);     // widths
+    if (e == 0) return v <= 1.0f && gu > 0.0f;                                               // N_k
     return (v <= -0.5f && gu > 0.0f) || (v >= (float)R + 0.5f && gu < 0.0f);                // x_k, y_k
 }
 
@@ -452,17 +694,17 @@ __device__ __forceinline__ bool cf_pinned(const float (&th)[CP<K>::P], int i, fl
 // non-positive slope (this includes a zero free gradient).  Otherwise du, slope = du . gu < 0 and
 // dmax = max |du| are set; a non-descent direction restarts from the gradient with D = I (first
 // = true), and with D = I the step is limited to 0.1 scaled units.
-template <int K, int R>
-__device__ __forceinline__ bool cf_direction(const float (&th)[CP<K>::P], const float (&g)[CP<K>::P], float *D, bool &first,
-                                             float tol, bool active_set, float (&du)[CP<K>::P], float &slope, float &dmax)
+template <int KK, int R>
+__device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const float (&g)[CP<KK>::P], float *D, bool &first,
+                                             float tol, bool active_set, float (&du)[CP<KK>::P], float &slope, float &dmax)
 {
-    constexpr int P = CP<K>::P;
+    constexpr int P = CP<KK>::P;
     float gu[P];
     bool pin[P];
 #pragma unroll
     for (int i = 0; i < P; i++) {
-        gu[i] = g[i] * cf_scale(i, P);
-        pin[i] = active_set && cf_pinned<K, R>(th, i, gu[i]);
+        gu[i] = g[i] * cf_scale<KK>(i);
+        pin[i] = active_set && cf_pinned<KK, R>(th, i, gu[i]);
     }
     slope = 0.0f;
 #pragma unroll
@@ -514,17 +756,17 @@ __device__ __forceinline__ bool cf_direction(const float (&th)[CP<K>::P], const 
 
 // BFGS update of the packed inverse Hessian after an accepted step from th to tn with gradients
 // g -> gn (physical units); with D = I (first) D is first rescaled to (s.y / y.y) I.
-template <int K>
-__device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const float (&th)[CP<K>::P], const float (&tn)[CP<K>::P],
-                                               const float (&g)[CP<K>::P], const float (&gn)[CP<K>::P])
+template <int KK>
+__device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const float (&th)[CP<KK>::P], const float (&tn)[CP<KK>::P],
+                                               const float (&g)[CP<KK>::P], const float (&gn)[CP<KK>::P])
 {
-    constexpr int P = CP<K>::P;
+    constexpr int P = CP<KK>::P;
     float s[P], y[P], sy = 0.0f, yy = 0.0f;
 #pragma unroll
-    for (int i = 0; i < P; i++) s[i] = (tn[i] - th[i]) / cf_scale(i, P);
+    for (int i = 0; i < P; i++) s[i] = (tn[i] - th[i]) / cf_scale<KK>(i);
 #pragma unroll
     for (int i = 0; i < P; i++) {
-        y[i] = (gn[i] - g[i]) * cf_scale(i, P);
+        y[i] = (gn[i] - g[i]) * cf_scale<KK>(i);
         sy = fmaf(s[i], y[i], sy);
         yy = fmaf(y[i], y[i], yy);
     }
@@ -599,9 +841,24 @@ struct CfqArgs {
 };
 
 CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_mode);
-// persistent work-queue fit of emitter count K over the sets (cfq.hip), each set started by its
-// own init_mode
+// persistent work-queue fit of K emitters over the sets (cfq.hip), each set started by its own
+// init_mode; the model (2D or elliptical) follows the context's LocType
 int cfq_launch(qcs_ctx *ctx, int K, const CfqSet *sets, int nsets, hipStream_t s);
+
+// model key of K emitters under the context's LocType, and its parameter count
+inline int cf_kk(const qcs_params &p, int K) { return p.LocType == QCS_LOCTYPE_AS3D ? K + CF_ELLIPTIC : K; }
+inline int cf_np(int KK) { return KK > CF_ELLIPTIC ? 5 * (KK - CF_ELLIPTIC) + 1 : 3 * KK + 2; }
+
+#define CF_DISPATCH_KK(KK_, ...)                                                          \
+    switch (KK_) {                                                                         \
+    case 1: { constexpr int KT = 1; __VA_ARGS__; } break;                                  \
+    case 2: { constexpr int KT = 2; __VA_ARGS__; } break;                                  \
+    case 3: { constexpr int KT = 3; __VA_ARGS__; } break;                                  \
+    case 11: { constexpr int KT = 11; __VA_ARGS__; } break;                                \
+    case 12: { constexpr int KT = 12; __VA_ARGS__; } break;                                \
+    case 13: { constexpr int KT = 13; __VA_ARGS__; } break;                                \
+    default: return fail(QCS_ERR_ARG, "model %d: emitter count not in 1..3", KK_);          \
+    }
 
 #define CF_DISPATCH_R(R_, ...)                                                            \
     switch (R_) {                                                                          \

@@ -26,18 +26,18 @@
 namespace qcs {
 
 // record of one work item: theta, loss, gradient, "identity start" flag, packed D0, padding
-template <int K> struct CQ {
-    static constexpr int P = CP<K>::P, NS = CP<K>::NS;
+template <int KK> struct CQ {
+    static constexpr int P = CP<KK>::P, NS = CP<KK>::NS;
     static constexpr int OL = P, OG = P + 1, OF = 2 * P + 1, OD = 2 * P + 2;
     static constexpr int W = (2 * P + 2 + NS + 3) & ~3;   // floats, a multiple of 4
 };
 
 enum { CQ_IDLE = 0, CQ_DIR = 1, CQ_LS = 2 };
 
-template <int K, int R>
+template <int KK, int R>
 __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
 {
-    using Q = CQ<K>;
+    using Q = CQ<KK>;
     constexpr int P = Q::P, NS = Q::NS;
     __shared__ float qsh[R * R * 64];
     __shared__ float Dsh[NS * 64];
@@ -48,10 +48,10 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
     float *qs = qsh + lane, *D = Dsh + lane;
     cf_load_q<R>(S.rois, roi, c, qs);
     float th[P], g[P];
-    cf_init<K, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? S.init_mode : CF_INIT_HEUR, c.sigma0, th);
-    cf_project<K, R>(th);
-    const float L = cf_lossgrad<K, R, true>(qs, th, g);
-    const bool first = !cf_fisher_inverse<K, R>(th, D);
+    cf_init<KK, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? S.init_mode : CF_INIT_HEUR, c.sigma0, th);
+    cf_project<KK, R>(th);
+    const float L = cf_lossgrad<KK, R, true>(qs, th, g);
+    const bool first = !cf_fisher_inverse<KK, R>(th, D);
     if (first) {
 #pragma unroll
         for (int i = 0; i < P; i++)
@@ -75,10 +75,10 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
     }
 }
 
-template <int K, int R>
+template <int KK, int R>
 __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitConst &c, float *qsh, float *Dsh)
 {
-    using Q = CQ<K>;
+    using Q = CQ<KK>;
     constexpr int P = Q::P;
     const int lane = threadIdx.x;
     float *qs = qsh + lane, *D = Dsh + lane;
@@ -140,7 +140,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
     // quasi-Newton direction from a fresh gradient; converged fits end here
     auto direction = [&]() {
         if (st != CQ_DIR) return;
-        if (cf_direction<K, R>(th, g, D, first, c.tol, c.active_set != 0, du, slope, dmax)) {
+        if (cf_direction<KK, R>(th, g, D, first, c.tol, c.active_set != 0, du, slope, dmax)) {
             flags |= CF_CONVERGED;
             finish();
             return;
@@ -159,14 +159,14 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         if (st != CQ_LS) continue;
         float tn[P], gt[P];
 #pragma unroll
-        for (int i = 0; i < P; i++) tn[i] = fmaf(alpha, du[i] * cf_scale(i, P), th[i]);
-        cf_project<K, R>(tn);
-        const float Lt = cf_lossgrad<K, R, true>(qs, tn, gt);
+        for (int i = 0; i < P; i++) tn[i] = fmaf(alpha, du[i] * cf_scale<KK>(i), th[i]);
+        cf_project<KK, R>(tn);
+        const float Lt = cf_lossgrad<KK, R, true>(qs, tn, gt);
         evals++;
         if (Lt == Lt && Lt < L && Lt <= fmaf(1e-4f * alpha, slope, L)) {
             // accepted: BFGS update of the inverse Hessian with s = step, y = gradient change
             iters++;
-            cf_bfgs_update<K>(D, first, th, tn, g, gt);
+            cf_bfgs_update<KK>(D, first, th, tn, g, gt);
 #pragma unroll
             for (int i = 0; i < P; i++) {
                 th[i] = tn[i];
@@ -201,24 +201,24 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
     }
 }
 
-template <int K, int RMAX>
+template <int KK, int RMAX>
 __global__ void __launch_bounds__(64) cfq_kernel(CfqArgs a)
 {
     __shared__ float qsh[RMAX * RMAX * 64];
-    __shared__ float Dsh[CP<K>::NS * 64];
+    __shared__ float Dsh[CP<KK>::NS * 64];
     for (int si = 0; si < a.nsets; si++) {
         const int R = a.set[si].R;
         if constexpr (RMAX >= 11) {
-            if (R == 11) cfq_run<K, 11>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+            if (R == 11) cfq_run<KK, 11>(a.set[si], a.heads + si, a.c, qsh, Dsh);
         }
         if constexpr (RMAX >= 9) {
-            if (R == 9) cfq_run<K, 9>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+            if (R == 9) cfq_run<KK, 9>(a.set[si], a.heads + si, a.c, qsh, Dsh);
         }
-        if (R == 7) cfq_run<K, 7>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+        if (R == 7) cfq_run<KK, 7>(a.set[si], a.heads + si, a.c, qsh, Dsh);
     }
 }
 
-template <int K, int R>
+template <int KK, int R>
 __global__ void __launch_bounds__(64) cfq_ll_kernel(CfqSet S, CFitConst c)
 {
     __shared__ float qsh[R * R * 64];
@@ -228,15 +228,25 @@ __global__ void __launch_bounds__(64) cfq_ll_kernel(CfqSet S, CFitConst c)
     const int roi = S.idx ? S.idx[j] : j;
     float *qs = qsh + lane;
     cf_load_q<R>(S.rois, roi, c, qs);
-    float th[CP<K>::P];
+    float th[CP<KK>::P];
 #pragma unroll
-    for (int i = 0; i < CP<K>::P; i++) th[i] = S.theta[(size_t)roi * S.ostride + i];
-    S.ll[roi] = cf_loglik64<K, R>(qs, th);
+    for (int i = 0; i < CP<KK>::P; i++) th[i] = S.theta[(size_t)roi * S.ostride + i];
+    S.ll[roi] = cf_loglik64<KK, R>(qs, th);
 }
 
 // ---- host -----------------------------------------------------------------------------------
 
-static int cq_rec_w(int K) { return K == 1 ? CQ<1>::W : (K == 2 ? CQ<2>::W : CQ<3>::W); }
+static int cq_rec_w(int KK)
+{
+    switch (KK) {
+    case 1: return CQ<1>::W;
+    case 2: return CQ<2>::W;
+    case 3: return CQ<3>::W;
+    case 11: return CQ<11>::W;
+    case 12: return CQ<12>::W;
+    default: return CQ<13>::W;
+    }
+}
 
 // resident work-groups per CU of kernel f
 static int cq_occupancy(const void *f)
@@ -246,26 +256,16 @@ static int cq_occupancy(const void *f)
     return nb;
 }
 
-template <int K, int RMAX> static const void *cq_kernel_ptr() { return reinterpret_cast<const void *>(&cfq_kernel<K, RMAX>); }
-
-static int cq_waves_per_cu(int K, int RMAX)
+// resident persistent work-groups per CU for model key KK and largest ROI size RMAX (cached)
+static int cq_waves_per_cu(int KK, int RMAX)
 {
-    static int cache[3][3] = {{0}};
+    static int cache[2][3][3] = {{{0}}};
     const int ri = RMAX == 7 ? 0 : (RMAX == 9 ? 1 : 2);
-    int &v = cache[K - 1][ri];
+    const int as = KK > CF_ELLIPTIC ? 1 : 0, K = as ? KK - CF_ELLIPTIC : KK;
+    int &v = cache[as][K - 1][ri];
     if (v == 0) {
         const void *f = nullptr;
-        switch (K * 100 + RMAX) {
-        case 107: f = cq_kernel_ptr<1, 7>(); break;
-        case 109: f = cq_kernel_ptr<1, 9>(); break;
-        case 111: f = cq_kernel_ptr<1, 11>(); break;
-        case 207: f = cq_kernel_ptr<2, 7>(); break;
-        case 209: f = cq_kernel_ptr<2, 9>(); break;
-        case 211: f = cq_kernel_ptr<2, 11>(); break;
-        case 307: f = cq_kernel_ptr<3, 7>(); break;
-        case 309: f = cq_kernel_ptr<3, 9>(); break;
-        default: f = cq_kernel_ptr<3, 11>(); break;
-        }
+        CF_DISPATCH_KK(KK, CF_DISPATCH_R(RMAX, f = reinterpret_cast<const void *>(&cfq_kernel<KT, RT>)))
         v = cq_occupancy(f);
     }
     return v;
@@ -278,7 +278,8 @@ int cfq_launch(qcs_ctx *ctx, int K, const CfqSet *sets_in, int nsets, hipStream_
     CfqArgs a;
     memset(&a, 0, sizeof(a));
     a.c = cfit_const(ctx->p, ctx->tune, K, CF_INIT_HEUR);
-    const int W = cq_rec_w(K);
+    const int KK = cf_kk(ctx->p, K);
+    const int W = cq_rec_w(KK);
     size_t total = 0;
     int rmax = 0;
     long long items = 0;
@@ -318,41 +319,23 @@ int cfq_launch(qcs_ctx *ctx, int K, const CfqSet *sets_in, int nsets, hipStream_
     for (int i = 0; i < a.nsets; i++) {
         const CfqSet &x = a.set[i];
         const dim3 grid((x.n + 63) / 64), block(64);
-        if (K == 1) {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<1, RT>), grid, block, 0, s, x, a.c))
-        } else if (K == 2) {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<2, RT>), grid, block, 0, s, x, a.c))
-        } else {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<3, RT>), grid, block, 0, s, x, a.c))
-        }
+        CF_DISPATCH_KK(KK, CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_prep_kernel<KT, RT>), grid, block, 0, s, x, a.c)))
     }
     QCS_HIP(hipGetLastError());
     int cus = 0;
     QCS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    int wpc = cq_waves_per_cu(K, rmax);
+    int wpc = cq_waves_per_cu(KK, rmax);
     if (ctx->tune.waves_per_cu > 0 && ctx->tune.waves_per_cu < wpc) wpc = ctx->tune.waves_per_cu;
     long long waves = (items + 63) / 64;
     if (waves > (long long)cus * wpc) waves = (long long)cus * wpc;
     const dim3 pgrid((unsigned)waves), block(64);
-    if (K == 1) {
-        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<1, RT>), pgrid, block, 0, s, a))
-    } else if (K == 2) {
-        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<2, RT>), pgrid, block, 0, s, a))
-    } else {
-        CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<3, RT>), pgrid, block, 0, s, a))
-    }
+    CF_DISPATCH_KK(KK, CF_DISPATCH_R(rmax, hipLaunchKernelGGL((cfq_kernel<KT, RT>), pgrid, block, 0, s, a)))
     QCS_HIP(hipGetLastError());
     for (int i = 0; i < a.nsets; i++) {
         const CfqSet &x = a.set[i];
         if (!x.ll) continue;
         const dim3 grid((x.n + 63) / 64);
-        if (K == 1) {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<1, RT>), grid, block, 0, s, x, a.c))
-        } else if (K == 2) {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<2, RT>), grid, block, 0, s, x, a.c))
-        } else {
-            CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<3, RT>), grid, block, 0, s, x, a.c))
-        }
+        CF_DISPATCH_KK(KK, CF_DISPATCH_R(x.R, hipLaunchKernelGGL((cfq_ll_kernel<KT, RT>), grid, block, 0, s, x, a.c)))
     }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
