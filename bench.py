@@ -23,7 +23,9 @@ A second leg (`multi_emitter` in the JSON line) fits 2^20 simulator 7x7 ROIs wit
 1-4 px apart twice: the plain converged path (K = 1, 2, 3 fits and FP64 likelihood-ratio model
 selection) and the learned warm start (network -> converged fit of the predicted count, plain
 fallback), reporting both throughputs, the speedup, and RMSE / recall / RMSE-over-CRLB against
-the simulated truth on the first 65,536 ROIs.
+the simulated truth on the first 65,536 ROIs.  A third (`as3d_multi_emitter`) fits 2^18 astigmatic
+9x9 ROIs with 1-3 emitters with the elliptical model (per-emitter widths, up to 16 parameters) and
+reports fits/s, the count accuracy, RMSE over CRLB and the z error against the simulated truth.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--math exact|fast] [--no-cpu-baseline]
 """
@@ -108,17 +110,18 @@ def drift_workload(W, H, nframes, npts, seed, ax=0.005, ay=-0.003, wobble=0.1):
     return np.concatenate(out), dx, dy
 
 
-def match_rmse(loc, truth, pixel_size):
+def match_rmse(loc, truth, pixel_size, R=7, zerr=False):
     """Greedy nearest matching (within 1 px) of fitted rows [n][3][12] to simulated emitters
-    [n][3][8] of 7x7 ROIs (header Xc = Yc = 100); returns rmse (px, 2D: sqrt of mean dx^2 + dy^2),
-    recall, precision and that RMSE over the bound sqrt(mean CRLBx^2 + CRLBy^2) of the matched rows."""
+    [n][3][8] of R x R ROIs (header Xc = Yc = 100); returns rmse (px, 2D: sqrt of mean dx^2 + dy^2),
+    recall, precision and that RMSE over the bound sqrt(mean CRLBx^2 + CRLBy^2) of the matched rows
+    (and, with zerr, the z RMSE and median |dz| of the matched emitters, nm)."""
     n = loc.shape[0]
-    fx, fy = loc[:, :, 1] - 97.0, loc[:, :, 2] - 97.0
+    fx, fy = loc[:, :, 1] - (100.0 - R // 2), loc[:, :, 2] - (100.0 - R // 2)
     fv = loc[:, :, 0] > 0
     tv = truth[:, :, 4] > 0
     D = (fx[:, None, :] - truth[:, :, 1][:, :, None]) ** 2 + (fy[:, None, :] - truth[:, :, 2][:, :, None]) ** 2
     D[~(tv[:, :, None] & fv[:, None, :])] = np.inf
-    d2, crb, roi = [], [], []
+    d2, crb, roi, dz = [], [], [], []
     rows = np.arange(n)
     for _ in range(3):
         flat = D.reshape(n, 9)
@@ -130,16 +133,18 @@ def match_rmse(loc, truth, pixel_size):
         c = loc[rows, fi, 9:11][ok] / pixel_size
         crb.append((c ** 2).sum(1))   # CRLBx^2 + CRLBy^2: the bound on E[dx^2 + dy^2]
         roi.append(rows[ok])
+        dz.append(loc[rows, fi, 3][ok] - truth[rows, ti, 3][ok])
         D[rows[ok], ti[ok], :] = np.inf
         D[rows[ok], :, fi[ok]] = np.inf
-    d2, crb, roi = np.concatenate(d2), np.concatenate(crb), np.concatenate(roi)
+    d2, crb, roi, dz = np.concatenate(d2), np.concatenate(crb), np.concatenate(roi), np.concatenate(dz)
     # RMSE against the CRLB over the matched emitters with a finite, informative bound (< 0.5 px per
     # axis); near-degenerate multi-emitter fits have unbounded CRLBs that would swamp the mean
     ok = np.isfinite(crb) & (crb > 0) & (crb < 0.5)
     # the same over the ROIs whose fitted emitter count equals the true count (the model the CRLB
     # is computed for), where the fit's precision, not the count selection, sets the error
     right = (fv.sum(1) == tv.sum(1))[roi] & ok
-    return {"rmse_px": float(np.sqrt(d2.mean())), "recall": float(len(d2) / max(tv.sum(), 1)),
+    zs = {"z_rmse_nm": float(np.sqrt(np.mean(dz ** 2))), "z_median_abs_err_nm": float(np.median(np.abs(dz)))} if zerr else {}
+    return {**zs, "rmse_px": float(np.sqrt(d2.mean())), "recall": float(len(d2) / max(tv.sum(), 1)),
             "precision": float(len(d2) / max(fv.sum(), 1)),
             "rmse_over_crlb": float(np.sqrt(d2[ok].mean() / crb[ok].mean())) if ok.any() else None,
             "crlb_informative_fraction": float(ok.mean()) if len(ok) else None,
@@ -191,6 +196,43 @@ def multi_emitter_leg(n, seed, log):
             "rmse_over_crlb_count_correct_warm": rw["rmse_over_crlb_count_correct"],
             "fallback_fraction": st["fallbacks"] / n, "warm_iterations_mean": st["iters"] / n,
             "weights": wsrc, "accuracy_sample_rois": m}
+
+
+def as3d_multi_emitter_leg(n, seed, log):
+    """Converged fitting of the elliptical model (per-emitter widths, K = 1..3, up to 16
+    parameters, FP64 selection and CRLB, z from the calibration polynomial) over n simulator 9x9
+    astigmatic ROIs with 1-3 emitters 1-4 px apart; throughput from library HIP events."""
+    import torch
+    R, kmax = 9, 3
+    sp = Q.default_sim(width=R, height=R, astig=1, n_per_unit_max=kmax, min_sep=1.0, max_sep=4.0, photons_min=1000.0,
+                       photons_max=8000.0, bg_min=20.0, bg_max=80.0)
+    rois = torch.empty((n, Q.roi_words(R)), dtype=torch.int16, device="cuda")
+    truth = torch.empty((n, kmax, Q.TRUTH_FLOATS), dtype=torch.float32, device="cuda")
+    Q.check(Q.lib().qcs_sim_rois_dev(C.byref(sp), seed, n, 1, C.c_void_p(rois.data_ptr()), C.c_void_p(truth.data_ptr()), None),
+            "qcs_sim_rois_dev")
+    # sigma_x/y = 1.2 sqrt(1 + ((z -+ 250) / 400)^2): z = -111.11 (sigma_x^2 - sigma_y^2)
+    p = Q.default_params(QE=1.0, ROISize=R, LocType=Q.LOCTYPE_AS3D, PixelSize=100.0, p1_XGY=-111.11, p1_XLY=-111.11,
+                         ZDepthCorrFactor=1.0)
+    ctx = Q.Context(p)
+    loc = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
+    sel = torch.zeros(n, dtype=torch.int32, device="cuda")
+    ctx.fit_mle_dev(rois, n, kmax, loc, sel)
+    torch.cuda.synchronize()
+    ctx.timing_enable(True)
+    ctx.fit_mle_dev(rois, n, kmax, loc, sel)
+    t = ctx.timing()
+    st = ctx.fit_plain_stats_dev(rois, n, kmax, loc, sel)
+    torch.cuda.synchronize()
+    m = min(n, 65536)
+    th = truth[:m].cpu().numpy()
+    acc = match_rmse(loc[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize, R=R, zerr=True)
+    s_h, k_true = sel[:m].cpu().numpy(), (th[:, :, 4] > 0).sum(1)
+    ctx.close()
+    return {"rois": n, "roi": R, "fits_per_s": n / (t["cfit_ms"] * 1e-3), "ms": t["cfit_ms"],
+            "iterations_mean": st["iters"] / n, "final_counts": [st["final_k1"], st["final_k2"], st["final_k3"]],
+            "count_accuracy_by_true_count": {str(k): float((s_h[k_true == k] == k).mean()) for k in (1, 2, 3)},
+            "accuracy": acc, "accuracy_sample_rois": m,
+            "data": "simulator: photons log-U(1000, 8000), background U(20, 80), z U(-500, 500) nm"}
 
 
 class Pipeline:
@@ -272,6 +314,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--multi-rois", type=int, default=1 << 20,
                     help="ROIs of the multi-emitter plain vs warm-start leg (0 skips it)")
+    ap.add_argument("--as3d-rois", type=int, default=1 << 18,
+                    help="ROIs of the elliptical-model multi-emitter leg (0 skips it)")
     ap.add_argument("--no-fast-leg", action="store_true", help="skip the fast-math pipeline leg")
     ap.add_argument("--lanes", type=int, default=1,
                     help="frame groups processed concurrently (own context, stream and host thread each)")
@@ -478,6 +522,10 @@ def main():
     if args.multi_rois > 0:
         log("bench: multi-emitter leg (plain vs warm start, %d ROIs)" % args.multi_rois)
         multi = multi_emitter_leg(args.multi_rois, 1003, log)
+    as3d = None
+    if args.as3d_rois > 0:
+        log("bench: elliptical-model multi-emitter leg (%d 9x9 ROIs)" % args.as3d_rois)
+        as3d = as3d_multi_emitter_leg(args.as3d_rois, 1004, log)
 
     cpu = None
     parity = None
@@ -555,7 +603,8 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "fast_math": fast, "multi_emitter": multi, "statistics": statistics, "consecutive_merge": consec,
+            "fast_math": fast, "multi_emitter": multi, "as3d_multi_emitter": as3d, "statistics": statistics,
+            "consecutive_merge": consec,
             "drift_correction": drift}
     cons.close()
     ctx_c.close()
