@@ -100,6 +100,9 @@ extern "C" int qcs_autotune_mle_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *
         !(opts->max_dx_p99 >= 0.0f))
         return fail(QCS_ERR_ARG, "invalid autotune options");
     if (net && ctx->p.ROISize != 7) return fail(QCS_ERR_ARG, "the warm start takes 7x7 ROIs");
+    if (net && ctx->p.LocType != QCS_LOCTYPE_GS2D) return fail(QCS_ERR_ARG, "the warm start supports the 2D model only");
+    // the elliptical model's fits fix their own caps and stopping rule: only the kernel choice is tuned
+    const bool elliptical = ctx->p.LocType == QCS_LOCTYPE_AS3D;
     hipStream_t s = (hipStream_t)stream;
     const qcs_tuning base = ctx->tune;
     int rc = QCS_OK;
@@ -126,11 +129,12 @@ extern "C" int qcs_autotune_mle_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *
     std::vector<qcs_tuning> cand;
     const int caps[][3] = {{100, 60, 60}, {100, 40, 40}, {100, 30, 30}, {100, 20, 25}};
     const float ftols[] = {0.0f, 1e-3f, 1e-2f};
-    for (const auto &cp : caps)
-        for (float ft : ftols) {
+    for (int ci = 0; ci < 4; ci++)
+        for (int fi = 0; fi < 3; fi++) {
+            if (elliptical && (ci > 0 || fi > 0)) continue;
             qcs_tuning t = base;
-            for (int k = 0; k < 3; k++) t.max_iter[k] = cp[k];
-            t.ftol = ft;
+            for (int k = 0; k < 3; k++) t.max_iter[k] = caps[ci][k];
+            t.ftol = ftols[fi];
             t.queue = 1;
             t.waves_per_cu = 0;
             cand.push_back(t);

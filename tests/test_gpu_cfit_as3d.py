@@ -221,3 +221,45 @@ def test_elliptical_model_refused_by_the_2d_only_paths(gpu):
         ctx.fit_warm_dev(Q.Net(seed=1), d_rois, n, loc, None)
     with pytest.raises(RuntimeError, match="2D model only"):
         ctx.label_dev(d_rois, n, torch.zeros((n, 16), dtype=torch.float32, device="cuda"))
+
+
+def test_elliptical_graph_replays_match_eager(gpu):
+    """A captured plain-path batch of the elliptical model: every replay equals the eager call."""
+    R, n = 9, 1201
+    ctx = Q.Context(_params(R))
+    buf = torch.from_numpy(_sim(R, n, 1900, kmax=3, min_sep=1.5, max_sep=4.0)[0].view(np.int16)).cuda()
+    loc = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
+    sel = torch.zeros(n, dtype=torch.int32, device="cuda")
+    g = ctx.graph_mle(buf, n, 3, loc, sel)
+    for rep, seed in enumerate((1901, 1902, 1901)):
+        new = torch.from_numpy(_sim(R, n, seed, kmax=3, min_sep=1.5, max_sep=4.0)[0].view(np.int16)).cuda()
+        buf.copy_(new)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        got_l, got_s = loc.cpu().numpy(), sel.cpu().numpy()
+        ref_l, ref_s = torch.zeros_like(loc), torch.zeros_like(sel)
+        ctx.fit_mle_dev(new, n, 3, ref_l, ref_s)
+        torch.cuda.synchronize()
+        print("replay", rep, "counts", np.bincount(got_s, minlength=4))
+        np.testing.assert_array_equal(got_s, ref_s.cpu().numpy())
+        np.testing.assert_array_equal(got_l.view(np.uint32), ref_l.cpu().numpy().view(np.uint32))
+    g.close()
+
+
+def test_elliptical_autotune_tunes_the_kernel_choice_only(gpu):
+    R, n = 9, 1 << 13
+    ctx = Q.Context(_params(R))
+    d_rois = torch.from_numpy(_sim(R, n, 2000, kmax=3, min_sep=1.5, max_sep=4.0)[0].view(np.int16)).cuda()
+    tab = ctx.autotune_dev(d_rois, n, trials=1, min_select_agree=0.99, max_dx_p99=0.02)
+    for e in tab:
+        print(e)
+    # one candidate per kernel choice: the persistent queue at all, 1, 2 and 4 waves per CU, and
+    # the one-pass kernels; all give the reference's rows (the caps and stop are the model's own)
+    assert len(tab) == 5 and not any(e["warm"] for e in tab)
+    assert all(e["select_agree"] == 1.0 and e["dx_p99"] == 0.0 for e in tab)
+    chosen = [e for e in tab if e["chosen"]]
+    assert len(chosen) == 1 and chosen[0]["ms"] == min(e["ms"] for e in tab)
+    c7 = Q.Context(_params(7))
+    with pytest.raises(RuntimeError, match="2D model only"):
+        c7.autotune_dev(torch.zeros((64, 56), dtype=torch.int16, device="cuda"), 64, net=Q.Net(seed=1))
