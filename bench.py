@@ -17,7 +17,10 @@ entry is for the dominant kernel (the 7x7 one-emitter BFGS fit): algorithmic FLO
 (120,736 FLOP per fit = 176 loss evaluations x 49 pixels x 14 FLOP, FMA = 2) divided by the
 average launch duration, measured with HIP events recorded on the launch stream by the library.
 The CPU baseline is the oracle's OpenMP fit + CRLB (same schedule, bit-identical output) on a
-bounded sample of the same ROIs, on the host cores.
+bounded sample of the same ROIs, on the host cores.  `parity_sample` counts the bit-exact rows of
+that sample and (`vs_fp64_mle`) states the rows' error against the oracle's FP64 maximum-
+likelihood fit run to convergence, on the first 16,384 kept rows, separately for unweighted
+(MLEFit) and WLE-weighted ROIs.
 
 A second leg (`multi_emitter` in the JSON line) fits 2^20 simulator 7x7 ROIs with 1-3 emitters
 1-4 px apart twice: the plain converged path (K = 1, 2, 3 fits and FP64 likelihood-ratio model
@@ -153,6 +156,38 @@ def match_rmse(loc, truth, pixel_size, R=7, zerr=False):
             "rmse_over_crlb_count_correct_by_count": {
                 str(k): (float(np.sqrt(d2[sel].mean() / crb[sel].mean())) if sel.any() else None)
                 for k in (1, 2, 3) for sel in [right & (tv.sum(1)[roi] == k)]}}
+
+
+def fp64_error(params, rois, wle, rows, m):
+    """Error of single-emitter rows against the FP64 oracle MLE (Levenberg-Marquardt on the
+    unweighted Poisson likelihood, run to convergence from the row's parameters) over the first m
+    kept rows.  The reference schedule weights all but MLEFit ROIs (WLE), so the two estimators
+    share an optimum only for MLEFit ROIs; the classes are reported separately."""
+    import oracle_lib as O   # checker only
+    R, half = params.ROISize, params.ROISize // 2
+    keep = np.nonzero(rows[:, 0] > 0)[0][:m]
+    out = {}
+    for name, cls in (("mlefit", 0), ("weighted", None)):   # WLE type 0: MLEFit (unweighted)
+        idx = keep[(wle[keep, 5] == cls) if cls is not None else (wle[keep, 5] != 0)]
+        dxy, dn, db, conv = [], [], [], 0
+        for i in idx:
+            img = rois[i, :R * R].astype(np.float64).reshape(R, R)
+            q = np.maximum((img - params.Offset) * np.float32(params.KAdc), 1.0) / params.QE
+            xc, yc = float(rois[i, R * R]), float(rois[i, R * R + 1])
+            x, y = rows[i, 1] - xc + half, rows[i, 2] - yc + half
+            t, it, _ = O.mle(q, K=1, theta0=np.array([rows[i, 6], x, y, rows[i, 4], rows[i, 7]], np.float64))
+            conv += it < 200
+            dxy.append(max(abs(t[1] - x), abs(t[2] - y)))
+            dn.append(abs(t[0] - rows[i, 6]) / t[0])
+            db.append(abs(t[4] - rows[i, 7]) / t[4])
+        if not dxy:
+            continue
+        dxy, dn, db = map(np.array, (dxy, dn, db))
+        out[name] = {"rows": int(len(idx)), "oracle_converged": conv / len(idx),
+                     "dxy_px_p50": float(np.median(dxy)), "dxy_px_p99": float(np.percentile(dxy, 99)),
+                     "photons_rel_p50": float(np.median(dn)), "photons_rel_p99": float(np.percentile(dn, 99)),
+                     "bkg_rel_p50": float(np.median(db)), "bkg_rel_p99": float(np.percentile(db, 99))}
+    return out
 
 
 def multi_emitter_leg(n, seed, log):
@@ -539,6 +574,7 @@ def main():
             got = L0["loc"][:ref.shape[0]].cpu().numpy()
             same = int(np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1).sum())
             parity = {"rows": int(ref.shape[0]), "bit_exact_rows": same}
+            parity["vs_fp64_mle"] = fp64_error(params, h_rois, h_wle, got, 16384)
         # statistics: the oracle (per-frame all-pairs neighbour scan, one thread) on the rows of the
         # first 6 frames, against the device call on the same rows
         import oracle_lib as O   # checker only
