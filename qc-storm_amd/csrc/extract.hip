@@ -239,6 +239,11 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     int *rcount = row_count + (size_t)b * c.B * c.H;
     const int nseg = c.nseg, H = c.H;
     int ryd = ((rb - 3) % H + H) % H;   // frame row of detection row r-3 for the current step
+    // raw columns xc+4 (rows r-4 .. r+6) and xc-4 (rows r-3 .. r+7) of the current step; an
+    // interior step after another step keeps the previous step's seven newest rows of each and
+    // reads four from LDS (a walk's first step may be interior: then all eleven are read)
+    int colR[RS + 7], colL[RS + 7];
+    bool have_cols = false;   // uniform
 
     // One step of 4 rows.  EDGE: rows near the batch or walk ends (bounds checks, clamped rows,
     // dynamic ring slots); otherwise every row is in range and PH fixes the ring slots.
@@ -263,11 +268,24 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
         }
         __syncthreads();
         // background sums of rows r .. r+3
-        int colR[RS + 7], colL[RS + 7];   // column xc+4 rows r-4.., column xc-4 rows r-3..
+        if (EDGE || !have_cols) {
 #pragma unroll
-        for (int i = 0; i < RS + 7; i++) {
-            colR[i] = raw[rs(i - 4)][l + 8];
-            colL[i] = raw[rs(i - 3)][l];
+            for (int i = 0; i < RS + 7; i++) {
+                colR[i] = raw[rs(i - 4)][l + 8];
+                colL[i] = raw[rs(i - 3)][l];
+            }
+            have_cols = true;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                colR[i] = colR[i + RS];
+                colL[i] = colL[i + RS];
+            }
+#pragma unroll
+            for (int i = 7; i < RS + 7; i++) {
+                colR[i] = raw[rs(i - 4)][l + 8];
+                colL[i] = raw[rs(i - 3)][l];
+            }
         }
 #pragma unroll
         for (int k = 0; k < RS; k++) {
