@@ -525,7 +525,7 @@ def main():
     params_d = Q.QcsParams.from_buffer_copy(params)
     ctx_d = Q.Context(params_d)
     d_rows = torch.from_numpy(d_rows_h).to(dev)
-    corr = 50
+    corr = min(50, max(F // 4, 1))   # 50-frame groups at the default 200 frames
     ctx_d.drift_correct_dev(d_rows.clone(), d_rows_h.shape[0], corr, stream=sh)   # warm-up (allocates the SR images)
     ctx_d.timing_enable(True)
     reps = 2
