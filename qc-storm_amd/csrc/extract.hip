@@ -161,7 +161,6 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     constexpr int NR = 16;   // raw ring rows (a step needs rows r-4 .. r+RS+3); power of two
     __shared__ uint16_t raw[NR][RAWW];
     __shared__ int bks[RS][64];
-    __shared__ float tvs[RS][64];
     __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
     __shared__ int svs[8][64];
 
@@ -230,6 +229,9 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     int hci[5];   // H-pass source lanes (columns clamped to the image)
 #pragma unroll
     for (int t = 0; t < 5; t++) hci[t] = max(min(clampi(xc - 2 + t, 0, W - 1) - (x0 - 4), 63), 0);
+    // lanes of image columns 0 and W-1 when the strip reaches a border (the H pass clamps to them)
+    const int lane_lo = max(min(0 - (x0 - 4), 63), 0), lane_hi = max(min(W - 1 - (x0 - 4), 63), 0);
+    const bool border = x0 - 4 + 1 < 2 || x0 - 4 + 62 > W - 3;   // uniform: some lane 1..62 clamps
     const int half = c.R / 2;
     const bool out_lane = (l >= 4 && l < 4 + SEGW);
     const bool det_col = out_lane && xc < W && xc >= half && xc < W - half;
@@ -317,20 +319,29 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
             }
             return;
         }
-        if constexpr (SMOOTH) {
-#pragma unroll
-            for (int k = 0; k < RS; k++) tvs[k][l] = (float)tvk[k];
-            __syncthreads();
-        }
-        // H pass into the lane's ring (rows inside the batch only)
+        // H pass into the lane's ring (rows inside the batch only).  The values of lanes l-2 .. l+2
+        // come by DPP wave shifts; a strip at the image border takes its clamped columns from the
+        // border lane (readlane).  Lanes 3 .. 60, the ones whose results are used, never read a
+        // lane outside the wave.
 #pragma unroll
         for (int k = 0; k < RS; k++) {
             if (EDGE && (r + k < 0 || r + k >= BH)) continue;   // uniform
             float hc;
             if constexpr (SMOOTH) {
+                const int v0 = tvk[k];
+                const int m1 = __builtin_amdgcn_update_dpp(0, v0, 0x138, 0xf, 0xf, false);   // lane l-1
+                const int m2 = __builtin_amdgcn_update_dpp(0, m1, 0x138, 0xf, 0xf, false);   // lane l-2
+                const int p1 = __builtin_amdgcn_update_dpp(0, v0, 0x130, 0xf, 0xf, false);   // lane l+1
+                const int p2 = __builtin_amdgcn_update_dpp(0, p1, 0x130, 0xf, 0xf, false);   // lane l+2
+                int src[5] = {m2, m1, v0, p1, p2};
+                if (border) {   // uniform
+                    const int vl = __builtin_amdgcn_readlane(v0, lane_lo), vr = __builtin_amdgcn_readlane(v0, lane_hi);
+#pragma unroll
+                    for (int t = 0; t < 5; t++) src[t] = hci[t] == lane_lo ? vl : (hci[t] == lane_hi ? vr : src[t]);
+                }
                 float acc = 0.0f;
 #pragma unroll
-                for (int t = 0; t < 5; t++) acc = __builtin_fmaf(tvs[k][hci[t]], tap[t], acc);
+                for (int t = 0; t < 5; t++) acc = __builtin_fmaf((float)src[t], tap[t], acc);
                 acc = acc > 65535.0f ? 65535.0f : acc;
                 hc = (float)(int)acc;
             } else {
