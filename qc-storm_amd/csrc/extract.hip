@@ -162,7 +162,6 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     __shared__ uint16_t raw[NR][RAWW];
     __shared__ int bks[RS][64];
     __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
-    __shared__ int svs[8][64];
 
     const int b = blockIdx.z;
     const int BH = batch_rows(c, b);
@@ -246,6 +245,11 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     // reads four from LDS (a walk's first step may be interior: then all eleven are read)
     int colR[RS + 7], colL[RS + 7];
     bool have_cols = false;   // uniform
+    // the lane's V-smoothed rows r-4 .. r+RS-3: [0, 2) from the previous step, [2, RS+2) this
+    // step's V pass; detection takes lanes l-1 and l+1 of them by DPP wave shifts
+    int svr[RS + 2];
+#pragma unroll
+    for (int i = 0; i < RS + 2; i++) svr[i] = 0;
 
     // One step of 4 rows.  EDGE: rows near the batch or walk ends (bounds checks, clamped rows,
     // dynamic ring slots); otherwise every row is in range and PH fixes the ring slots.
@@ -369,25 +373,26 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
             } else {
                 sv = (int)hcs[h8(k - 2)][l];
             }
-            svs[h8(k - 2)][l] = sv;
+            svr[2 + k] = sv;
         }
-        __syncthreads();
-        // detection rows r-3 .. r
+        // detection rows r-3 .. r (sv rows r-4 .. r+1); the shifts run on the whole wave, ahead
+        // of the per-lane tests
 #pragma unroll
         for (int k = 0; k < RS; k++) {
             const int d = r - 3 + k;
             int ry = ryd + k;
             ry = ry >= H ? ry - H : ry;
             if (EDGE && (d < ys || d >= ye)) continue;   // uniform
+            const int p01 = svr[k], cv = svr[k + 1], p21 = svr[k + 2];
+            const int p00 = __builtin_amdgcn_update_dpp(0, p01, 0x138, 0xf, 0xf, false);   // lane l-1
+            const int p02 = __builtin_amdgcn_update_dpp(0, p01, 0x130, 0xf, 0xf, false);   // lane l+1
+            const int p10 = __builtin_amdgcn_update_dpp(0, cv, 0x138, 0xf, 0xf, false);
+            const int p12 = __builtin_amdgcn_update_dpp(0, cv, 0x130, 0xf, 0xf, false);
+            const int p20 = __builtin_amdgcn_update_dpp(0, p21, 0x138, 0xf, 0xf, false);
+            const int p22 = __builtin_amdgcn_update_dpp(0, p21, 0x130, 0xf, 0xf, false);
             bool cand = false, mark = false;
             if (det_col && ry >= half && ry < H - half) {
-                const int *md = svs[h8(k - 3)];
-                const int cv = md[l];
                 if ((float)cv >= th) {
-                    const int *up = svs[h8(k - 4)], *dn = svs[h8(k - 2)];
-                    const int p00 = up[l - 1], p01 = up[l], p02 = up[l + 1];
-                    const int p10 = md[l - 1], p12 = md[l + 1];
-                    const int p20 = dn[l - 1], p21 = dn[l], p22 = dn[l + 1];
                     const int s8 = ((float)p00 > th1) + ((float)p01 > th1) + ((float)p02 > th1) + ((float)p10 > th1) +
                                    ((float)p12 > th1) + ((float)p20 > th1) + ((float)p21 > th1) + ((float)p22 > th1);
                     const int sin = (cv > p01) + (cv >= p12) + (cv >= p21) + (cv > p10);
@@ -404,6 +409,8 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
                 if (cm) atomicAdd(&rcount[d], __popcll(cm));
             }
         }
+        svr[0] = svr[RS];
+        svr[1] = svr[RS + 1];
     };
 
     using E = std::true_type;
