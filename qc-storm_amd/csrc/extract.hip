@@ -160,7 +160,6 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     constexpr int RS = 4;    // image rows per step
     constexpr int NR = 16;   // raw ring rows (a step needs rows r-4 .. r+RS+3); power of two
     __shared__ uint16_t raw[NR][RAWW];
-    __shared__ int bks[RS][64];
     __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
 
     const int b = blockIdx.z;
@@ -293,6 +292,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
                 colL[i] = raw[rs(i - 3)][l];
             }
         }
+        int bkv[RS];
 #pragma unroll
         for (int k = 0; k < RS; k++) {
             const uint16_t *rA = raw[rs(k - 4)], *rB = raw[rs(k + 4)];
@@ -304,14 +304,17 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
                 const int m1 = l1 < l2 ? l1 : l2, m2 = l3 < l4 ? l3 : l4;
                 bk += m1 < m2 ? m1 : m2;
             }
-            bks[k][l] = bk;
+            bkv[k] = bk;
         }
-        __syncthreads();
+        // the neighbours' sums by DPP wave shifts: li is l - 1 or l, ri is l or l + 1
         int tvk[RS];
 #pragma unroll
         for (int k = 0; k < RS; k++) {
+            const int bm = __builtin_amdgcn_update_dpp(0, bkv[k], 0x138, 0xf, 0xf, false);   // lane l-1
+            const int bp = __builtin_amdgcn_update_dpp(0, bkv[k], 0x130, 0xf, 0xf, false);   // lane l+1
+            const int bl = li == l ? bkv[k] : bm, br = ri == l ? bkv[k] : bp;
             const int rawc = raw[rs(k)][l + 4];
-            const int t = rawc - (bks[k][li] + bks[k][l] + bks[k][ri] + 23) / 24;
+            const int t = rawc - (bl + bkv[k] + br + 23) / 24;
             tvk[k] = t > 0 ? t : 0;
         }
         if constexpr (MODE == 0) {
