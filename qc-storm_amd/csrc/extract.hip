@@ -128,7 +128,7 @@ __device__ __forceinline__ void batch_thresholds(const ExtractConst &c, const un
 // H-smoothed values on 3..60, detection on 4..59 -- SEGW = 56 output columns per wave.
 //
 // A step handles 4 image rows r..r+3: the raw rows r+4..r+7 (loaded from HBM during the previous
-// step) enter a 12-row LDS ring, then for each row r:
+// step) enter a 16-row LDS ring, then for each row r:
 //   bk(r, x) = sum_{i<8} min(R(r-4, x-4+i), R(r-4+i, x+4), R(r+4, x+4-i), R(r+4-i, x-4))
 //              with R the raw image, row/column indices clamped to the batch
 //              (gpuRemoveBackgroundIntensity, SE/LDROIExtraction_ImageFiltering.cu:125-267);
@@ -142,8 +142,12 @@ __device__ __forceinline__ void batch_thresholds(const ExtractConst &c, const un
 //              gpuImageConv_Seperable_H/V, SE/LDROIExtraction_ImageFiltering.cu:286-470);
 //   detection of rows r-3..r once sv(r+1) exists (gpuROIFindingLD,
 //   SE/LDROIExtraction_ROIExtraction.cu:25-164).
-// Cross-lane values (background sums, tv, sv) go through LDS with one barrier per phase; hc stays
-// lane-local.  Four barriers per four rows keep the wave's LDS round trips off the critical path.
+// Only the raw rows go through LDS (one barrier per step, after they are staged).  The other
+// cross-lane values -- the neighbours' background sums, tv of lanes l-2..l+2 for the H pass and
+// sv of lanes l-1, l+1 for detection -- move between lanes by DPP wave shifts, which keeps the
+// step's chain of dependent LDS round trips short; the lanes whose results are used never reach
+// past the wave, and a strip at the image border takes its clamped columns by readlane.  hc stays
+// in a lane-local LDS ring, sv in registers.
 // MODE 0 computes tv only, over the variance region, into tvbuf; MODE 1 is the full pass.
 constexpr int SEGW = 56;             // output columns per wave
 constexpr int RAWW = 72;             // raw row width in LDS
