@@ -261,6 +261,7 @@ int qcs_render_rgb_dev(qcs_ctx *ctx, const float *d_img, const float *d_zimg, in
 #define QCS_CFIT_MAXITER    2
 #define QCS_CFIT_STALLED    4
 #define QCS_CFIT_FLAT       8   /* converged by ftol (set together with QCS_CFIT_CONVERGED) */
+#define QCS_CFIT_PINNED    16   /* ended by pin_stop (see qcs_tuning) */
 /* Fit model K to every ROI.  theta (n x P): N_k photons, x_k, y_k px (pixel edges), widths px,
  * b photons/px in the order above.  info (n x 4, may be NULL): iterations, flags, loss
  * evaluations, 0.  ll (n, may be NULL): FP64 log-likelihood sum(q ln mu - mu) at the fit. */
@@ -278,6 +279,11 @@ typedef struct qcs_tuning {
                              * lowers the negative log-likelihood by less than ftol (1e-3; 0 = off) */
     int   active_set;       /* 1: parameters held at a box bound with an outward gradient are left
                              * out of the quasi-Newton step (default 0) */
+    int   pin_stop;         /* 1: an elliptical three-emitter fit ends (QCS_CFIT_PINNED) after 10
+                             * accepted iterations in a row with a photon count, position or width
+                             * at its box bound, a point the selection rejects (default 0).  Measured
+                             * on 2^18 9x9 astigmatic ROIs: the elliptical path 1.5x faster, count
+                             * accuracy at three true emitters 0.33 instead of 0.36 */
 } qcs_tuning;
 /* ftol, active_set and max_iter apply as set to the 2D model.  Fits of the elliptical model
  * (LocType AS3D: theta = [N, x, y, sx, sy]_k, b) always run with the active set, without the

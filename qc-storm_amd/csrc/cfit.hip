@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
 #pragma unroll
             for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
     }
-    int iters = 0, flags = 0, evals = 1;
+    int iters = 0, flags = 0, evals = 1, pin_run = 0;
     bool done = !valid;
     while (true) {
         if (__all(done)) break;   // wave-uniform
@@ -141,8 +141,13 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
                     th[i] = tn[i];
                     g[i] = gnew[i];
                 }
+                if constexpr (CM<KK>::AS && CM<KK>::K == 3)
+                    if (c.pin_stop) pin_run = cf_at_bound<KK, R>(th) ? pin_run + 1 : 0;
                 if (Lold - L < c.ftol) {   // the likelihood has stopped moving
                     flags |= CF_CONVERGED | CF_FLAT;
+                    done = true;
+                } else if (pin_run >= CF_PIN_RUN) {
+                    flags |= CF_PINNED;
                     done = true;
                 } else if (iters >= c.max_iter) {
                     flags |= CF_MAXITER;
@@ -175,6 +180,7 @@ CFitConst cfit_const(const qcs_params &p, const qcs_tuning &t, int K, int init_m
     c.sigma0 = 1.3f;
     c.ftol = K >= 2 ? t.ftol : 0.0f;   // one-emitter fits always run to the step tolerance
     c.active_set = t.active_set;
+    c.pin_stop = (p.LocType == QCS_LOCTYPE_AS3D && K == 3) ? t.pin_stop : 0;
     if (p.LocType == QCS_LOCTYPE_AS3D) {
         // elliptical fits: bound-pinned parameters always leave the step, no flat-likelihood stop,
         // at least 100 iterations.  With five parameters per emitter the likelihood has long flat
@@ -821,6 +827,7 @@ extern "C" int qcs_tuning_set(qcs_ctx *ctx, const qcs_tuning *t)
     if ((t->queue != 0 && t->queue != 1) || t->waves_per_cu < 0 || t->waves_per_cu > 64)
         return fail(QCS_ERR_ARG, "queue must be 0 or 1 and waves_per_cu in 0..64");
     if (!(t->ftol >= 0.0f) || (t->active_set != 0 && t->active_set != 1)) return fail(QCS_ERR_ARG, "ftol must be >= 0 and active_set 0 or 1");
+    if (t->pin_stop != 0 && t->pin_stop != 1) return fail(QCS_ERR_ARG, "pin_stop must be 0 or 1");
     ctx->tune = *t;
     return QCS_OK;
 }

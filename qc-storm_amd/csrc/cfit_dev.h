@@ -25,9 +25,11 @@ struct CFitConst {
     float sigma0;       // heuristic initial PSF width (px)
     float ftol;         // convergence: an accepted step lowered the loss by less than this (0: off)
     int active_set;     // leave bound-pinned parameters out of the step
+    int pin_stop;       // elliptical three-emitter fits: end after CF_PIN_RUN iterations at a bound
 };
 enum { CF_INIT_HEUR = 0, CF_INIT_GIVEN = 1, CF_INIT_ADD = 2 };
-enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4, CF_FLAT = 8 };
+enum { CF_CONVERGED = 1, CF_MAXITER = 2, CF_STALLED = 4, CF_FLAT = 8, CF_PINNED = 16 };
+constexpr int CF_PIN_RUN = 10;   // accepted iterations in a row at a box bound that end a fit (pin_stop)
 constexpr int CF_ELLIPTIC = 10;   // KK = K + CF_ELLIPTIC selects the elliptical model
 
 // model traits: parameter count and indices
@@ -591,6 +593,24 @@ struct CSelConst {
     int kmax;
     float pG[5], pL[5], zcorr;   // astigmatic z calibration (elliptical model)
 };
+
+// some photon count, position or width of theta at its box bound (cf_project); every box bound
+// lies outside the range cf_valid accepts
+template <int KK, int R>
+__device__ __forceinline__ bool cf_at_bound(const float (&th)[CP<KK>::P])
+{
+    using M = CM<KK>;
+    bool b = false;
+#pragma unroll
+    for (int k = 0; k < M::K; k++) {
+        b = b || th[M::iN(k)] <= 1.0f;
+        b = b || th[M::ix(k)] <= -0.5f || th[M::ix(k)] >= (float)R + 0.5f;
+        b = b || th[M::iy(k)] <= -0.5f || th[M::iy(k)] >= (float)R + 0.5f;
+        b = b || th[M::isx(k)] <= 0.4f || th[M::isx(k)] >= 0.5f * (float)R;
+        b = b || th[M::isy(k)] <= 0.4f || th[M::isy(k)] >= 0.5f * (float)R;
+    }
+    return b;
+}
 
 template <int KK, int R>
 __device__ __forceinline__ bool cf_valid(const float *t, float n_min)

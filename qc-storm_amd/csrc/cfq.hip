@@ -83,7 +83,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
     const int lane = threadIdx.x;
     float *qs = qsh + lane, *D = Dsh + lane;
     const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
-    int st = CQ_IDLE, roi = 0, iters = 0, flags = 0, evals = 0, t = 0;
+    int st = CQ_IDLE, roi = 0, iters = 0, flags = 0, evals = 0, t = 0, pin_run = 0;
     bool first = false, dry = n <= 0;
     float th[P], g[P], du[P], L = 0.0f, slope = 0.0f, alpha = 1.0f, dmax = 0.0f;
 #pragma unroll
@@ -135,6 +135,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         iters = 0;
         flags = 0;
         evals = 1;
+        pin_run = 0;
         st = CQ_DIR;
     };
     // quasi-Newton direction from a fresh gradient; converged fits end here
@@ -174,8 +175,13 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
             }
             const float Lold = L;
             L = Lt;
+            if constexpr (CM<KK>::AS && CM<KK>::K == 3)
+                if (c.pin_stop) pin_run = cf_at_bound<KK, R>(th) ? pin_run + 1 : 0;
             if (Lold - L < c.ftol) {   // the likelihood has stopped moving
                 flags |= CF_CONVERGED | CF_FLAT;
+                finish();
+            } else if (pin_run >= CF_PIN_RUN) {
+                flags |= CF_PINNED;
                 finish();
             } else if (iters >= c.max_iter) {
                 flags |= CF_MAXITER;

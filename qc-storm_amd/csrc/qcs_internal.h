@@ -103,7 +103,7 @@ struct qcs_ctx {
     qcs::DevBuf rn_buf, st_buf;
     qcs::DevBuf dr_img, dr_cc, dr_shift, dr_frames;
     qcs::DevBuf dp_hist;                   // display image histogram   // drift correction: two SR images, FP64 window, per-frame shifts
-    qcs_tuning tune = {{100, 40, 40}, 1e-4f, 0.8f, 1, 0, 1e-3f, 0};
+    qcs_tuning tune = {{100, 40, 40}, 1e-4f, 0.8f, 1, 0, 1e-3f, 0, 0};
     // host staging for the host-pointer API
     qcs::DevBuf h_rois_dev, h_wle_dev, h_loc_dev;
     qcs::HostBuf pin_cnt;

@@ -18,7 +18,7 @@ QCS_ERR_ARG, QCS_ERR_HIP, QCS_ERR_CAPACITY, QCS_ERR_STATE, QCS_ERR_NOMEM, QCS_ER
 LOCTYPE_GS2D, LOCTYPE_AS3D, LOCTYPE_DH3D = 0, 1, 2
 MATH_EXACT, MATH_FAST = 0, 1
 CFIT_INIT_HEUR, CFIT_INIT_GIVEN, CFIT_INIT_ADD = 0, 1, 2
-CFIT_CONVERGED, CFIT_MAXITER, CFIT_STALLED, CFIT_FLAT = 1, 2, 4, 8
+CFIT_CONVERGED, CFIT_MAXITER, CFIT_STALLED, CFIT_FLAT, CFIT_PINNED = 1, 2, 4, 8, 16
 # statistics of the multi-emitter paths (qcs_fit_warm_dev / qcs_fit_plain_stats_dev)
 STAT_KEYS = ("final_k1", "final_k2", "final_k3", "fallbacks", "iters", "evals", "capped", "pred_k2", "pred_k3")
 REC_FLOATS = 12
@@ -72,7 +72,7 @@ class QcsFitStats(C.Structure):
 
 class QcsTuning(C.Structure):
     _fields_ = [("max_iter", C.c_int * 3), ("tol", C.c_float), ("warm_pmin", C.c_float), ("queue", C.c_int),
-                ("waves_per_cu", C.c_int), ("ftol", C.c_float), ("active_set", C.c_int)]
+                ("waves_per_cu", C.c_int), ("ftol", C.c_float), ("active_set", C.c_int), ("pin_stop", C.c_int)]
 
 
 STAT_BINS = 1600
@@ -450,10 +450,10 @@ class Context:
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
         return {"max_iter": list(t.max_iter), "tol": t.tol, "warm_pmin": t.warm_pmin, "queue": t.queue,
-                "waves_per_cu": t.waves_per_cu, "ftol": t.ftol, "active_set": t.active_set}
+                "waves_per_cu": t.waves_per_cu, "ftol": t.ftol, "active_set": t.active_set, "pin_stop": t.pin_stop}
 
     def set_tuning(self, max_iter=None, tol=None, warm_pmin=None, queue=None, waves_per_cu=None, ftol=None,
-                   active_set=None):
+                   active_set=None, pin_stop=None):
         t = QcsTuning()
         check(lib().qcs_tuning_get(self.h, C.byref(t)), "qcs_tuning_get")
         if max_iter is not None:
@@ -471,6 +471,8 @@ class Context:
             t.ftol = ftol
         if active_set is not None:
             t.active_set = int(active_set)
+        if pin_stop is not None:
+            t.pin_stop = int(pin_stop)
         check(lib().qcs_tuning_set(self.h, C.byref(t)), "qcs_tuning_set")
 
     def label_dev(self, d_rois, n, d_labels, d_sel=None, stream=None):

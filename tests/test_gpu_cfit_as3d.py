@@ -263,3 +263,44 @@ def test_elliptical_autotune_tunes_the_kernel_choice_only(gpu):
     c7 = Q.Context(_params(7))
     with pytest.raises(RuntimeError, match="2D model only"):
         c7.autotune_dev(torch.zeros((64, 56), dtype=torch.int16, device="cuda"), 64, net=Q.Net(seed=1))
+
+
+def test_elliptical_pin_stop_option(gpu):
+    """pin_stop (off by default) ends a three-emitter elliptical fit after 10 accepted iterations in
+    a row at a box bound.  The ended fits are flagged and are points the selection rejects; every
+    other fit is the default run's bit for bit; one- and two-emitter fits are untouched; the queue
+    and one-pass kernels still agree bit for bit."""
+    R, n = 9, 577
+    rois = _sim(R, n, 2100, kmax=3, min_sep=1.0, max_sep=4.0, photons_min=800.0, photons_max=6000.0)[0]
+    d_rois = torch.from_numpy(rois.view(np.int16)).cuda()
+    ctx = Q.Context(_params(R))
+
+    def chain():
+        out, prev = [], None
+        for K in (1, 2, 3):
+            th, info, ll = _cfit(ctx, K, d_rois, n, Q.CFIT_INIT_HEUR if K == 1 else Q.CFIT_INIT_ADD, prev)
+            prev = th
+            out.append((th, info, ll))
+        torch.cuda.synchronize()
+        return [(t.cpu().numpy(), i.cpu().numpy(), l.cpu().numpy()) for t, i, l in out]
+
+    assert ctx.tuning()["pin_stop"] == 0
+    base = chain()
+    ctx.set_tuning(pin_stop=1, queue=0)
+    one = chain()
+    ctx.set_tuning(queue=1)
+    got = chain()
+    for (t0, i0, l0), (t1, i1, l1) in zip(one, got):
+        np.testing.assert_array_equal(t1.view(np.uint32), t0.view(np.uint32))
+        np.testing.assert_array_equal(i1[:, :2], i0[:, :2])
+        np.testing.assert_array_equal(l1.view(np.uint64), l0.view(np.uint64))
+    for K in (1, 2):
+        np.testing.assert_array_equal(got[K - 1][0].view(np.uint32), base[K - 1][0].view(np.uint32))
+    t3, i3, _ = got[2]
+    pinned = (i3[:, 1] & Q.CFIT_PINNED) != 0
+    print("pinned", pinned.sum(), "of", n, "iterations", base[2][1][:, 0].mean(), "->", i3[:, 0].mean())
+    assert pinned.sum() > 0 and i3[:, 0].mean() < base[2][1][:, 0].mean()
+    assert not any(_valid(t3[i], 3, R) for i in np.nonzero(pinned)[0])
+    np.testing.assert_array_equal(t3[~pinned].view(np.uint32), base[2][0][~pinned].view(np.uint32))
+    with pytest.raises(RuntimeError, match="pin_stop"):
+        ctx.set_tuning(pin_stop=2)
