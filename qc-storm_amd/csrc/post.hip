@@ -188,7 +188,30 @@ __global__ void __launch_bounds__(256) render_tile_kernel(const float *__restric
         }
     }
     __syncthreads();
-    // the tile is owned by this work-group: plain coalesced read-modify-write
+    // the tile is owned by this work-group: plain coalesced read-modify-write, four pixels per
+    // thread when rows start 16-byte aligned (aligned images, width a multiple of 4) and the
+    // tile lies inside the image
+    const bool aligned = ((reinterpret_cast<uintptr_t>(img) | (zon ? reinterpret_cast<uintptr_t>(zimg) : 0)) & 15) == 0;
+    if (aligned && (c.srw & 3) == 0 && x0 + RTILE <= c.srw && y0 + RTILE <= c.srh) {   // uniform
+        for (int p = 4 * threadIdx.x; p < RTILE * RTILE; p += 4 * blockDim.x) {
+            const size_t g = (size_t)(y0 + p / RTILE) * c.srw + (x0 + p % RTILE);
+            float4 v = *reinterpret_cast<const float4 *>(img + g);
+            v.x += acc[p];
+            v.y += acc[p + 1];
+            v.z += acc[p + 2];
+            v.w += acc[p + 3];
+            *reinterpret_cast<float4 *>(img + g) = v;
+            if (zon) {
+                float4 u = *reinterpret_cast<const float4 *>(zimg + g);
+                u.x += zacc[p];
+                u.y += zacc[p + 1];
+                u.z += zacc[p + 2];
+                u.w += zacc[p + 3];
+                *reinterpret_cast<float4 *>(zimg + g) = u;
+            }
+        }
+        return;
+    }
     for (int p = threadIdx.x; p < RTILE * RTILE; p += blockDim.x) {
         const int gy = y0 + p / RTILE, gx = x0 + p % RTILE;
         if (gx >= c.srw || gy >= c.srh) continue;
