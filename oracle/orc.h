@@ -98,6 +98,13 @@ int   orc_fit_params_f64(const qcs_params *p, int model, const uint16_t *rois, c
                          int n, double *theta, int iters);
 /* loss value of `model` at theta for ROI 0 (scaled units); FP32 and FP64 */
 float orc_loss(const qcs_params *p, int model, const uint16_t *roi, const float *wle, const float *theta);
+/* loss evaluations made by the calling thread since the last reset */
+long  orc_loss_evals(int reset);
+/* acceptance of a one-emitter fit vector x (returns 1 and writes the row if accepted) and the
+ * emitter selection of a multi-emitter fit vector (returns 0, 1 or 2 rows written) */
+int   orc_select_1e(const qcs_params *p, int model, const float *x, const uint16_t *roi, float *row);
+int   orc_select_me(const qcs_params *p, int model, const float *x, float nn, const uint16_t *roi, float *row,
+                    float *row2);
 
 /* ---------------- CRLB (DLL/bfgsMLE_Loc_top/LocPrecCalc_GaussianCRLB.h) ---------------- */
 void  orc_crlb(const qcs_params *p, float *loc, int n);

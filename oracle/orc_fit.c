@@ -11,6 +11,16 @@ float orc_exp32(float x);
 float orc_log32(float x);
 int   orc_model_separable(void);
 
+/* loss evaluations of the calling thread (the schedule's cost; read by the tests) */
+static _Thread_local long g_loss_evals;
+static void orc_count_loss_eval(void) { g_loss_evals++; }
+long orc_loss_evals(int reset)
+{
+    long v = g_loss_evals;
+    if (reset) g_loss_evals = 0;
+    return v;
+}
+
 /* ---------------- FP32 ---------------- */
 #define REAL        float
 #define SUF(n)      n##_f32
@@ -134,4 +144,17 @@ float orc_loss(const qcs_params *p, int model, const uint16_t *roi, const float 
 }
 
 void orc_crlb(const qcs_params *p, float *loc, int n) { crlb_rows_f32(p, loc, n); }
+
+/* Acceptance rules on a given fit vector (scaled units, as the fitters hold it). */
+int orc_select_1e(const qcs_params *p, int model, const float *x, const uint16_t *roi, float *row)
+{
+    float snr;
+    int valid;
+    return select_1e_f32(p, model, x, roi, row, &snr, &valid);
+}
+
+int orc_select_me(const qcs_params *p, int model, const float *x, float nn, const uint16_t *roi, float *row, float *row2)
+{
+    return select_me_f32(p, model, x, nn, roi, row, row2);
+}
 void orc_crlb_f64(const qcs_params *p, double *loc, int n) { crlb_rows_f64(p, loc, n); }

@@ -79,6 +79,7 @@ static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
 {
     const int R = rc->R;
     int i, r, c, k;
+    orc_count_loss_eval();
     for (i = 0; i < m->P; i++)
         if (p[i] < m->clampv) p[i] = m->clampv;
 
@@ -478,16 +479,27 @@ static void SUF(roi_header)(const uint16_t *roi, int R, int *xo, int *yo, int *f
  * Writes the row; returns 1 if accepted, 0 if rejected; *snr_out receives the SNR used
  * for the multi-emitter re-route, *valid_out the NaN-background test.
  */
+static int SUF(select_1e)(const qcs_params *p, int model, const REAL *x, const uint16_t *roi, REAL *row,
+                          REAL *snr_out, int *valid_out);
+
 static int SUF(fit_1e)(const qcs_params *p, int model, const uint16_t *roi, const float *wle,
                        REAL *row, REAL *snr_out, int *valid_out)
 {
-    const int R = p->ROISize, half = R / 2;
     SUF(roictx) rc;
     REAL x[MAXP];
     SUF(mdesc) m;
     SUF(get_desc)(model, &m);
     SUF(setup)(p, model, roi, wle, &rc, x, NULL);
     SUF(bfgs_run)(&m, &rc, x, m.iters);
+    return SUF(select_1e)(p, model, x, roi, row, snr_out, valid_out);
+}
+
+/* Validity test and row of a one-emitter fit vector x (bfgsMLELoc_Gauss2D :230-309,
+ * bfgsMLELoc_AS3D :230-336); the ROI supplies the header only. */
+static int SUF(select_1e)(const qcs_params *p, int model, const REAL *x, const uint16_t *roi, REAL *row,
+                          REAL *snr_out, int *valid_out)
+{
+    const int R = p->ROISize, half = R / 2;
     int xo, yo, fr;
     SUF(roi_header)(roi, R, &xo, &yo, &fr);
     const REAL QE = (REAL)p->QE;
@@ -562,18 +574,31 @@ static int SUF(fit_1e)(const qcs_params *p, int model, const uint16_t *roi, cons
  * AS3D_2E :186-419).  Writes the first emitter into `row` and, when accepted, the second into
  * `row2` (returns 2); returns 1 if only the first is written, 0 if rejected (row zeroed).
  */
+static int SUF(select_me)(const qcs_params *p, int model, const REAL *x, REAL nn, const uint16_t *roi, REAL *row,
+                          REAL *row2);
+
 static int SUF(fit_me)(const qcs_params *p, int model, const uint16_t *roi, const float *wle,
                        REAL *row, REAL *row2)
 {
-    const int R = p->ROISize, half = R / 2;
-    const REAL ctr = (REAL)R / C(2.0);
     SUF(roictx) rc;
     REAL x[MAXP], nn;
     SUF(mdesc) m;
-    int xo, yo, fr, k;
     SUF(get_desc)(model, &m);
     SUF(setup)(p, model, roi, wle, &rc, x, &nn);
     SUF(bfgs_run)(&m, &rc, x, m.iters);
+    return SUF(select_me)(p, model, x, nn, roi, row, row2);
+}
+
+/* Emitter selection of a multi-emitter fit vector x with the nearest-neighbour distance nn the
+ * model uses (WLE NearDist for the 2-emitter 2D fit, 1000 otherwise). */
+static int SUF(select_me)(const qcs_params *p, int model, const REAL *x, REAL nn, const uint16_t *roi, REAL *row,
+                          REAL *row2)
+{
+    const int R = p->ROISize, half = R / 2;
+    const REAL ctr = (REAL)R / C(2.0);
+    SUF(mdesc) m;
+    int xo, yo, fr, k;
+    SUF(get_desc)(model, &m);
     SUF(roi_header)(roi, R, &xo, &yo, &fr);
     const REAL QE = (REAL)p->QE;
     const REAL sqe = SQRTR(QE);

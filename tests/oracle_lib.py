@@ -56,6 +56,9 @@ PROTOS = {
     "orc_fit_params": (_i, [_PP, _i, _vp, _vp, _i, _vp, _i]),
     "orc_fit_params_f64": (_i, [_PP, _i, _vp, _vp, _i, _vp, _i]),
     "orc_loss": (_f, [_PP, _i, _vp, _vp, _vp]),
+    "orc_loss_evals": (C.c_long, [_i]),
+    "orc_select_1e": (_i, [_PP, _i, _vp, _vp, _vp]),
+    "orc_select_me": (_i, [_PP, _i, _vp, _f, _vp, _vp, _vp]),
     "orc_crlb": (None, [_PP, _vp, _i]),
     "orc_crlb_f64": (None, [_PP, _vp, _i]),
     "orc_compact_sorted": (_i, [_vp, _i, _i, _i, _vp]),
@@ -166,6 +169,31 @@ def fit_params(params, model, rois, wle_rec, iters=0, f64=False):
     fn(C.byref(params), model, p(np.ascontiguousarray(rois)), p(np.ascontiguousarray(wle_rec)) if wle_rec is not None
        else None, n, p(th), iters)
     return th
+
+
+def loss_evals(reset=True):
+    return lib().orc_loss_evals(int(reset))
+
+
+def select_1e(params, model, x, roi=None):
+    """(accepted, row) of the one-emitter acceptance rule on the fit vector x."""
+    R = params.ROISize
+    roi = np.zeros(Q.roi_words(R), np.uint16) if roi is None else np.ascontiguousarray(roi, np.uint16)
+    x = np.ascontiguousarray(x, np.float32)
+    row = np.full(12, -9.0, np.float32)
+    ok = lib().orc_select_1e(C.byref(params), model, p(x), p(roi), p(row))
+    return ok, row
+
+
+def select_me(params, model, x, nn=1000.0, roi=None):
+    """(rows written 0/1/2, first row, second row) of the multi-emitter selection on x."""
+    R = params.ROISize
+    roi = np.zeros(Q.roi_words(R), np.uint16) if roi is None else np.ascontiguousarray(roi, np.uint16)
+    x = np.ascontiguousarray(x, np.float32)
+    row = np.full(12, -9.0, np.float32)
+    row2 = np.full(12, -9.0, np.float32)
+    k = lib().orc_select_me(C.byref(params), model, p(x), nn, p(roi), p(row), p(row2))
+    return k, row, row2
 
 
 def extract(params, frames, start_frame=1, cap=200000):
