@@ -30,6 +30,15 @@ the simulated truth on the first 65,536 ROIs.  A third (`as3d_multi_emitter`) fi
 9x9 ROIs with 1-3 emitters with the elliptical model (per-emitter widths, up to 16 parameters) and
 reports fits/s, the count accuracy, RMSE over CRLB and the z error against the simulated truth.
 
+`hd_frames`: 1000 high-density 1024 x 1024 frames (2-5 emitters / um^2 with planted pairs and
+triples) through the frames -> rows path of the converged fitter (mixed 7/9/11 ROIs, persistent
+work-queue fits, FP64 selection and CRLB), one HIP graph replay per 50-frame batch; frames/s,
+ROIs/s, accuracy of the first batch against the simulated emitters, and the learned warm start
+against the plain path on the 7 x 7 ROIs those frames yield (median of 5 timed calls each).
+`rois_100k`: the 100k seeded 7 x 7 ROIs of the single-emitter reference case, the oracle on every
+usable host core against the library, rows compared bit for bit.  `session_path`: the plug-ins'
+drop-in path, host frames through the online session with the result file, H2D included.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--math exact|fast] [--no-cpu-baseline]
 """
 import argparse
@@ -68,11 +77,30 @@ def traffic_per_fit():
     return None
 
 
+def host_cpu():
+    """(usable host cores, nproc, CPU model) of the machine running the benchmark."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return usable, os.cpu_count() or 1, model
+
+
 def cpu_baseline(params, rois, wle, seconds):
-    """Oracle fit + CRLB (OpenMP over ROIs) on a bounded sample; returns (dict, sample rows)."""
+    """Oracle fit + CRLB (OpenMP over ROIs, every usable host core) on a bounded sample; returns
+    (dict, sample rows)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O  # the oracle is used here only as the CPU baseline and checker
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, nproc, model = host_cpu()
     pilot = min(rois.shape[0], 4096 * threads)
     t0 = time.perf_counter()
     O.fit_omp(params, rois[:pilot], wle[:pilot], threads)
@@ -81,9 +109,53 @@ def cpu_baseline(params, rois, wle, seconds):
     t0 = time.perf_counter()
     loc = O.fit_omp(params, rois[:n], wle[:n], threads)
     dt = time.perf_counter() - t0
-    return ({"value": n / dt, "unit": "fits/s", "cores": threads, "kind": "port",
+    return ({"value": n / dt, "unit": "fits/s", "cores": threads, "kind": "port", "nproc": nproc, "cpu_model": model,
+             "per_core_fits_per_s": n / dt / threads,
              "sample": "fit+CRLB of the first %d extracted 7x7 ROIs of the workload (%.1f s, %d OpenMP threads)"
                        % (n, dt, threads)}, loc)
+
+
+def c0_leg(log, n=100000, seed=1000):
+    """The 100k-ROI single-emitter case on the reference CPU path: n seeded 7 x 7 ROIs (one
+    Gaussian each, centre within +-1 px, sigma U(1.0, 1.6), photons log-U(500, 5000), background
+    U(10, 100), Poisson noise, ADU = e/0.45 + 100), WLE then fit + CRLB: the oracle on every usable
+    host core against the library (host-pointer call, PCIe copies included, and the device call
+    on resident ROIs), with the rows compared bit for bit."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O   # CPU baseline and checker only
+    sp = Q.default_sim()
+    rois, _ = Q.sim_rois_host(sp, seed, n)
+    p = Q.default_params(QE=1.0)
+    wle = O.wle(rois, 7)
+    threads, nproc, model = host_cpu()
+    O.fit_omp(p, rois[:4096], wle[:4096], threads)
+    t0 = time.perf_counter()
+    ref = O.fit_omp(p, rois, wle, threads)
+    cpu_s = time.perf_counter() - t0
+    ctx = Q.Context(p)
+    got = ctx.fit_rois(rois, wle)     # warm-up, also the parity rows
+    t0 = time.perf_counter()
+    got = ctx.fit_rois(rois, wle)
+    host_s = time.perf_counter() - t0
+    d_rois = torch.from_numpy(rois.view(np.int16)).cuda()
+    d_wle = torch.from_numpy(wle).cuda()
+    d_loc = torch.empty((n, 12), dtype=torch.float32, device="cuda")
+    ctx.fit_dev(d_rois, d_wle, n, d_loc)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        ctx.fit_dev(d_rois, d_wle, n, d_loc)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ctx.close()
+    same = int(np.all(got.view(np.uint32) == ref.view(np.uint32), axis=1).sum())
+    log("bench: 100k-ROI case: CPU %.3g fits/s (%d threads), GPU %.3g fits/s" % (n / cpu_s, threads, n / np.median(ts)))
+    return {"rois": n, "seed": seed, "cpu": {"value": n / cpu_s, "unit": "fits/s", "cores": threads, "nproc": nproc,
+                                            "cpu_model": model, "kind": "port", "seconds": cpu_s},
+            "gpu_host_pointer_fits_per_s": n / host_s, "gpu_device_fits_per_s_median": n / float(np.median(ts)),
+            "rows_bit_exact": same, "rows": n, "accepted": int((ref[:, 0] > 0).sum())}
 
 
 def drift_workload(W, H, nframes, npts, seed, ax=0.005, ay=-0.003, wobble=0.1):
@@ -208,10 +280,21 @@ def multi_emitter_leg(n, seed, log):
     loc_w = torch.zeros((3 * n, 12), dtype=torch.float32, device="cuda")
     ctx.fit_mle_dev(rois, n, 3, loc_p)
     st = ctx.fit_warm_dev(net, rois, n, loc_w)
+    stp = ctx.fit_plain_stats_dev(rois, n, 3, loc_p)
     torch.cuda.synchronize()
-    ctx.timing_enable(True)
-    ctx.fit_mle_dev(rois, n, 3, loc_p)
-    tp = ctx.timing()
+
+    def timed(fn, reps=5):
+        """median over reps calls of the library's HIP-event time (fits + network)"""
+        ts = []
+        for _ in range(reps):
+            ctx.timing_enable(True)
+            fn()
+            t = ctx.timing()
+            ts.append(t["cfit_ms"] + t["nn_ms"])
+        return float(np.median(ts)), ts
+
+    plain_ms, plain_all = timed(lambda: ctx.fit_mle_dev(rois, n, 3, loc_p))
+    warm_ms, warm_all = timed(lambda: ctx.fit_warm_dev(net, rois, n, loc_w, stats=False))
     ctx.timing_enable(True)
     ctx.fit_warm_dev(net, rois, n, loc_w, stats=False)
     tw = ctx.timing()
@@ -220,11 +303,16 @@ def multi_emitter_leg(n, seed, log):
     th = truth[:m].cpu().numpy()
     rp = match_rmse(loc_p[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize)
     rw = match_rmse(loc_w[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize)
-    plain_ms, warm_ms = tp["cfit_ms"], tw["cfit_ms"] + tw["nn_ms"]
     ctx.close()
     net.close()
+    # evaluations: device-counted loss-and-gradient evaluations of the plain path (pixel evaluations
+    # = evaluations x 49); about 45 FLOP per pixel per evaluation (loss and 3K + 2 gradient terms)
+    evals = stp["evals"]
     return {"rois": n, "plain_fits_per_s": n / (plain_ms * 1e-3), "warm_fits_per_s": n / (warm_ms * 1e-3),
             "warm_speedup": plain_ms / warm_ms, "nn_ms": tw["nn_ms"], "plain_ms": plain_ms, "warm_ms": warm_ms,
+            "timed_calls": len(plain_all), "plain_ms_all": plain_all, "warm_ms_all": warm_all,
+            "plain_evaluations": int(evals), "plain_pixel_evaluations_per_s": evals * 49 / (plain_ms * 1e-3),
+            "plain_iterations_mean": stp["iters"] / n,
             "plain": rp, "warm": rw, "rmse_ratio_warm_vs_plain": rw["rmse_px"] / rp["rmse_px"],
             "rmse_over_crlb_plain": rp["rmse_over_crlb"], "rmse_over_crlb_warm": rw["rmse_over_crlb"],
             "rmse_over_crlb_count_correct_plain": rp["rmse_over_crlb_count_correct"],
@@ -253,9 +341,12 @@ def as3d_multi_emitter_leg(n, seed, log):
     sel = torch.zeros(n, dtype=torch.int32, device="cuda")
     ctx.fit_mle_dev(rois, n, kmax, loc, sel)
     torch.cuda.synchronize()
-    ctx.timing_enable(True)
-    ctx.fit_mle_dev(rois, n, kmax, loc, sel)
-    t = ctx.timing()
+    ts = []
+    for _ in range(5):
+        ctx.timing_enable(True)
+        ctx.fit_mle_dev(rois, n, kmax, loc, sel)
+        ts.append(ctx.timing()["cfit_ms"])
+    t = {"cfit_ms": float(np.median(ts))}
     st = ctx.fit_plain_stats_dev(rois, n, kmax, loc, sel)
     torch.cuda.synchronize()
     m = min(n, 65536)
@@ -263,11 +354,202 @@ def as3d_multi_emitter_leg(n, seed, log):
     acc = match_rmse(loc[:3 * m].cpu().numpy().reshape(m, 3, 12), th, p.PixelSize, R=R, zerr=True)
     s_h, k_true = sel[:m].cpu().numpy(), (th[:, :, 4] > 0).sum(1)
     ctx.close()
-    return {"rois": n, "roi": R, "fits_per_s": n / (t["cfit_ms"] * 1e-3), "ms": t["cfit_ms"],
+    return {"rois": n, "roi": R, "fits_per_s": n / (t["cfit_ms"] * 1e-3), "ms": t["cfit_ms"], "ms_all": ts,
             "iterations_mean": st["iters"] / n, "final_counts": [st["final_k1"], st["final_k2"], st["final_k3"]],
             "count_accuracy_by_true_count": {str(k): float((s_h[k_true == k] == k).mean()) for k in (1, 2, 3)},
             "accuracy": acc, "accuracy_sample_rois": m,
             "data": "simulator: photons log-U(1000, 8000), background U(20, 80), z U(-500, 500) nm"}
+
+
+def hd_frames_leg(nframes, batch, seed, log, warm_rois=1 << 20, reps=5):
+    """High-density frames to localizations through the converged multi-emitter path.
+
+    Workload: nframes uint16 frames of 1024 x 1024 px, simulated on the GPU (seed) at 2-5
+    emitters / um^2 (0.02-0.05 per 100 nm px, per-frame density uniform in that range), emitters
+    in sites of 1-3 members 1-2.5 px apart (pairs and triples closer than 2 sigma), photons
+    log-U(500, 5000), background U(10, 100).  Each batch of `batch` frames is one HIP graph
+    replay of qcs_loc_mle_frames_dev: mixed-size extraction (7/9/11), converged 1-3 emitter fits
+    with FP64 likelihood-ratio selection and FP64 CRLB in persistent work-queue kernels, rows in
+    frame order.  The frames are resident in HBM; each replay takes its batch by a device copy
+    into the graph's input buffer (part of the timed work).
+
+    The warm-start comparison (the learned network ahead of the 2D fits) runs on the 7 x 7 ROIs
+    the same extraction cuts from these frames (up to warm_rois of them), plain vs warm, each
+    timed as the median of `reps` calls, with accuracy against the simulated emitters."""
+    import torch
+    from scipy.spatial import cKDTree
+    from qcstorm import warmstart as WS
+    W = H = 1024
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.02, density_max_per_px=0.05, n_per_unit_max=3, min_sep=1.0,
+                       max_sep=2.5, qe=1.0)
+    p = Q.default_params(QE=1.0, ImageWidth=W, ImageHigh=H, ROISize=7, MultiEmitterFitEn=1, PixelSize=100.0,
+                         BatchedImageNum=1)
+    frames = torch.empty((nframes, H, W), dtype=torch.int16, device="cuda")
+    for f0 in range(0, nframes, 25):
+        nf = min(25, nframes - f0)
+        Q.check(Q.lib().qcs_sim_frames_dev(C.byref(sp), seed, 1 + f0, nf, C.c_void_p(frames[f0].data_ptr()), None),
+                "qcs_sim_frames_dev")
+    torch.cuda.synchronize()
+    ctx = Q.Context(p)
+    cap = 40000 * batch
+    staging = frames[:batch].clone()
+    d_out = torch.empty((3 * cap, 12), dtype=torch.float32, device="cuda")
+    d_n = torch.zeros(4, dtype=torch.int32, device="cuda")
+    d_start = torch.ones(1, dtype=torch.int32, device="cuda")
+    g = ctx.graph_frames(staging, batch, d_start, cap, 3, d_out, d_n)
+    nb = nframes // batch
+    counts = torch.zeros((nb, 4), dtype=torch.int32, device="cuda")
+    first_rows = None
+
+    def run_all():
+        for b in range(nb):
+            staging.copy_(frames[b * batch:(b + 1) * batch])
+            d_start.fill_(1 + b * batch)
+            g.launch()
+            counts[b].copy_(d_n)
+
+    run_all()   # warm-up pass
+    torch.cuda.synchronize()
+    first_n = int(counts[0, 0].item())
+    staging.copy_(frames[:batch])
+    d_start.fill_(1)
+    g.launch()
+    torch.cuda.synchronize()
+    first_rows = d_out[:int(d_n[0].item())].cpu().numpy()
+    t0 = time.perf_counter()
+    run_all()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cnt = counts.cpu().numpy().astype(np.int64)
+    rows_tot, rois_tot, cand_tot = int(cnt[:, 1].sum()), int(cnt[:, 2].sum()), int(cnt[:, 3].sum())
+    overflow = bool((cnt[:, 1] > cnt[:, 0]).any() or (cnt[:, 3] > cnt[:, 2]).any())
+    # accuracy of the first batch against the simulated emitters (matched within 0.5 px)
+    errs, crb, nrows, ntrue, found = [], [], 0, 0, 0
+    for f in range(1, batch + 1):
+        tr = Q.frame_emitters(sp, seed, f)
+        r = first_rows[first_rows[:, 11] == f]
+        d, i = cKDTree(tr[:, 1:3]).query(r[:, 1:3]) if len(r) else (np.zeros(0), np.zeros(0, int))
+        ok = d < 0.5
+        errs.append(d[ok])
+        crb.append(np.hypot(r[ok, 9], r[ok, 10]) / p.PixelSize)
+        nrows += len(r)
+        ntrue += len(tr)
+        found += len(np.unique(i[ok]))
+    e, c = np.concatenate(errs), np.concatenate(crb)
+    fin = np.isfinite(c) & (c > 0) & (c < 0.5)
+    g.close()
+    res = {"frames": nb * batch, "frame": [W, H], "batch_frames": batch, "graph_per_batch": True,
+           "ms": dt * 1e3, "frames_per_s": nb * batch / dt, "rois_per_s": rois_tot / dt, "localizations_per_s": rows_tot / dt,
+           "rois": rois_tot, "candidates": cand_tot, "localizations": rows_tot, "capacity_overflow": overflow,
+           "first_batch_accuracy": {"emitters": ntrue, "rows": nrows, "recall_0p5px": found / max(ntrue, 1),
+                                    "precision_0p5px": float(len(e)) / max(nrows, 1),
+                                    "median_error_px": float(np.median(e)) if len(e) else None,
+                                    "rmse_px": float(np.sqrt(np.mean(e ** 2))) if len(e) else None,
+                                    "rmse_over_crlb": float(np.sqrt(np.mean(e[fin] ** 2) / np.mean(c[fin] ** 2))) if fin.any() else None},
+           "data": "simulator seed %d: 2-5 emitters/um^2, sites of 1-3 emitters 1-2.5 px apart, photons log-U(500, 5000), "
+                   "background U(10, 100)" % seed}
+    log("bench: high-density frames %.0f frames/s, %.3g ROIs/s" % (res["frames_per_s"], res["rois_per_s"]))
+
+    # warm start on the 7 x 7 ROIs of these frames: plain vs warm, median of reps timed calls each
+    if warm_rois > 0 and os.path.exists(WS.DEFAULT_WEIGHTS):
+        net = WS.load()
+        sets7, frames_of = [], []
+        got = 0
+        for b in range(nb):
+            sets = ctx.extract_mixed_dev(frames[b * batch:(b + 1) * batch], batch, 1 + b * batch, cap, (cap, 16, 16))
+            R, d_rois, n, _ = sets[0]
+            sets7.append(d_rois[:n])
+            got += n
+            if got >= warm_rois:
+                break
+        rois7 = torch.cat(sets7)[:warm_rois].contiguous()
+        n7 = rois7.shape[0]
+        ctx7 = Q.Context(Q.default_params(QE=1.0, ROISize=7, PixelSize=100.0))
+        loc_p = torch.zeros((3 * n7, 12), dtype=torch.float32, device="cuda")
+        loc_w = torch.zeros_like(loc_p)
+        ctx7.fit_mle_dev(rois7, n7, 3, loc_p)
+        stw = ctx7.fit_warm_dev(net, rois7, n7, loc_w)
+        stp = ctx7.fit_plain_stats_dev(rois7, n7, 3, loc_p)
+        torch.cuda.synchronize()
+
+        def med(fn):
+            ts = []
+            for _ in range(reps):
+                ctx7.timing_enable(True)
+                fn()
+                t = ctx7.timing()
+                ts.append(t["cfit_ms"] + t["nn_ms"])
+            return float(np.median(ts)), [float(x) for x in ts]
+
+        plain_ms, plain_all = med(lambda: ctx7.fit_mle_dev(rois7, n7, 3, loc_p))
+        warm_ms, warm_all = med(lambda: ctx7.fit_warm_dev(net, rois7, n7, loc_w, stats=False))
+        torch.cuda.synchronize()
+        # accuracy: rows of the ROIs from the first frames against the simulated emitters
+        hdr = rois7[:, 49:53].cpu().numpy().view(np.uint16).astype(np.int64)
+        fr = hdr[:, 2] + 65536 * hdr[:, 3]
+        acc = {}
+        for name, loc in (("plain", loc_p), ("warm", loc_w)):
+            L = loc.cpu().numpy().reshape(n7, 3, 12)
+            e_all, c_all, nr = [], [], 0
+            for f in np.unique(fr)[:batch]:
+                tr = Q.frame_emitters(sp, seed, int(f))
+                r = L[fr == f].reshape(-1, 12)
+                r = r[r[:, 0] > 0]
+                d, _ = cKDTree(tr[:, 1:3]).query(r[:, 1:3])
+                ok = d < 0.5
+                e_all.append(d[ok])
+                c_all.append(np.hypot(r[ok, 9], r[ok, 10]) / p.PixelSize)
+                nr += len(r)
+            e2, c2 = np.concatenate(e_all), np.concatenate(c_all)
+            fin2 = np.isfinite(c2) & (c2 > 0) & (c2 < 0.5)
+            acc[name] = {"rows": nr, "matched_0p5px": int(len(e2)), "rmse_px": float(np.sqrt(np.mean(e2 ** 2))),
+                         "rmse_over_crlb": float(np.sqrt(np.mean(e2[fin2] ** 2) / np.mean(c2[fin2] ** 2)))}
+        res["warm_start_7x7"] = {
+            "rois": n7, "plain_ms_median": plain_ms, "warm_ms_median": warm_ms, "plain_ms_all": plain_all,
+            "warm_ms_all": warm_all, "plain_rois_per_s": n7 / (plain_ms * 1e-3), "warm_rois_per_s": n7 / (warm_ms * 1e-3),
+            "warm_speedup": plain_ms / warm_ms, "plain_iterations_mean": stp["iters"] / n7,
+            "warm_iterations_mean": stw["iters"] / n7, "fallback_fraction": stw["fallbacks"] / n7,
+            "accuracy_first_frames": acc,
+            "rmse_ratio_warm_vs_plain": acc["warm"]["rmse_px"] / acc["plain"]["rmse_px"]}
+        ctx7.close()
+        net.close()
+    ctx.close()
+    return res
+
+
+def session_leg(frames_dev, params, F, log, group=1):
+    """The drop-in session path the plug-ins use (lm_StartLocThread .. lm_StopLocThread): F host
+    frames fed through qcs_session_feed in groups of `group` (each copied into pinned memory and
+    sent to the device by the session's worker: extraction, fit + CRLB, frame-sorted compaction,
+    rendering), rows streamed to a result file by the writer thread; timed from the first feed
+    until the session reports finished (file closed).  The frames reach host memory before the
+    timed region (a device-to-host copy of the simulated frames)."""
+    import tempfile
+    h = frames_dev[:F].cpu().numpy().view(np.uint16)
+    p = Q.QcsParams.from_buffer_copy(params)
+    p.BatchedImageNum = group
+    out_dir = tempfile.mkdtemp(prefix="qcs_session_")
+    path = os.path.join(out_dir, "bench_result.txt")
+    s = Q.Session(p)
+    s.set_output(path)
+    t0 = time.perf_counter()
+    for f0 in range(0, F, group):
+        s.feed(h[f0:f0 + group])
+    s.finish()
+    s.wait(timeout_s=600.0)
+    dt = time.perf_counter() - t0
+    rows = s.locs().shape[0]
+    s.close()
+    size = os.path.getsize(path) if os.path.exists(path) else -1
+    try:
+        os.remove(path)
+        os.rmdir(out_dir)
+    except OSError:
+        pass
+    log("bench: session path %.0f frames/s (host frames, H2D and result file included)" % (F / dt))
+    return {"frames": F, "feed_group_frames": group, "ms": dt * 1e3, "frames_per_s": F / dt, "rows": int(rows),
+            "rows_per_s": rows / dt, "result_file_bytes": int(size), "result_file_complete": size == rows * 48,
+            "host_bytes_per_s": F * h.shape[1] * h.shape[2] * 2 / dt}
 
 
 class Pipeline:
@@ -351,6 +633,11 @@ def main():
                     help="ROIs of the multi-emitter plain vs warm-start leg (0 skips it)")
     ap.add_argument("--as3d-rois", type=int, default=1 << 18,
                     help="ROIs of the elliptical-model multi-emitter leg (0 skips it)")
+    ap.add_argument("--hd-frames", type=int, default=1000,
+                    help="1024x1024 high-density frames of the converged frames leg (0 skips it)")
+    ap.add_argument("--hd-batch", type=int, default=50, help="frames per HIP-graph batch of the high-density leg")
+    ap.add_argument("--session-frames", type=int, default=200,
+                    help="host frames of the drop-in session-path leg (0 skips it)")
     ap.add_argument("--no-fast-leg", action="store_true", help="skip the fast-math pipeline leg")
     ap.add_argument("--lanes", type=int, default=1,
                     help="frame groups processed concurrently (own context, stream and host thread each)")
@@ -562,6 +849,21 @@ def main():
         log("bench: elliptical-model multi-emitter leg (%d 9x9 ROIs)" % args.as3d_rois)
         as3d = as3d_multi_emitter_leg(args.as3d_rois, 1004, log)
 
+    sess = None
+    if args.session_frames > 0:
+        log("bench: drop-in session path (%d host frames)" % args.session_frames)
+        sess = session_leg(frames, params, min(args.session_frames, F), log)
+
+    c0 = None
+    if not args.no_cpu_baseline:
+        log("bench: 100k-ROI single-emitter case (oracle on the host cores vs the library)")
+        c0 = c0_leg(log)
+
+    hd = None
+    if args.hd_frames > 0:
+        log("bench: high-density frames leg (%d frames 1024x1024, graphs of %d frames)" % (args.hd_frames, args.hd_batch))
+        hd = hd_frames_leg(args.hd_frames, args.hd_batch, 1003, log)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -639,7 +941,9 @@ def main():
                        "frames": F, "frame": [W, H], "roi": R, "fit_math": args.math,
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
-            "fast_math": fast, "multi_emitter": multi, "as3d_multi_emitter": as3d, "statistics": statistics,
+            "fast_math": fast, "multi_emitter": multi, "as3d_multi_emitter": as3d, "hd_frames": hd, "rois_100k": c0,
+            "session_path": sess,
+            "statistics": statistics,
             "consecutive_merge": consec,
             "drift_correction": drift}
     cons.close()

@@ -103,7 +103,7 @@ typedef struct qcs_params {
     float QE;                /* e-/photon */
     float ReadNoise_e;       /* e- */
     /* localization */
-    int   ROISize;           /* odd, 5..17 */
+    int   ROISize;           /* odd, 5..17 (5..13 with MultiEmitterFitEn, as the reference's kernels) */
     int   LocType;           /* QCS_LOCTYPE_* */
     int   MultiEmitterFitEn;
     int   WLEEn;
@@ -145,6 +145,12 @@ void qcs_params_update_sr_size(qcs_params *p);
 const char *qcs_last_error(void);
 /* Library version string. */
 const char *qcs_version(void);
+
+/* Test hook (no reference counterpart): the n-th device or pinned-host allocation the library
+ * makes from now on fails as if the memory were exhausted (QCS_ERR_NOMEM); n < 0 disarms it.
+ * Lets tests check that every entry point reports an allocation failure and leaves its context
+ * usable. */
+void qcs_test_fail_alloc_after(int n);
 
 /* Number of HIP devices; <0 on runtime error.  Replaces CudaDeviceNum (REL/bfgs_MLE_dll.h:806-821). */
 int qcs_device_count(void);
@@ -304,15 +310,47 @@ int qcs_fit_plain_stats_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int kma
 /* One set of same-size ROIs in a mixed-size call. */
 typedef struct qcs_roi_set {
     int roi_size;             /* 7, 9 or 11 */
-    int n;                    /* ROIs in the set */
-    const uint16_t *d_rois;   /* n ROI records of roi_size */
+    int n;                    /* ROIs in the set; with d_n: the set's capacity */
+    uint16_t *d_rois;         /* n ROI records of roi_size (written by qcs_extract_mixed_dev) */
     float *d_loc;             /* 3n rows, rows 3j..3j+2 belong to ROI j (as qcs_fit_mle_dev) */
     int *d_sel;               /* optional: chosen emitter count per ROI */
+    int *d_n;                 /* optional: device int holding the set's ROI count (<= n); written by
+                               * qcs_extract_mixed_dev, read by qcs_fit_mle_mixed_dev, so a set flows
+                               * from extraction into fitting with no host round trip */
 } qcs_roi_set;
 /* qcs_fit_mle_dev over up to three sets of different ROI sizes: one persistent work-queue launch
  * per emitter count covers every set (largest ROIs first); the context's ROISize is not used.
- * Results equal per-set qcs_fit_mle_dev calls. */
+ * Results equal per-set qcs_fit_mle_dev calls.  A set with d_n fits its first *d_n ROIs (grids
+ * sized for n); nothing is waited for. */
 int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int nsets, int kmax, void *stream);
+
+/* Mixed-size ROI extraction for the converged fitter (no counterpart in the reference, which
+ * cuts one ROI size per run, LDROIExtraction_ROIExtraction.cu:257-302).  Detection,
+ * classification and WLE as qcs_extract_dev at ROI size 7 (the context's other parameters),
+ * at most `cap` candidates per call in raster order; each candidate then gets a size in
+ * {7, 9, 11}:
+ *   s = min(SigmaL, SigmaR, SigmaU, SigmaD) of its 7 x 7 WLE record; Rpsf = 7 (s <= 1.4),
+ *   9 (s <= 1.9) or 11; with dmin the Chebyshev distance (1..5, same frame) of the nearest other
+ *   detector mark, Rnb = 2 dmin + 3 for dmin 3 or 4, else 7; R = max(Rpsf, Rnb), reduced by 2
+ *   while the R x R window holds more than 2 other marks, then while it leaves the frame.
+ * sets[0..2] (roi_size 7, 9, 11 in that order, d_n required) receive the records of their
+ * candidates in raster order, at most sets[k].n each, and the counts in *sets[k].d_n.
+ * n_counts (3 ints, may be NULL) receives the counts after a stream synchronisation; with
+ * NULL nothing is waited for. */
+int qcs_extract_mixed_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, int cap,
+                          const qcs_roi_set *sets, int *n_counts, void *stream);
+
+/* Frames to localizations through the converged multi-emitter fitter (no counterpart in the
+ * reference; its frame-driven multi-emitter flow is IJ/OnlineLocalizationLD.cpp:173-232 with the
+ * fixed-schedule cascade): qcs_extract_mixed_dev (at most cap candidates, every set of capacity
+ * cap), qcs_fit_mle_mixed_dev with emitter counts up to kmax, then the rows of every chosen
+ * emitter in candidate order -- frame, row, column of the detection, then emitter -- so the
+ * output is frame-sorted.  Rows carry the FP64 full-Fisher CRLB.  d_n_out (4 device ints)
+ * receives the rows written (at most out_cap), the rows produced, the ROIs fitted and the
+ * candidates found.  Nothing is waited for and
+ * every count stays on the device, so the call can be captured in a HIP graph. */
+int qcs_loc_mle_frames_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, int cap, int kmax,
+                           float *d_out, int out_cap, int *d_n_out, void *stream);
 
 /* ========================================================================================
  * Learned warm start (no counterpart in the reference): a 3-conv + 2-layer MLP on 7x7 ROIs
@@ -454,6 +492,13 @@ int qcs_drift_correct_dev(qcs_ctx *ctx, float *d_rows, long long n, int corr_fra
 typedef struct qcs_graph qcs_graph;
 int qcs_graph_create_mle(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, int kmax, float *d_loc, int *d_sel,
                          void *stream, qcs_graph **out);
+/* One frame batch through qcs_loc_mle_frames_dev (mixed-size extraction, converged fits,
+ * frame-sorted rows) captured over fixed buffers: each launch localizes the nframes frames then
+ * in d_frames, numbered from the device int *d_start_frame as it is at run time, into d_out /
+ * d_n_out.  A per-frame-batch pipeline writes each batch and its first frame number and launches
+ * the graph; replays equal eager calls. */
+int qcs_graph_create_frames(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, const int *d_start_frame, int cap, int kmax,
+                            float *d_out, int out_cap, int *d_n_out, void *stream, qcs_graph **out);
 int qcs_graph_launch(qcs_graph *g, void *stream);
 int qcs_graph_destroy(qcs_graph *g);
 
@@ -568,8 +613,11 @@ typedef struct qcs_sim_params {
     float    center_jitter;              /* ROIs: centre uniform within +-jitter of R/2 */
     int      astig;                      /* 1: sigma_x/sigma_y from z (astigmatic model) */
     float    z_min, z_max, s0, c_nm, d_nm;   /* astigmatic model */
-    int      n_per_unit_max;             /* ROIs: 1..n emitters (multi-emitter ROIs) */
-    float    min_sep, max_sep;           /* ROIs with >1 emitter: separation range, px */
+    int      n_per_unit_max;             /* ROIs: 1..n emitters (multi-emitter ROIs); frames: emitters
+                                          * come in sites of 1..n (clusters) */
+    float    min_sep, max_sep;           /* ROIs / sites with >1 emitter: separation range, px */
+    float    density_max_per_px;         /* frames: when above density_per_px, each frame's density
+                                          * is uniform in [density_per_px, density_max_per_px] */
 } qcs_sim_params;
 void qcs_sim_params_default(qcs_sim_params *sp);
 
@@ -583,7 +631,8 @@ int qcs_sim_rois_host(const qcs_sim_params *sp, uint64_t seed, int n, int first_
                       uint16_t *rois, float *truth);
 int qcs_sim_rois_dev(const qcs_sim_params *sp, uint64_t seed, int n, int first_frame,
                      uint16_t *d_rois, float *d_truth, void *stream);
-/* Number of emitters the simulator places in frame `frame` (deterministic). */
+/* Number of emitters the simulator places in frame `frame` (deterministic); truth (cap rows of
+ * QCS_TRUTH_FLOATS, may be NULL) receives them. */
 int qcs_sim_frame_emitters(const qcs_sim_params *sp, uint64_t seed, int frame, float *truth, int cap);
 /* Simulate `nframes` frames width x height starting at frame index `first_frame`. */
 int qcs_sim_frames_host(const qcs_sim_params *sp, uint64_t seed, int first_frame, int nframes,

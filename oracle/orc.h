@@ -68,6 +68,14 @@ int   orc_extract(const qcs_params *p, const uint16_t *frames, int nframes, int 
                   uint16_t *rois, float *wle, int cap, int *n_found, uint16_t *filtered,
                   float *variance_out);
 
+/* mixed-size extraction of the converged multi-emitter path (orc_mixed.c): detection at size 7,
+ * the library's size rule per candidate, the first cap[s] candidates of each size cut into
+ * rois[s] (sizes 7, 9, 11); candidates examined: the first cand_cap of the call */
+int   orc_mixed_size(const uint8_t *mark, int w, int h, int x, int y, const float *wle7);
+int   orc_extract_mixed(const qcs_params *p, const uint16_t *frames, int nframes, int start_frame, int cand_cap,
+                        uint16_t *const rois[3], const int cap[3], int counts[3], int *sizes, int *n_cand,
+                        int *n_found);
+
 /* ---------------- WLE pre-estimation (DLL/WLEParaEstimation) ---------------- */
 void  orc_spline_interp(const float *x_i, const float *y_i, int n_in, const float *x_o, float *y_o,
                         int n_out);

@@ -2,6 +2,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <atomic>
 #include <new>
 #include <string>
 #include "qcs_internal.h"
@@ -26,11 +27,24 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line)
     return fail(QCS_ERR_HIP, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
+// test hook (qcs_test_fail_alloc_after): allocations left before one fails, -1 = never
+static std::atomic<int> g_fail_alloc_after{-1};
+
+static bool injected_alloc_failure()
+{
+    int v = g_fail_alloc_after.load();
+    while (v >= 0) {
+        if (g_fail_alloc_after.compare_exchange_weak(v, v - 1)) return v == 0;
+    }
+    return false;
+}
+
 int DevBuf::ensure(size_t b)
 {
     if (b <= bytes && ptr) return QCS_OK;
     release();
     if (b == 0) return QCS_OK;
+    if (injected_alloc_failure()) return fail(QCS_ERR_NOMEM, "hipMalloc(%zu) failed: injected by qcs_test_fail_alloc_after", b);
     hipError_t e = hipMalloc(&ptr, b);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -53,6 +67,7 @@ int HostBuf::ensure(size_t b)
     if (b <= bytes && ptr) return QCS_OK;
     release();
     if (b == 0) return QCS_OK;
+    if (injected_alloc_failure()) return fail(QCS_ERR_NOMEM, "hipHostMalloc(%zu) failed: injected by qcs_test_fail_alloc_after", b);
     hipError_t e = hipHostMalloc(&ptr, b, hipHostMallocDefault);
     if (e != hipSuccess) {
         ptr = nullptr;
@@ -227,7 +242,8 @@ int qcs_ctx_destroy(qcs_ctx *c)
                            &c->ex_frames, &c->fit_idx1, &c->fit_idx2, &c->fit_idx3, &c->fit_flag, &c->fit_cnt,
                            &c->fit_rows, &c->h_rois_dev, &c->h_wle_dev, &c->h_loc_dev, &c->cf_theta, &c->cf_ll,
                            &c->cf_list, &c->cf_rows, &c->cf_queue, &c->rn_buf, &c->st_buf,
-                           &c->dr_img, &c->dr_cc, &c->dr_shift, &c->dr_frames, &c->dp_hist};
+                           &c->dr_img, &c->dr_cc, &c->dr_shift, &c->dr_frames, &c->dp_hist,
+                           &c->mx_rois, &c->mx_list, &c->mx_sets, &c->mx_rows};
     for (auto *b : bufs) b->release();
     c->pin_cnt.release();
     (void)timer_resolve(c->timer);
@@ -277,6 +293,8 @@ int qcs_timing_get(qcs_ctx *c, qcs_timing *out)
     out->drift_cc_pixels = t.items[TM_DRIFTCC];
     return QCS_OK;
 }
+
+void qcs_test_fail_alloc_after(int n) { g_fail_alloc_after.store(n < 0 ? -1 : n); }
 
 int qcs_fit_stats_get(qcs_ctx *c, qcs_fit_stats *out)
 {

@@ -727,6 +727,7 @@ extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int 
             const size_t o = off[i];
             x.rois = sets[i].d_rois;
             x.n = sets[i].n;
+            x.n_dev = sets[i].d_n;
             x.R = sets[i].roi_size;
             x.init = K == 1 ? nullptr : (K == 2 ? m.t1 + o * n1 : m.t2 + o * n2);
             x.istride = K == 1 ? 0 : (K == 2 ? n1 : n2);
@@ -746,8 +747,8 @@ extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int 
                 p.ROISize = sets[i].roi_size;
                 const size_t o = off[i];
                 int *sel = sets[i].d_sel ? sets[i].d_sel : m.sel + o;
-                if ((rc = select12_launch(p, kmax, nullptr, sets[i].n, nullptr, m.t1 + o * n1, m.t2 + o * n2, m.l1 + o, m.l2 + o,
-                                          nullptr, sel, m.cnt + i, m.list3 + o, s)))
+                if ((rc = select12_launch(p, kmax, nullptr, sets[i].n, sets[i].d_n, m.t1 + o * n1, m.t2 + o * n2, m.l1 + o,
+                                          m.l2 + o, nullptr, sel, m.cnt + i, m.list3 + o, s)))
                     return rc;
             }
         }
@@ -757,12 +758,133 @@ extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int 
         p.ROISize = sets[i].roi_size;
         const size_t o = off[i];
         int *sel = sets[i].d_sel ? sets[i].d_sel : m.sel + o;
-        if ((rc = rows_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, nullptr, m.t1 + o * n1, m.t2 + o * n2, m.t3 + o * n3,
-                              m.l2 + o, m.l3 + o, sel, sets[i].d_loc, s)))
+        if ((rc = rows_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, sets[i].d_n, m.t1 + o * n1, m.t2 + o * n2,
+                              m.t3 + o * n3, m.l2 + o, m.l3 + o, sel, sets[i].d_loc, s)))
             return rc;
     }
     return timer_end(ctx->timer, TM_CFIT, s, ev, (long long)total);
 }
+
+namespace qcs {
+
+int extract_mixed_launch(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, const int *d_start, int cap,
+                         const qcs_roi_set *sets, int *cand_lists[3], int **cand_info, hipStream_t s);
+
+// rows kept per candidate: cnt[cand[j]] = sel[j] for the set's ROIs j < *n
+__global__ void __launch_bounds__(256) frame_rows_count_kernel(const int *__restrict__ cand, const int *__restrict__ n_dev, int n,
+                                                               const int *__restrict__ sel, int *__restrict__ cnt)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || j >= *n_dev) return;
+    cnt[cand[j]] = sel[j];
+}
+
+// the chosen emitters' rows of ROI j (rows 3j .. 3j + sel - 1) at out[off[cand[j]] ...]
+__global__ void __launch_bounds__(256) frame_rows_gather_kernel(const int *__restrict__ cand, const int *__restrict__ n_dev, int n,
+                                                                const int *__restrict__ sel, const int *__restrict__ off,
+                                                                const float *__restrict__ rows, int out_cap, float *__restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || j >= *n_dev) return;
+    const int k = sel[j], o = off[cand[j]];
+    for (int e = 0; e < k; e++) {
+        if (o + e >= out_cap) break;
+        const float4 *src = reinterpret_cast<const float4 *>(rows + ((size_t)3 * j + e) * QCS_REC_FLOATS);
+        float4 *dst = reinterpret_cast<float4 *>(out + (size_t)(o + e) * QCS_REC_FLOATS);
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
+    }
+}
+
+// [0] rows written, [1] rows produced, [2] ROIs fitted (the three sets), [3] candidates found
+__global__ void frame_rows_total_kernel(const int *__restrict__ tile_sums, int ntiles, int out_cap, const int *__restrict__ set_n,
+                                        const int *__restrict__ info, int *__restrict__ n_out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int t = tile_sums[ntiles];
+    n_out[0] = t < out_cap ? t : out_cap;
+    n_out[1] = t;
+    n_out[2] = set_n[0] + set_n[1] + set_n[2];
+    n_out[3] = info[1];
+}
+
+// qcs_loc_mle_frames_dev; with d_start set the first frame number is read from it at run time
+int loc_mle_frames(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, const int *d_start, int cap, int kmax,
+                   float *d_out, int out_cap, int *d_n_out, hipStream_t s);
+
+}  // namespace qcs
+
+extern "C" int qcs_loc_mle_frames_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, int cap, int kmax,
+                                      float *d_out, int out_cap, int *d_n_out, void *stream)
+{
+    return loc_mle_frames(ctx, d_frames, nframes, start_frame, nullptr, cap, kmax, d_out, out_cap, d_n_out, (hipStream_t)stream);
+}
+
+namespace qcs {
+
+int loc_mle_frames(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, const int *d_start, int cap, int kmax,
+                   float *d_out, int out_cap, int *d_n_out, hipStream_t s)
+{
+    if (!ctx || nframes < 0 || cap < 0 || (nframes > 0 && !d_frames) || kmax < 1 || kmax > 3 || out_cap < 0 ||
+        (out_cap > 0 && !d_out) || !d_n_out)
+        return fail(QCS_ERR_ARG, "invalid argument");
+    if (ctx->p.LocType != QCS_LOCTYPE_GS2D && ctx->p.LocType != QCS_LOCTYPE_AS3D)
+        return fail(QCS_ERR_ARG, "converged fitting supports the 2D and astigmatic 3D models");
+    if ((long long)cap * 3 > (long long)INT32_MAX / 32) return fail(QCS_ERR_ARG, "capacity too large");
+    const int ncap = cap > 0 ? cap : 1;
+    if (nframes == 0 || cap == 0) {
+        fill_async(d_n_out, 4, 0, s);
+        QCS_HIP(hipGetLastError());
+        return QCS_OK;
+    }
+    // sets of capacity cap each: records, 3 rows per ROI, chosen counts, device counts
+    const size_t rw[3] = {(size_t)QCS_ROI_WORDS(7) * ncap, (size_t)QCS_ROI_WORDS(9) * ncap, (size_t)QCS_ROI_WORDS(11) * ncap};
+    const size_t rec_bytes = ((rw[0] + rw[1] + rw[2]) * sizeof(uint16_t) + 255) & ~(size_t)255;
+    const int ntiles = scan_tile_count((size_t)ncap);
+    int rc;
+    if ((rc = ctx->mx_sets.ensure(rec_bytes + sizeof(int) * (3 * (size_t)ncap + 16))) ||
+        (rc = ctx->mx_rows.ensure(sizeof(float) * QCS_REC_FLOATS * 9 * (size_t)ncap + sizeof(int) * (2 * (size_t)ncap + ntiles + 1))))
+        return rc;
+    uint16_t *recs = ctx->mx_sets.as<uint16_t>();
+    int *sel = reinterpret_cast<int *>(ctx->mx_sets.as<char>() + rec_bytes);
+    int *cnts = sel + 3 * (size_t)ncap;
+    float *rows = ctx->mx_rows.as<float>();
+    int *rcnt = reinterpret_cast<int *>(rows + (size_t)QCS_REC_FLOATS * 9 * ncap);
+    int *roff = rcnt + ncap, *rts = roff + ncap;
+    qcs_roi_set sets[3];
+    size_t ro = 0;
+    for (int k = 0; k < 3; k++) {
+        sets[k].roi_size = 7 + 2 * k;
+        sets[k].n = cap;
+        sets[k].d_rois = recs + ro;
+        sets[k].d_loc = rows + (size_t)QCS_REC_FLOATS * 3 * ncap * k;
+        sets[k].d_sel = sel + (size_t)ncap * k;
+        sets[k].d_n = cnts + k;
+        ro += rw[k];
+    }
+    int *cand[3], *info = nullptr;
+    hipEvent_t ev;
+    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+    if ((rc = extract_mixed_launch(ctx, d_frames, nframes, start_frame, d_start, cap, sets, cand, &info, s))) return rc;
+    if ((rc = timer_end(ctx->timer, TM_EXTRACT, s, ev, nframes))) return rc;
+    if ((rc = qcs_fit_mle_mixed_dev(ctx, sets, 3, kmax, s))) return rc;
+    // rows in candidate order: per-candidate counts, their scan, then the gather
+    fill_async(rcnt, ncap, 0, s);
+    const dim3 g((ncap + 255) / 256), b(256);
+    for (int k = 0; k < 3; k++)
+        hipLaunchKernelGGL(frame_rows_count_kernel, g, b, 0, s, cand[k], sets[k].d_n, cap, sets[k].d_sel, rcnt);
+    QCS_HIP(hipGetLastError());
+    if ((rc = scan_exclusive(rcnt, ncap, roff, rts, s))) return rc;
+    for (int k = 0; k < 3; k++)
+        hipLaunchKernelGGL(frame_rows_gather_kernel, g, b, 0, s, cand[k], sets[k].d_n, cap, sets[k].d_sel, roff, sets[k].d_loc, out_cap,
+                           d_out);
+    hipLaunchKernelGGL(frame_rows_total_kernel, dim3(1), dim3(1), 0, s, rts, ntiles, out_cap, cnts, info, d_n_out);
+    QCS_HIP(hipGetLastError());
+    return QCS_OK;
+}
+
+}  // namespace qcs
 
 extern "C" int qcs_fit_warm_dev(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float *d_loc, int *d_sel,
                                 long long *h_stats, void *stream)

@@ -21,7 +21,6 @@
 // writes; rows zeroed by an earlier call have frame 0), which the call checks.  The merge walks
 // and the removal walks are the reference's; the weighted sums are explicit FMAs where the
 // reference build contracts `w += x * p`.
-#include <hipcub/hipcub.hpp>
 #include <limits.h>
 #include <math.h>
 #include <string.h>
@@ -41,6 +40,14 @@ struct ConsecAcc {
     int flag;   // 1: frame out of [0, 2^30); 2: non-zero frames decrease along the rows
     int valid;
     int hist[ONT];
+    int fmin, fmax, cymax, cxmax;   // key ranges of the call (frame clamped to [0, 2^30), cells)
+};
+
+// sort key (frame - fmin, cell row, cell column) packed with the call's field widths: every
+// cell a link search visits (one beyond the largest present, at most 65535) and every frame
+// up to the largest + 1 has its own key
+struct KeyPack {
+    int fmin, fshift, yshift;
 };
 
 // float -> int, truncating; NaN -> 0 and saturation (the conversion of the reference's device)
@@ -58,19 +65,48 @@ __device__ __forceinline__ int cell_of(int v, int cs)
     return (int)(q < 0 ? 0 : (q > 65535 ? 65535 : q));
 }
 
-__device__ __forceinline__ unsigned long long cell_key(int f, int cy, int cx)
+__device__ __forceinline__ unsigned long long cell_key(const KeyPack &k, int f, int cy, int cx)
 {
-    return ((unsigned long long)(unsigned)f << 32) | ((unsigned long long)(unsigned)cy << 16) | (unsigned long long)(unsigned)cx;
+    return ((unsigned long long)(unsigned)(f - k.fmin) << k.fshift) | ((unsigned long long)(unsigned)cy << k.yshift) |
+           (unsigned long long)(unsigned)cx;
 }
 
 __global__ void consec_clear_kernel(ConsecAcc *acc)
 {
     int *p = reinterpret_cast<int *>(acc);
     for (int k = threadIdx.x; k < (int)(sizeof(ConsecAcc) / sizeof(int)); k += blockDim.x) p[k] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) acc->fmin = INT_MAX;
+}
+
+// frames (clamped, as the keys use them) and cells: their ranges in acc
+__global__ void __launch_bounds__(256) consec_range_kernel(const float *__restrict__ a, int N, int cs, ConsecAcc *__restrict__ acc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int fmin = INT_MAX, fmax = 0, cymax = 0, cxmax = 0;
+    if (i < N) {
+        const float *r = a + (size_t)i * QCS_REC_FLOATS;
+        const int f = f2i(r[QCS_POS_FRME]);
+        fmin = fmax = min(max(f, 0), FRAME_LIMIT - 1);
+        cymax = cell_of(f2i(r[QCS_POS_YPOS]), cs);
+        cxmax = cell_of(f2i(r[QCS_POS_XPOS]), cs);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        fmin = min(fmin, __shfl_xor(fmin, off, 64));
+        fmax = max(fmax, __shfl_xor(fmax, off, 64));
+        cymax = max(cymax, __shfl_xor(cymax, off, 64));
+        cxmax = max(cxmax, __shfl_xor(cxmax, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&acc->fmin, fmin);
+        atomicMax(&acc->fmax, fmax);
+        atomicMax(&acc->cymax, cymax);
+        atomicMax(&acc->cxmax, cxmax);
+    }
 }
 
 // keys (frame, cell row, cell column), identity values, frames; clears the forward flags
-__global__ void __launch_bounds__(256) consec_keys_kernel(const float *__restrict__ a, int N, int cs,
+__global__ void __launch_bounds__(256) consec_keys_kernel(const float *__restrict__ a, int N, int cs, KeyPack kp,
                                                           unsigned long long *__restrict__ keys, int *__restrict__ vals,
                                                           int *__restrict__ fr, int *__restrict__ fwd, ConsecAcc *__restrict__ acc)
 {
@@ -81,7 +117,7 @@ __global__ void __launch_bounds__(256) consec_keys_kernel(const float *__restric
     const int x = f2i(r[QCS_POS_XPOS]), y = f2i(r[QCS_POS_YPOS]);
     if (f < 0 || f >= FRAME_LIMIT) atomicOr(&acc->flag, 1);
     const int fc = min(max(f, 0), FRAME_LIMIT - 1);
-    keys[i] = cell_key(fc, cell_of(y, cs), cell_of(x, cs));
+    keys[i] = cell_key(kp, fc, cell_of(y, cs), cell_of(x, cs));
     vals[i] = i;
     fr[i] = fc;
     fwd[i] = 0;
@@ -99,7 +135,7 @@ __device__ __forceinline__ int lower_bound_u64(const unsigned long long *__restr
 }
 
 // gpuFindConsecutiveMolecules: the closest row of the next frame, ties to the lower row index
-__global__ void __launch_bounds__(256) consec_link_kernel(const float *__restrict__ a, int N, int cs, float th2,
+__global__ void __launch_bounds__(256) consec_link_kernel(const float *__restrict__ a, int N, int cs, KeyPack kp, float th2,
                                                           const unsigned long long *__restrict__ keys,
                                                           const int *__restrict__ vals, const int *__restrict__ fr,
                                                           const int *__restrict__ pmax, int *__restrict__ back,
@@ -118,7 +154,7 @@ __global__ void __launch_bounds__(256) consec_link_kernel(const float *__restric
     if (f + 1 < FRAME_LIMIT) {
         for (int yy = max(cy - 1, 0); yy <= min(cy + 1, 65535); yy++)
             for (int xx = max(cx - 1, 0); xx <= min(cx + 1, 65535); xx++) {
-                const unsigned long long k = cell_key(f + 1, yy, xx);
+                const unsigned long long k = cell_key(kp, f + 1, yy, xx);
                 for (int q = lower_bound_u64(keys, N, k); q < N && keys[q] == k; q++) {
                     const int j = vals[q];
                     if (j < j0) continue;
@@ -214,10 +250,6 @@ __global__ void __launch_bounds__(256) consec_remove_kernel(int N, const int *__
     }
 }
 
-struct MaxOp {
-    __device__ __forceinline__ int operator()(int a, int b) const { return a > b ? a : b; }
-};
-
 }  // namespace
 
 }  // namespace qcs
@@ -291,10 +323,7 @@ int qcs_consec_fit_dev(qcs_consec *c, const float *d_rows, int n, int is_end, fl
         if (is_end) c->n_last = 0;
         return QCS_OK;
     }
-    size_t sort_b = 0, scan_b = 0;
-    QCS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_b, (const unsigned long long *)nullptr,
-                                               (unsigned long long *)nullptr, (const int *)nullptr, (int *)nullptr, N, 0, 62, s));
-    QCS_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, scan_b, (const int *)nullptr, (int *)nullptr, MaxOp(), N, s));
+    const size_t sort_b = sizeof(int) * radix_sort_tmp_ints(N), scan_b = sizeof(int) * max_scan_tmp_ints(N);
     const int nxt = c->cur ^ 1;
     if ((rc = c->cat.ensure((size_t)N * rb)) || (rc = c->mrg.ensure((size_t)N * rb)) ||
         (rc = c->keys.ensure((size_t)2 * N * 8)) || (rc = c->vals.ensure((size_t)2 * N * 4)) ||
@@ -315,11 +344,27 @@ int qcs_consec_fit_dev(qcs_consec *c, const float *d_rows, int n, int is_end, fl
     ConsecAcc *acc = c->acc.as<ConsecAcc>();
     const dim3 g((N + 255) / 256), b(256);
     hipLaunchKernelGGL(consec_clear_kernel, dim3(1), dim3(64), 0, s, acc);
-    hipLaunchKernelGGL(consec_keys_kernel, g, b, 0, s, cat, N, cs, k_in, v_in, c->fr.as<int>(), c->fwd.as<int>(), acc);
+    hipLaunchKernelGGL(consec_range_kernel, g, b, 0, s, cat, N, cs, acc);
     QCS_HIP(hipGetLastError());
-    QCS_HIP(hipcub::DeviceRadixSort::SortPairs(c->tmp.ptr, sort_b, k_in, k_out, v_in, v_out, N, 0, 62, s));
-    QCS_HIP(hipcub::DeviceScan::InclusiveScan(c->tmp.ptr, scan_b, c->fr.as<int>(), c->pmax.as<int>(), MaxOp(), N, s));
-    hipLaunchKernelGGL(consec_link_kernel, g, b, 0, s, cat, N, cs, th2, k_out, v_out, c->fr.as<int>(), c->pmax.as<int>(),
+    // the key fields' widths from the call's ranges
+    QCS_HIP(hipMemcpyAsync(c->h_acc.ptr, acc, sizeof(ConsecAcc), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipStreamSynchronize(s));
+    const ConsecAcc *hr = c->h_acc.as<ConsecAcc>();
+    auto nbits = [](long long v) { int b = 0; while (b < 62 && (v >> b) != 0) b++; return b; };
+    KeyPack kp;
+    kp.fmin = hr->fmin;
+    const int xb = nbits(std::min(hr->cxmax + 1, 65535)), yb = nbits(std::min(hr->cymax + 1, 65535));
+    const int fb = nbits((long long)hr->fmax + 1 - hr->fmin);
+    kp.yshift = xb;
+    kp.fshift = xb + yb;
+    hipLaunchKernelGGL(consec_keys_kernel, g, b, 0, s, cat, N, cs, kp, k_in, v_in, c->fr.as<int>(), c->fwd.as<int>(), acc);
+    QCS_HIP(hipGetLastError());
+    unsigned long long *k_sorted;
+    int *v_sorted;
+    if ((rc = radix_sort_pairs_u64(k_in, v_in, k_out, v_out, N, fb + yb + xb, c->tmp.as<int>(), s, &k_sorted, &v_sorted)))
+        return rc;
+    if ((rc = scan_inclusive_max(c->fr.as<int>(), N, c->pmax.as<int>(), c->tmp.as<int>(), s))) return rc;
+    hipLaunchKernelGGL(consec_link_kernel, g, b, 0, s, cat, N, cs, kp, th2, k_sorted, v_sorted, c->fr.as<int>(), c->pmax.as<int>(),
                        c->back.as<int>(), c->fwd.as<int>(), acc);
     QCS_HIP(hipGetLastError());
     hipLaunchKernelGGL(consec_merge_kernel, g, b, 0, s, cat, N, L, sqrtf(p.QE), c->back.as<int>(), c->fwd.as<int>(), mrg, acc);

@@ -76,6 +76,7 @@ struct ExtractConst {
     int nseg;              // SEGW-column segments per row (one walking wave each)
     int walk_rows;         // image rows per walking wave
     float taps[5];
+    const int *start_dev;  // when set: the call's first frame number is read here at run time (graph replays)
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -609,20 +610,23 @@ __device__ __forceinline__ int mark_count(const unsigned long long *mark, int ns
 // memory round trips in flight instead of one
 constexpr int CUT_U = 8;
 
-__global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
+// R is the cut size; with sel set, record j is candidate sel[j] for j < *sel_n (a set of the
+// mixed-size extraction), else record i is candidate i for i < info[0]
+__global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ frames, ExtractConst c, int R,
                                                  const unsigned long long *__restrict__ mark_mask,
                                                  const int *__restrict__ info, const int3 *__restrict__ xyb,
+                                                 const int *__restrict__ sel, const int *__restrict__ sel_n,
                                                  uint16_t *__restrict__ rois)
 {
-    const int n = info[0];
-    const int R = c.R, half = R / 2, WR = R * (R + 1), RR = R * R;
+    const int n = sel ? *sel_n : info[0];
+    const int half = R / 2, WR = R * (R + 1), RR = R * R;
     const int l = threadIdx.x;
     for (int i0 = blockIdx.x * CUT_U; i0 < n; i0 += gridDim.x * CUT_U) {
         int3 e[CUT_U];
         int ft[CUT_U];
 #pragma unroll
         for (int j = 0; j < CUT_U; j++) {
-            e[j] = (i0 + j < n) ? xyb[i0 + j] : make_int3(half, half, 0);
+            e[j] = (i0 + j < n) ? xyb[sel ? sel[i0 + j] : i0 + j] : make_int3(half, half, 0);
             ft[j] = QCS_ROITYPE_SINGLE;
         }
         if (c.multi) {
@@ -653,7 +657,7 @@ __global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ fr
                     v[j] = (i0 + j < n) ? img[(size_t)(y - half + r) * c.W + (x - half + cc)] : (uint16_t)0;
                 } else {
                     const int h = k - RR;
-                    const int fr = c.start_frame + b * c.B + y / c.H;
+                    const int fr = (c.start_dev ? *c.start_dev : c.start_frame) + b * c.B + y / c.H;
                     v[j] = h == 0 ? (uint16_t)x
                          : h == 1 ? (uint16_t)(y % c.H)
                          : h == 2 ? (uint16_t)(fr % 65536)
@@ -716,23 +720,23 @@ int compact_scatter(const int *flags, const int *offs, int n, const int *src, co
 
 }  // namespace qcs
 
-using namespace qcs;
+namespace qcs {
 
-extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, uint16_t *d_rois,
-                               float *d_wle, int cap, int *n_out, int *n_found, void *stream)
-{
-    if (n_out) *n_out = 0;
-    if (n_found) *n_found = 0;
-    if (!ctx || nframes < 0 || cap < 0 || (nframes > 0 && !d_frames) || (cap > 0 && (!d_rois || !d_wle)))
-        return fail(QCS_ERR_ARG, "invalid argument");
-    const qcs_params &p = ctx->p;
-    if (p.ImageWidth <= 0 || p.ImageHigh <= 0) return fail(QCS_ERR_ARG, "ImageWidth/ImageHigh not set");
-    const int R = p.ROISize;
-    if (p.ImageWidth < R || p.ImageHigh < R) return fail(QCS_ERR_ARG, "image smaller than the ROI");
-    if (nframes == 0) return QCS_OK;
-    hipStream_t s = (hipStream_t)stream;
-
+// Everything of one extraction call up to the candidate list (no host synchronisation):
+// variance, walking filter + detection, row scan, per-batch bookkeeping, raster-order (x, y,
+// batch) list.  info[0] receives the kept count (at most cap), info[1] the found count.
+struct ExtractRun {
     ExtractConst c;
+    int *info;
+    int3 *xyb;
+    const unsigned long long *mark;
+};
+
+static int extract_candidates(qcs_ctx *ctx, const qcs_params &p, const uint16_t *d_frames, int nframes, int start_frame,
+                              const int *d_start, int cap, hipStream_t s, ExtractRun &run)
+{
+    const int R = p.ROISize;
+    ExtractConst &c = run.c;
     memset(&c, 0, sizeof(c));
     c.W = p.ImageWidth;
     c.H = p.ImageHigh;
@@ -746,6 +750,7 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
     c.batch_cap = p.MaxROIPerBatch > 0 ? p.MaxROIPerBatch : QCS_MAX_POINT_NUM;
     c.cap = cap;
     c.start_frame = start_frame;
+    c.start_dev = d_start;
     c.nseg = (c.W + SEGW - 1) / SEGW;
     filter_taps_host(R, p.LocType, p.MultiEmitterFitEn, c.taps);
     // rows per walking wave: long walks amortise the 14-row pipeline fill; shorter ones keep
@@ -766,8 +771,7 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
         (rc = ctx->ex_mark.ensure(sizeof(unsigned long long) * nrows * c.nseg * 2)) ||
         (rc = ctx->ex_cnt.ensure(sizeof(int) * (nrows * 2 + ntiles + 1))) ||
         (rc = ctx->ex_xy.ensure(sizeof(int3) * (size_t)(cap > 0 ? cap : 1))) ||
-        (rc = ctx->ex_tmp.ensure(sizeof(int) * (2 + 3 * (size_t)c.nbatch))) ||
-        (rc = ctx->pin_cnt.ensure(64)))
+        (rc = ctx->ex_tmp.ensure(sizeof(int) * (2 + 3 * (size_t)c.nbatch))))
         return rc;
     unsigned long long *var_sums = ctx->ex_var.as<unsigned long long>();
     uint16_t *tvbuf = ctx->ex_det.as<uint16_t>();
@@ -787,8 +791,6 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
         sh = sh > 256 ? 256 : (sh < 1 ? 1 : sh);
         Smax = sw < sh ? sw : sh;
     }
-    hipEvent_t ev;
-    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
     fill_async(reinterpret_cast<int *>(var_sums), 4 * (size_t)c.nbatch, 0, s);
     fill_async(row_count, nrows, 0, s);
     hipLaunchKernelGGL((walk_kernel<0, false>), dim3((2 * Smax + SEGW - 1) / SEGW, (2 * Smax + c.walk_rows - 1) / c.walk_rows, c.nbatch),
@@ -809,20 +811,209 @@ extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nfram
     hipLaunchKernelGGL(emit_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, c, cand, row_off, row_count,
                        (int)nrows, info, xyb);
     QCS_HIP(hipGetLastError());
+    run.info = info;
+    run.xyb = xyb;
+    run.mark = mark;
+    return QCS_OK;
+}
+
+static int check_extract_args(const qcs_params &p, int R)
+{
+    if (p.ImageWidth <= 0 || p.ImageHigh <= 0) return fail(QCS_ERR_ARG, "ImageWidth/ImageHigh not set");
+    if (p.ImageWidth < R || p.ImageHigh < R) return fail(QCS_ERR_ARG, "image smaller than the ROI");
+    return QCS_OK;
+}
+
+// ---- mixed ROI sizes (no reference counterpart) ----------------------------------------------
+// Each candidate of the base-size-7 extraction gets an ROI size in {7, 9, 11}:
+//   s    = min(SigmaL, SigmaR, SigmaU, SigmaD) of its 7 x 7 WLE record;
+//   Rpsf = 7 if s <= 1.4, 9 if s <= 1.9, else 11 (about +-2.5 s);
+//   c(d) = detector marks in the (2d + 1)^2 window, rows of the candidate's own frame only;
+//   dmin = smallest d in 1..5 with c(d) > 1 (6 if none);
+//   Rnb  = 2 dmin + 3 for dmin 3 or 4 (a neighbour on the edge of the 7 x 7 window is taken in
+//          whole, with a pixel of margin), else 7;
+//   R    = max(Rpsf, Rnb), reduced by 2 while R > 7 and the window holds more than 2 other marks
+//          (a fit models at most 3 emitters), then while R > 7 and the window leaves the frame.
+// flags[s * ncap + i] = 1 when candidate i gets size 7 + 2 s; entries i >= *n are cleared.
+__global__ void __launch_bounds__(256) mixed_size_kernel(ExtractConst c, const unsigned long long *__restrict__ mark_mask,
+                                                         const int *__restrict__ info, const int3 *__restrict__ xyb,
+                                                         const float *__restrict__ wle7, int ncap, int *__restrict__ flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncap) return;
+    int R = 0;
+    if (i < info[0]) {
+        const int3 e = xyb[i];
+        const int x = e.x, y = e.y;                 // y: row of the stacked batch
+        const int fy0 = (y / c.H) * c.H, ry = y - fy0;
+        const size_t rbase = (size_t)e.z * c.B * c.H;
+        const float *w = wle7 + (size_t)i * QCS_WLE_FLOATS;
+        float sg = w[QCS_WLE_SIGMAL];
+        sg = w[QCS_WLE_SIGMAR] < sg ? w[QCS_WLE_SIGMAR] : sg;
+        sg = w[QCS_WLE_SIGMAU] < sg ? w[QCS_WLE_SIGMAU] : sg;
+        sg = w[QCS_WLE_SIGMAD] < sg ? w[QCS_WLE_SIGMAD] : sg;
+        const int rpsf = (sg <= 1.4f) ? 7 : ((sg <= 1.9f) ? 9 : 11);
+        int cnt[6];
+        cnt[0] = 1;
+        for (int d = 1; d <= 5; d++) {
+            const int xa = x - d < 0 ? 0 : x - d, xb = x + d > c.W - 1 ? c.W - 1 : x + d;
+            int ya = y - d, yb = y + d;
+            ya = ya < fy0 ? fy0 : ya;
+            yb = yb > fy0 + c.H - 1 ? fy0 + c.H - 1 : yb;
+            int n = 0;
+            for (int yy = ya; yy <= yb; yy++) n += mark_count(mark_mask, c.nseg, rbase + yy, xa, xb);
+            cnt[d] = n;
+        }
+        int dmin = 6;
+        for (int d = 5; d >= 1; d--)
+            if (cnt[d] > 1) dmin = d;
+        const int rnb = (dmin == 3 || dmin == 4) ? 2 * dmin + 3 : 7;
+        R = rpsf > rnb ? rpsf : rnb;
+        while (R > 7 && cnt[R / 2] - 1 > 2) R -= 2;
+        while (R > 7 && !(x - R / 2 >= 0 && x + R / 2 < c.W && ry - R / 2 >= 0 && ry + R / 2 < c.H)) R -= 2;
+    }
+    flags[i] = R == 7;
+    flags[(size_t)ncap + i] = R == 9;
+    flags[2 * (size_t)ncap + i] = R == 11;
+}
+
+// set lists: sel[off[i]] = i for flagged i with off[i] < cap; *n_out = min(total, cap)
+__global__ void __launch_bounds__(256) mixed_scatter_kernel(const int *__restrict__ flags, const int *__restrict__ offs, int ncap,
+                                                            int cap, int *__restrict__ sel)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncap || !flags[i]) return;
+    const int o = offs[i];
+    if (o < cap) sel[o] = i;
+}
+
+__global__ void mixed_count_kernel(const int *__restrict__ tile_sums, int ntiles, int cap, int *__restrict__ n_out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int t = tile_sums[ntiles];
+    *n_out = t < cap ? t : cap;
+}
+
+// Mixed-size extraction (launches only): candidates of the base-size-7 extraction (at most cap),
+// their 7 x 7 cuts and WLE records in ctx scratch, the size rule, and per set s (sizes 7, 9, 11)
+// the candidate list cand_s (ctx scratch, returned), the count sets[s].d_n and the records
+// sets[s].d_rois (at most sets[s].n).
+int extract_mixed_launch(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, const int *d_start, int cap,
+                         const qcs_roi_set *sets, int *cand_lists[3], int **cand_info, hipStream_t s)
+{
+    qcs_params p = ctx->p;
+    p.ROISize = 7;
+    int rc;
+    if ((rc = check_extract_args(p, 11))) return rc;
+    for (int k = 0; k < 3; k++)
+        if (sets[k].roi_size != 7 + 2 * k || sets[k].n < 0 || (sets[k].n > 0 && !sets[k].d_rois) || !sets[k].d_n)
+            return fail(QCS_ERR_ARG, "set %d: sizes 7, 9, 11 in order, a record buffer and a device count are required", k);
+    ExtractRun run;
+    if ((rc = extract_candidates(ctx, p, d_frames, nframes, start_frame, d_start, cap, s, run))) return rc;
+    const int ncap = cap > 0 ? cap : 1;
+    const int ntiles = scan_tile_count((size_t)ncap);
+    // scratch: 7 x 7 cuts, WLE records, flags (3 ncap), offsets (3 ncap), tile sums (3 (ntiles + 1)), lists (3 ncap)
+    const size_t r7w = (size_t)QCS_ROI_WORDS(7) * ncap;
+    if ((rc = ctx->mx_rois.ensure(sizeof(uint16_t) * r7w + sizeof(float) * QCS_WLE_FLOATS * (size_t)ncap)) ||
+        (rc = ctx->mx_list.ensure(sizeof(int) * (9 * (size_t)ncap + 3 * ((size_t)ntiles + 1)))))
+        return rc;
+    uint16_t *r7 = ctx->mx_rois.as<uint16_t>();
+    float *w7 = reinterpret_cast<float *>(r7 + r7w);
+    int *flags = ctx->mx_list.as<int>();
+    int *offs = flags + 3 * (size_t)ncap;
+    int *lists = offs + 3 * (size_t)ncap;
+    int *tsum = lists + 3 * (size_t)ncap;
     if (cap > 0) {
         const int cut_blocks = cap < 16384 ? cap : 16384;
-        hipLaunchKernelGGL(cut_kernel, dim3(cut_blocks), dim3(64), 0, s, d_frames, c, mark, info, xyb, d_rois);
+        hipLaunchKernelGGL(cut_kernel, dim3(cut_blocks), dim3(64), 0, s, d_frames, run.c, 7, run.mark, run.info, run.xyb, nullptr, nullptr,
+                           r7);
         QCS_HIP(hipGetLastError());
-        rc = wle_launch_extract(p, d_rois, xyb, c.B, c.H, info, cap, d_wle, s);
+        if ((rc = wle_launch_extract(p, r7, run.xyb, run.c.B, run.c.H, run.info, cap, w7, s))) return rc;
+    }
+    hipLaunchKernelGGL(mixed_size_kernel, dim3((ncap + 255) / 256), dim3(256), 0, s, run.c, run.mark, run.info, run.xyb, w7, ncap, flags);
+    QCS_HIP(hipGetLastError());
+    for (int k = 0; k < 3; k++) {
+        int *f = flags + (size_t)k * ncap, *o = offs + (size_t)k * ncap, *l = lists + (size_t)k * ncap;
+        int *ts = tsum + (size_t)k * (ntiles + 1);
+        if ((rc = scan_exclusive(f, ncap, o, ts, s))) return rc;
+        hipLaunchKernelGGL(mixed_scatter_kernel, dim3((ncap + 255) / 256), dim3(256), 0, s, f, o, ncap, sets[k].n, l);
+        hipLaunchKernelGGL(mixed_count_kernel, dim3(1), dim3(1), 0, s, ts, ntiles, sets[k].n, sets[k].d_n);
+        QCS_HIP(hipGetLastError());
+        if (sets[k].n > 0) {
+            const int blocks = sets[k].n < 16384 ? sets[k].n : 16384;
+            hipLaunchKernelGGL(cut_kernel, dim3(blocks), dim3(64), 0, s, d_frames, run.c, 7 + 2 * k, run.mark, run.info, run.xyb, l,
+                               sets[k].d_n, sets[k].d_rois);
+            QCS_HIP(hipGetLastError());
+        }
+        cand_lists[k] = l;
+    }
+    if (cand_info) *cand_info = run.info;
+    return QCS_OK;
+}
+
+}  // namespace qcs
+
+using namespace qcs;
+
+extern "C" int qcs_extract_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, uint16_t *d_rois,
+                               float *d_wle, int cap, int *n_out, int *n_found, void *stream)
+{
+    if (n_out) *n_out = 0;
+    if (n_found) *n_found = 0;
+    if (!ctx || nframes < 0 || cap < 0 || (nframes > 0 && !d_frames) || (cap > 0 && (!d_rois || !d_wle)))
+        return fail(QCS_ERR_ARG, "invalid argument");
+    const qcs_params &p = ctx->p;
+    int rc;
+    if ((rc = check_extract_args(p, p.ROISize))) return rc;
+    if (nframes == 0) return QCS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = ctx->pin_cnt.ensure(64))) return rc;
+    hipEvent_t ev;
+    if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+    ExtractRun run;
+    if ((rc = extract_candidates(ctx, p, d_frames, nframes, start_frame, nullptr, cap, s, run))) return rc;
+    if (cap > 0) {
+        const int cut_blocks = cap < 16384 ? cap : 16384;
+        hipLaunchKernelGGL(cut_kernel, dim3(cut_blocks), dim3(64), 0, s, d_frames, run.c, run.c.R, run.mark, run.info, run.xyb, nullptr,
+                           nullptr, d_rois);
+        QCS_HIP(hipGetLastError());
+        rc = wle_launch_extract(p, d_rois, run.xyb, run.c.B, run.c.H, run.info, cap, d_wle, s);
         if (rc) return rc;
     }
     if ((rc = timer_end(ctx->timer, TM_EXTRACT, s, ev, nframes))) return rc;
     int *h = ctx->pin_cnt.as<int>();
-    QCS_HIP(hipMemcpyAsync(h, info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipMemcpyAsync(h, run.info, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     QCS_HIP(hipStreamSynchronize(s));
     if (n_out) *n_out = h[0];
     if (n_found) *n_found = h[1];
     if (h[1] > h[0]) return fail(QCS_ERR_CAPACITY, "%d candidates found, %d kept (capacity)", h[1], h[0]);
+    return QCS_OK;
+}
+
+extern "C" int qcs_extract_mixed_dev(qcs_ctx *ctx, const uint16_t *d_frames, int nframes, int start_frame, int cap,
+                                     const qcs_roi_set *sets, int *n_counts, void *stream)
+{
+    if (!ctx || !sets || nframes < 0 || cap < 0 || (nframes > 0 && !d_frames)) return fail(QCS_ERR_ARG, "invalid argument");
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    if (nframes == 0) {
+        for (int k = 0; k < 3; k++) {
+            if (!sets[k].d_n) return fail(QCS_ERR_ARG, "set %d: a device count is required", k);
+            fill_async(sets[k].d_n, 1, 0, s);
+        }
+        QCS_HIP(hipGetLastError());
+    } else {
+        hipEvent_t ev;
+        if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
+        int *lists[3], *info = nullptr;
+        if ((rc = extract_mixed_launch(ctx, d_frames, nframes, start_frame, nullptr, cap, sets, lists, &info, s))) return rc;
+        if ((rc = timer_end(ctx->timer, TM_EXTRACT, s, ev, nframes))) return rc;
+    }
+    if (!n_counts) return QCS_OK;
+    int h[3];
+    for (int k = 0; k < 3; k++) QCS_HIP(hipMemcpyAsync(&h[k], sets[k].d_n, sizeof(int), hipMemcpyDeviceToHost, s));
+    QCS_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 3; k++) n_counts[k] = h[k];
     return QCS_OK;
 }
 

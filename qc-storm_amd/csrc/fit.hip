@@ -603,7 +603,9 @@ static int launch_1e_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, co
     case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
     case 11: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 11, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
     case 13: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 13, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the fitter (5..13)", R);
+    case 15: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 15, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 17: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 17, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the fitter (odd, 5..17)", R);
     }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
@@ -634,7 +636,7 @@ static int launch_me_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, co
     case 9: hipLaunchKernelGGL((fit_me_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
     case 11: hipLaunchKernelGGL((fit_me_kernel<MODEL, 11, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
     case 13: hipLaunchKernelGGL((fit_me_kernel<MODEL, 13, FAST>), grid, dim3(64), 0, s, rois, wle, list, n_dev, loc, second, fs, ff, fc); break;
-    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the multi-emitter fitter (5..13)", R);
+    default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the multi-emitter fitter (odd, 5..13)", R);
     }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
@@ -740,27 +742,48 @@ __global__ void clear_tail_kernel(int *__restrict__ f, int n, const int *__restr
     if (j < n && j >= *n_list) f[j] = 0;
 }
 
-int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc, int loc_cap, int *n_rows,
-                hipStream_t s)
+size_t cascade_scratch_bytes(int n)
 {
-    const qcs_params &p = c->p;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t ni = (size_t)(n > 0 ? n : 1);
+    return up(64) + 3 * up(sizeof(int) * ni) + up(sizeof(int) * (7 * ni + scan_tile_count(ni) + 1)) +
+           up(sizeof(float) * QCS_REC_FLOATS * 2 * ni);
+}
+
+void cascade_carve(void *base, int n, CascadeBufs &b)
+{
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t ni = (size_t)(n > 0 ? n : 1);
+    char *p = static_cast<char *>(base);
+    b.cnt = reinterpret_cast<int *>(p);
+    p += up(64);
+    b.list1 = reinterpret_cast<int *>(p);
+    p += up(sizeof(int) * ni);
+    b.list2 = reinterpret_cast<int *>(p);
+    p += up(sizeof(int) * ni);
+    b.list3 = reinterpret_cast<int *>(p);
+    p += up(sizeof(int) * ni);
+    b.flag = reinterpret_cast<int *>(p);
+    p += up(sizeof(int) * (7 * ni + scan_tile_count(ni) + 1));
+    b.rows = reinterpret_cast<float *>(p);
+}
+
+int fit_cascade_launch(const qcs_params &p, Timer *tm, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc,
+                       int loc_cap, const CascadeBufs &b, hipStream_t s)
+{
     const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
     const int ntiles = scan_tile_count((size_t)n);
     int rc;
-    // scratch: lists (3 x n), flags (6 x n), offsets (n), tile sums, seconds (2 x n rows), counters
-    if ((rc = c->fit_idx1.ensure(sizeof(int) * (size_t)n)) || (rc = c->fit_idx2.ensure(sizeof(int) * (size_t)n)) ||
-        (rc = c->fit_idx3.ensure(sizeof(int) * (size_t)n)) ||
-        (rc = c->fit_flag.ensure(sizeof(int) * ((size_t)n * 7 + ntiles + 1))) ||
-        (rc = c->fit_rows.ensure(sizeof(float) * QCS_REC_FLOATS * 2 * (size_t)n)) || (rc = c->fit_cnt.ensure(64)) ||
-        (rc = c->pin_cnt.ensure(64)))
-        return rc;
-    int *list1 = c->fit_idx1.as<int>(), *list2 = c->fit_idx2.as<int>(), *list3 = c->fit_idx3.as<int>();
-    int *f1 = c->fit_flag.as<int>(), *f2 = f1 + n, *rr = f2 + n, *fs2 = rr + n, *ff2 = fs2 + n, *fs3 = ff2 + n;
+    int *list1 = b.list1, *list2 = b.list2, *list3 = b.list3;
+    int *f1 = b.flag, *f2 = f1 + n, *rr = f2 + n, *fs2 = rr + n, *ff2 = fs2 + n, *fs3 = ff2 + n;
     int *offs = fs3 + n, *tsum = offs + n;
     int *ff3 = f1;   // f1 is free again once the 1E list is built
-    float *sec2 = c->fit_rows.as<float>(), *sec3 = sec2 + (size_t)QCS_REC_FLOATS * n;
-    int *cnt = c->fit_cnt.as<int>();
+    float *sec2 = b.rows, *sec3 = sec2 + (size_t)QCS_REC_FLOATS * n;
+    int *cnt = b.cnt;
     const dim3 g((n + 255) / 256);
+    // stage timers are recorded only when a timer is given and enabled
+    auto tb = [&](hipEvent_t *ev) { *ev = nullptr; return tm ? timer_begin(*tm, s, ev) : QCS_OK; };
+    auto te = [&](int cat, hipEvent_t ev) { return tm ? timer_end(*tm, cat, s, ev, n) : QCS_OK; };
 
     fill_async(cnt, 16, 0, s);
     hipLaunchKernelGGL(cascade_set_kernel, dim3(1), dim3(1), 0, s, cnt + 8, loc_cap);
@@ -779,18 +802,18 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 0);
     // one-emitter stage; rejected fits with SNR > 6 flag themselves in rr
     hipEvent_t ev;
-    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
+    if ((rc = tb(&ev))) return rc;
     if ((rc = fit_launch_1e(p, d_rois, d_wle, list1, n, cnt + 0, d_loc, rr, s))) return rc;
-    if ((rc = timer_end(c->timer, TM_FIT1E, s, ev, n))) return rc;
+    if ((rc = te(TM_FIT1E, ev))) return rc;
     // re-routed ROIs join the 2E list after the classified ones, in index order
     if ((rc = scan_exclusive(rr, n, offs, tsum, s)) || (rc = compact_scatter(rr, offs, n, nullptr, cnt + 1, 0, list2, s)))
         return rc;
     hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 2, 1);
     // two-emitter stage
-    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
+    if ((rc = tb(&ev))) return rc;
     if ((rc = fit_launch_me(p, as3d ? MODEL_AS3D2E : MODEL_2E, d_rois, d_wle, list2, n, cnt + 2, d_loc, sec2, fs2, ff2, s)))
         return rc;
-    if ((rc = timer_end(c->timer, TM_FITME, s, ev, n))) return rc;
+    if ((rc = te(TM_FITME, ev))) return rc;
     hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs2, n, cnt + 2);
     hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff2, n, cnt + 2);
     if ((rc = scan_exclusive(fs2, n, offs, tsum, s))) return rc;
@@ -803,9 +826,9 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
         if ((rc = compact_scatter(ff2, offs, n, list2, nullptr, 0, list3, s))) return rc;
         hipLaunchKernelGGL(cascade_count_kernel, dim3(1), dim3(1), 0, s, tsum, ntiles, cnt, 3, 0);
         fill_async(ff3, n, 0, s);
-        if ((rc = timer_begin(c->timer, s, &ev))) return rc;
+        if ((rc = tb(&ev))) return rc;
         if ((rc = fit_launch_me(p, MODEL_3E, d_rois, d_wle, list3, n, cnt + 3, d_loc, sec3, fs3, ff3, s))) return rc;
-        if ((rc = timer_end(c->timer, TM_FITME, s, ev, n))) return rc;
+        if ((rc = te(TM_FITME, ev))) return rc;
         hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, fs3, n, cnt + 3);
         hipLaunchKernelGGL(clear_tail_kernel, g, dim3(256), 0, s, ff3, n, cnt + 3);
         if ((rc = scan_exclusive(fs3, n, offs, tsum, s))) return rc;
@@ -816,9 +839,22 @@ int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     }
     hipLaunchKernelGGL(cascade_rows_kernel, dim3(1), dim3(1), 0, s, n, cnt);
     QCS_HIP(hipGetLastError());
-    if ((rc = timer_begin(c->timer, s, &ev))) return rc;
+    if ((rc = tb(&ev))) return rc;
     if ((rc = crlb_launch(p, d_loc, loc_cap, cnt + 7, s))) return rc;
-    if ((rc = timer_end(c->timer, TM_CRLB, s, ev, n))) return rc;
+    return te(TM_CRLB, ev);
+}
+
+int fit_cascade(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc, int loc_cap, int *n_rows,
+                hipStream_t s)
+{
+    const qcs_params &p = c->p;
+    const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
+    int rc;
+    if ((rc = c->fit_flag.ensure(cascade_scratch_bytes(n))) || (rc = c->pin_cnt.ensure(64))) return rc;
+    CascadeBufs b;
+    cascade_carve(c->fit_flag.ptr, n, b);
+    if ((rc = fit_cascade_launch(p, &c->timer, d_rois, d_wle, n, d_loc, loc_cap, b, s))) return rc;
+    int *cnt = b.cnt;
     int *h = c->pin_cnt.as<int>();
     QCS_HIP(hipMemcpyAsync(h, cnt, 16 * sizeof(int), hipMemcpyDeviceToHost, s));
     QCS_HIP(hipStreamSynchronize(s));

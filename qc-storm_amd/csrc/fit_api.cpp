@@ -22,6 +22,10 @@ int qcs_fit_dev(qcs_ctx *c, const uint16_t *d_rois, const float *d_wle, int n, f
     if (n == 0) return QCS_OK;
     if (loc_cap < n) return fail(QCS_ERR_CAPACITY, "loc_cap (%d) < number of ROIs (%d)", loc_cap, n);
     if (!d_wle) return fail(QCS_ERR_ARG, "WLE records are required (qcs_wle_dev computes them)");
+    // the reference instantiates its 2- and 3-emitter kernels for ROI sizes 5..13 only
+    // (bfgsMLE_Multi_Emitter_Fit.cu:44-75: "shared memory is not enough" for 15 and 17)
+    if (c->p.MultiEmitterFitEn && c->p.ROISize > 13)
+        return fail(QCS_ERR_ARG, "multi-emitter fitting supports ROISize 5..13 (got %d)", c->p.ROISize);
     hipStream_t s = (hipStream_t)stream;
     if (!c->p.MultiEmitterFitEn) {
         hipEvent_t ev;

@@ -9,14 +9,13 @@
 //
 // Sorted compaction: the reference launches one kernel per frame and appends with an atomic
 // counter (within-frame order arbitrary, O(frames x rows) work).  Here the kept rows are compacted
-// in row order (scan), then stably radix-sorted by frame (hipCUB), so rows of one frame keep their
+// in row order (scan), then stably radix-sorted by frame (sort.hip), so rows of one frame keep their
 // original order.  Rendering splats each localization's 7 x 7 Gaussian with float atomics, as the
 // reference does, but into LDS: localizations are binned by 64 x 64 tile of the SR image (a
 // footprint that straddles a tile edge is listed in each tile it touches), one work-group
 // accumulates a tile with LDS atomics and then adds it to the image with one coalesced
 // read-modify-write.  Scattered atomics into a 1.7 GB image would miss the caches on nearly
 // every update.  The PSF uses the deterministic exponential.
-#include <hipcub/hipcub.hpp>
 #include <string.h>
 #include "qcs_internal.h"
 #include "qcs_devmath.h"
@@ -292,13 +291,10 @@ extern "C" int qcs_compact_sorted_dev(qcs_ctx *c, const float *d_loc, int n, int
     if (n == 0 || end < first) return QCS_OK;
     hipStream_t s = (hipStream_t)stream;
     const int ntiles = scan_tile_count((size_t)n);
-    // flag, offsets, tile sums, keys in/out, indices
-    size_t temp = 0;
-    QCS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned *)nullptr, (unsigned *)nullptr,
-                                               (const int *)nullptr, (int *)nullptr, n, 0, 32, s));
+    // flag, offsets, tile sums, keys in/out, indices; sort workspace
     const size_t words = (size_t)n * 5 + ntiles + 1;
     int rc;
-    if ((rc = c->fit_flag.ensure(sizeof(int) * words)) || (rc = c->fit_rows.ensure(temp + 256)) ||
+    if ((rc = c->fit_flag.ensure(sizeof(int) * words)) || (rc = c->fit_rows.ensure(sizeof(int) * radix_sort_tmp_ints(n))) ||
         (rc = c->fit_idx1.ensure(sizeof(int) * (size_t)n)) || (rc = c->fit_cnt.ensure(64)) || (rc = c->pin_cnt.ensure(64)))
         return rc;
     int *flag = c->fit_flag.as<int>(), *offs = flag + n, *tsum = offs + n;
@@ -319,12 +315,11 @@ extern "C" int qcs_compact_sorted_dev(qcs_ctx *c, const float *d_loc, int n, int
     if (m > 0) {
         int bits = 1;
         while (bits < 32 && ((unsigned)(end - first) >> bits) != 0) bits++;
-        size_t tb = 0;
-        QCS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, iin, iout, m, 0, bits, s));
-        if (tb > c->fit_rows.bytes) return fail(QCS_ERR_NOMEM, "radix sort scratch larger than reserved");
-        QCS_HIP(hipcub::DeviceRadixSort::SortPairs(c->fit_rows.ptr, tb, kin, kout, iin, iout, m, 0, bits, s));
-        QCS_HIP(hipMemcpyAsync(c->fit_cnt.ptr, &h[0], sizeof(int), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(copy_rows_kernel, dim3((m + 255) / 256), dim3(256), 0, s, d_loc, iout, c->fit_cnt.as<int>(), d_out);
+        unsigned *kres;
+        int *ires;
+        if ((rc = radix_sort_pairs_u32(kin, iin, kout, iout, m, bits, c->fit_rows.as<int>(), s, &kres, &ires))) return rc;
+        fill_async(c->fit_cnt.as<int>(), 1, m, s);
+        hipLaunchKernelGGL(copy_rows_kernel, dim3((m + 255) / 256), dim3(256), 0, s, d_loc, ires, c->fit_cnt.as<int>(), d_out);
         QCS_HIP(hipGetLastError());
     }
     if ((rc = timer_end(c->timer, TM_COMPACT, s, ev, n))) return rc;

@@ -73,6 +73,20 @@ int fit_launch_me(const qcs_params &p, int model, const uint16_t *d_rois, const 
                   int n_max, const int *d_n, float *d_loc, float *d_second, int *d_fs, int *d_ff, hipStream_t s);
 int crlb_launch(const qcs_params &p, float *d_loc, int n, const int *d_n, hipStream_t s);
 
+// Scratch of the multi-emitter cascade (fit.hip), carved from one block of
+// cascade_scratch_bytes(n) bytes: counters (16 ints; [7] = rows written once the launches end),
+// the 1E / 2E / 3E lists, flags and scan offsets, and the second-emitter rows of each stage.
+struct CascadeBufs {
+    int *cnt, *list1, *list2, *list3, *flag;
+    float *rows;
+};
+size_t cascade_scratch_bytes(int n);
+void cascade_carve(void *base, int n, CascadeBufs &b);
+// Launch the whole cascade (fits, list building, appends, CRLB) on stream s without waiting;
+// stage intervals go to *tm when it is given and enabled.
+int fit_cascade_launch(const qcs_params &p, Timer *tm, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc,
+                       int loc_cap, const CascadeBufs &b, hipStream_t s);
+
 // rendering helpers (post.hip): splat rows into the SR image(s); max of (int) image value into
 // *d_out (which must start at 0)
 int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);
@@ -86,6 +100,17 @@ void fill_async(int *p, size_t n, int v, hipStream_t s);
 int compact_scatter(const int *flags, const int *offs, int n, const int *src, const int *base_dev, int base_const,
                     int *out, hipStream_t s);
 
+// stable LSD radix sort of (key, int) pairs by the low `bits` key bits (sort.hip): k0/v0 hold the
+// input, k1/v1 the same sizes; *kres/*vres receive whichever pair holds the result.  tmp holds
+// radix_sort_tmp_ints(n) ints.  Inclusive prefix maximum of n ints (tmp: max_scan_tmp_ints(n)).
+size_t radix_sort_tmp_ints(int n);
+int radix_sort_pairs_u32(unsigned *k0, int *v0, unsigned *k1, int *v1, int n, int bits, int *tmp, hipStream_t s,
+                         unsigned **kres, int **vres);
+int radix_sort_pairs_u64(unsigned long long *k0, int *v0, unsigned long long *k1, int *v1, int n, int bits, int *tmp,
+                         hipStream_t s, unsigned long long **kres, int **vres);
+size_t max_scan_tmp_ints(int n);
+int scan_inclusive_max(const int *in, int n, int *out, int *tmp, hipStream_t s);
+
 }  // namespace qcs
 
 struct qcs_ctx {
@@ -96,9 +121,11 @@ struct qcs_ctx {
     // extraction scratch
     qcs::DevBuf ex_bkg, ex_tmp, ex_det, ex_mark, ex_cnt, ex_xy, ex_var, ex_frames;
     // fit scratch
-    qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;
+    qcs::DevBuf fit_idx1, fit_idx2, fit_idx3, fit_flag, fit_cnt, fit_rows;   // fit_flag: the cascade's carve
     // converged fitting scratch
     qcs::DevBuf cf_theta, cf_ll, cf_list, cf_rows, cf_queue;
+    // mixed-size extraction (7 x 7 cuts, WLE, size flags, set lists) and the frames path's sets
+    qcs::DevBuf mx_rois, mx_list, mx_sets, mx_rows;
     // rendering scratch (tile bins), statistics scratch
     qcs::DevBuf rn_buf, st_buf;
     qcs::DevBuf dr_img, dr_cc, dr_shift, dr_frames;

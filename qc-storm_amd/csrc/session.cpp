@@ -276,6 +276,8 @@ int qcs_session_create(const qcs_params *pin, qcs_session **out)
     if (rc) return rc;
     if (p.ImageWidth <= 0 || p.ImageHigh <= 0) return fail(QCS_ERR_ARG, "ImageWidth/ImageHigh not set");
     if (p.LocType == QCS_LOCTYPE_DH3D) return fail(QCS_ERR_ARG, "LocType DH3D is not supported");
+    if (p.MultiEmitterFitEn && p.ROISize > 13)
+        return fail(QCS_ERR_ARG, "multi-emitter fitting supports ROISize 5..13 (got %d)", p.ROISize);
     if (p.SRImageWidth <= 16 || p.SRImageHigh <= 16) qcs_params_update_sr_size(&p);
     qcs_session *s = new (std::nothrow) qcs_session();
     if (!s) return fail(QCS_ERR_NOMEM, "session allocation failed");

@@ -26,7 +26,8 @@ __global__ void sim_splat_kernel(qcs_sim_params sp, uint64_t seed, int frame, in
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nem) return;
-    const Emitter e = qcs_sim::frame_emitter(sp, seed, frame, k);
+    Emitter e;
+    if (!qcs_sim::frame_slot_emitter(sp, seed, frame, k, e)) return;
     const double reach = qcs_sim::emitter_reach(e);
     int c0 = (int)ceil(e.x - reach - 0.5), c1 = (int)floor(e.x + reach - 0.5);
     int r0 = (int)ceil(e.y - reach - 0.5), r1 = (int)floor(e.y + reach - 0.5);
@@ -54,6 +55,7 @@ int check_sim(const qcs_sim_params *sp)
     if (!sp) return fail(QCS_ERR_ARG, "null simulator parameters");
     if (sp->width <= 0 || sp->height <= 0) return fail(QCS_ERR_ARG, "simulator size must be positive");
     if (sp->n_per_unit_max > 4) return fail(QCS_ERR_ARG, "n_per_unit_max must be <= 4");
+    if (sp->density_max_per_px < 0.0f) return fail(QCS_ERR_ARG, "density_max_per_px must be >= 0");
     return QCS_OK;
 }
 
@@ -87,6 +89,7 @@ void qcs_sim_params_default(qcs_sim_params *sp)
     sp->n_per_unit_max = 1;
     sp->min_sep = 1.0f;
     sp->max_sep = 3.0f;
+    sp->density_max_per_px = 0.0f;
 }
 
 int qcs_sim_rois_host(const qcs_sim_params *sp, uint64_t seed, int n, int first_frame, uint16_t *rois, float *truth)
@@ -119,15 +122,18 @@ int qcs_sim_frame_emitters(const qcs_sim_params *sp, uint64_t seed, int frame, f
 {
     int rc = check_sim(sp);
     if (rc) return -rc;
-    const int n = qcs_sim::frame_emitter_count(*sp);
-    if (truth) {
-        const double bg = qcs_sim::frame_background(*sp, seed, frame);
-        for (int k = 0; k < n && k < cap; k++) {
-            Emitter e = qcs_sim::frame_emitter(*sp, seed, frame, k);
-            float *t = truth + (size_t)k * QCS_TRUTH_FLOATS;
+    const int slots = qcs_sim::frame_slots(*sp, seed, frame);
+    const double bg = qcs_sim::frame_background(*sp, seed, frame);
+    int n = 0;
+    for (int k = 0; k < slots; k++) {
+        Emitter e;
+        if (!qcs_sim::frame_slot_emitter(*sp, seed, frame, k, e)) continue;
+        if (truth && n < cap) {
+            float *t = truth + (size_t)n * QCS_TRUTH_FLOATS;
             t[0] = (float)frame; t[1] = (float)e.x; t[2] = (float)e.y; t[3] = (float)e.z;
             t[4] = (float)e.n; t[5] = (float)bg; t[6] = (float)e.sx; t[7] = (float)e.sy;
         }
+        n++;
     }
     return n;
 }
@@ -138,13 +144,14 @@ int qcs_sim_frames_host(const qcs_sim_params *sp, uint64_t seed, int first_frame
     if (rc) return rc;
     if (nframes < 0 || (!frames && nframes > 0)) return fail(QCS_ERR_ARG, "invalid frame buffer");
     const size_t npx = (size_t)sp->width * sp->height;
-    const int nem = qcs_sim::frame_emitter_count(*sp);
     std::vector<double> img(npx);
     for (int f = 0; f < nframes; f++) {
         const int frame = first_frame + f;
+        const int nem = qcs_sim::frame_slots(*sp, seed, frame);
         std::fill(img.begin(), img.end(), 0.0);
         for (int k = 0; k < nem; k++) {
-            const Emitter e = qcs_sim::frame_emitter(*sp, seed, frame, k);
+            Emitter e;
+            if (!qcs_sim::frame_slot_emitter(*sp, seed, frame, k, e)) continue;
             const double reach = qcs_sim::emitter_reach(e);
             int c0 = (int)ceil(e.x - reach - 0.5), c1 = (int)floor(e.x + reach - 0.5);
             int r0 = (int)ceil(e.y - reach - 0.5), r1 = (int)floor(e.y + reach - 0.5);
@@ -175,11 +182,11 @@ int qcs_sim_frames_dev(const qcs_sim_params *sp, uint64_t seed, int first_frame,
     if (nframes <= 0) return QCS_OK;
     hipStream_t s = (hipStream_t)stream;
     const size_t npx = (size_t)sp->width * sp->height;
-    const int nem = qcs_sim::frame_emitter_count(*sp);
     double *img = nullptr;
     QCS_HIP(hipMallocAsync((void **)&img, npx * sizeof(double), s));
     for (int f = 0; f < nframes; f++) {
         const int frame = first_frame + f;
+        const int nem = qcs_sim::frame_slots(*sp, seed, frame);
         hipError_t e = hipMemsetAsync(img, 0, npx * sizeof(double), s);
         if (e == hipSuccess && nem > 0) {
             hipLaunchKernelGGL(sim_splat_kernel, dim3((nem + 255) / 256), dim3(256), 0, s, *sp, seed, frame, nem, img);

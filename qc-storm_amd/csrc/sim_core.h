@@ -21,7 +21,8 @@
 
 namespace qcs_sim {
 
-enum { TAG_ROI_PARAM = 1, TAG_ROI_PIX = 2, TAG_FRAME_EMIT = 3, TAG_FRAME_PIX = 4, TAG_FRAME_BG = 5 };
+enum { TAG_ROI_PARAM = 1, TAG_ROI_PIX = 2, TAG_FRAME_EMIT = 3, TAG_FRAME_PIX = 4, TAG_FRAME_BG = 5, TAG_FRAME_SITE = 6,
+       TAG_FRAME_MEMBER = 7, TAG_FRAME_DENSITY = 8 };
 
 QCS_HD inline void philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
 {
@@ -163,6 +164,64 @@ QCS_HD inline double frame_background(const qcs_sim_params &sp, uint64_t seed, i
 QCS_HD inline int frame_emitter_count(const qcs_sim_params &sp)
 {
     return (int)floor((double)sp.density_per_px * sp.width * sp.height + 0.5);
+}
+
+// Frames with clusters (n_per_unit_max = M > 1) or a density range (density_max_per_px >
+// density_per_px): the frame's density is uniform in [density_per_px, density_max_per_px];
+// emitters come in sites of 1..M members (uniform count), the first uniform over the frame, the
+// others at a separation in [min_sep, max_sep] from it.  Slot k = site * M + member; a slot
+// beyond its site's count is empty.  With M = 1 and no range this is frame_emitter.
+QCS_HD inline int frame_members(const qcs_sim_params &sp) { return sp.n_per_unit_max > 1 ? sp.n_per_unit_max : 1; }
+
+QCS_HD inline bool frame_plain(const qcs_sim_params &sp)
+{
+    return frame_members(sp) == 1 && !(sp.density_max_per_px > sp.density_per_px);
+}
+
+QCS_HD inline int frame_slots(const qcs_sim_params &sp, uint64_t seed, int frame)
+{
+    if (frame_plain(sp)) return frame_emitter_count(sp);
+    double dens = sp.density_per_px;
+    if (sp.density_max_per_px > sp.density_per_px) {
+        Gen g(seed, (uint32_t)frame, 0u, TAG_FRAME_DENSITY);
+        dens += g.u01() * ((double)sp.density_max_per_px - sp.density_per_px);
+    }
+    const int M = frame_members(sp);
+    const int sites = (int)floor(dens * sp.width * sp.height / (0.5 * (M + 1)) + 0.5);
+    return sites * M;
+}
+
+QCS_HD inline bool frame_slot_emitter(const qcs_sim_params &sp, uint64_t seed, int frame, int k, Emitter &e)
+{
+    if (frame_plain(sp)) {
+        e = frame_emitter(sp, seed, frame, k);
+        return true;
+    }
+    const int M = frame_members(sp), site = k / M, m = k - site * M;
+    Gen g(seed, (uint32_t)frame, (uint32_t)site, TAG_FRAME_SITE);
+    int K = 1 + (int)(g.u01() * M);
+    K = K > M ? M : K;
+    const double x0 = g.u01() * sp.width, y0 = g.u01() * sp.height;
+    if (m >= K) return false;
+    Gen gm(seed, (uint32_t)frame, (uint32_t)k, TAG_FRAME_MEMBER);
+    if (m == 0) {
+        e.x = x0;
+        e.y = y0;
+    } else {
+        const double ang = 6.283185307179586476925 * gm.u01();
+        const double sep = sp.min_sep + gm.u01() * (sp.max_sep - sp.min_sep);
+        e.x = x0 + sep * cos(ang);
+        e.y = y0 + sep * sin(ang);
+    }
+    e.n = log_uniform(gm, sp.photons_min, sp.photons_max);
+    const double s = sp.sigma_min + gm.u01() * (sp.sigma_max - sp.sigma_min);
+    e.z = 0.0;
+    e.sx = e.sy = s;
+    if (sp.astig) {
+        e.z = sp.z_min + gm.u01() * (sp.z_max - sp.z_min);
+        astig_sigma(sp, e.z, &e.sx, &e.sy);
+    }
+    return true;
 }
 
 // Half-width (pixels) of the window an emitter contributes to.

@@ -15,7 +15,6 @@
 // of all frames are sorted once by (frame, cell of the neighbour radius) and each scans the 3x3
 // cells around its own, which visits every pair closer than the radius: the counts are the
 // same integers.
-#include <hipcub/hipcub.hpp>
 #include <limits.h>
 #include <stddef.h>
 #include <algorithm>
@@ -234,9 +233,7 @@ extern "C" int qcs_loc_stats_dev(qcs_ctx *ctx, const float *d_loc, int n, qcs_lo
     hipStream_t s = (hipStream_t)stream;
     const qcs_params &p = ctx->p;
     // scratch: accumulator, keys and values (double-buffered for the sort), sort workspace
-    size_t temp = 0;
-    QCS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (const unsigned long long *)nullptr, (unsigned long long *)nullptr,
-                                               (const int *)nullptr, (int *)nullptr, n, 0, 64, s));
+    const size_t temp = sizeof(int) * radix_sort_tmp_ints(n);
     const size_t acc_b = (sizeof(StatAcc) + 255) & ~(size_t)255, keys_b = ((size_t)n * 8 + 255) & ~(size_t)255,
                  vals_b = ((size_t)n * 4 + 255) & ~(size_t)255;
     int rc;
@@ -298,12 +295,15 @@ extern "C" int qcs_loc_stats_dev(qcs_ctx *ctx, const float *d_loc, int n, qcs_lo
     const dim3 g((n + 255) / 256), blk(256);
     hipLaunchKernelGGL(stat_keys_kernel, g, blk, 0, s, d_loc, n, fs, fe, p.ImageWidth, p.ImageHigh, cell, ncx, ncy, none, k_in, v_in, acc);
     QCS_HIP(hipGetLastError());
-    QCS_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, temp, k_in, k_out, v_in, v_out, n, 0, bits, s));
+    unsigned long long *k_sorted;
+    int *v_sorted;
+    if ((rc = radix_sort_pairs_u64(k_in, v_in, k_out, v_out, n, bits, static_cast<int *>(tmp), s, &k_sorted, &v_sorted))) return rc;
     int kept = 0;
     QCS_HIP(hipMemcpyAsync(&kept, &acc->kept, sizeof(int), hipMemcpyDeviceToHost, s));
     QCS_HIP(hipStreamSynchronize(s));
     if (kept > 0) {
-        hipLaunchKernelGGL(stat_neigh_kernel, dim3((kept + 255) / 256), blk, 0, s, d_loc, k_out, v_out, kept, ncx, ncy, rad * rad, acc);
+        hipLaunchKernelGGL(stat_neigh_kernel, dim3((kept + 255) / 256), blk, 0, s, d_loc, k_sorted, v_sorted, kept, ncx, ncy, rad * rad,
+                           acc);
         QCS_HIP(hipGetLastError());
     }
     int cnt[2] = {0, 0};

@@ -57,6 +57,7 @@ class QcsSimParams(C.Structure):
         ("bg_min", C.c_float), ("bg_max", C.c_float), ("density_per_px", C.c_float), ("center_jitter", C.c_float),
         ("astig", C.c_int), ("z_min", C.c_float), ("z_max", C.c_float), ("s0", C.c_float), ("c_nm", C.c_float),
         ("d_nm", C.c_float), ("n_per_unit_max", C.c_int), ("min_sep", C.c_float), ("max_sep", C.c_float),
+        ("density_max_per_px", C.c_float),
     ]
 
 
@@ -170,7 +171,7 @@ class Consec:
 
 class QcsRoiSet(C.Structure):
     _fields_ = [("roi_size", C.c_int), ("n", C.c_int), ("d_rois", C.c_void_p), ("d_loc", C.c_void_p),
-                ("d_sel", C.c_void_p)]
+                ("d_sel", C.c_void_p), ("d_n", C.c_void_p)]
 
 
 class QcsTiming(C.Structure):
@@ -194,6 +195,7 @@ PROTOTYPES = {
     "qcs_last_error": (C.c_char_p, []),
     "qcs_version": (C.c_char_p, []),
     "qcs_device_count": (_i, []),
+    "qcs_test_fail_alloc_after": (None, [_i]),
     "qcs_ctx_create": (_i, [_P(QcsParams), _P(_vp)]),
     "qcs_ctx_set_params": (_i, [_vp, _P(QcsParams)]),
     "qcs_ctx_destroy": (_i, [_vp]),
@@ -211,6 +213,8 @@ PROTOTYPES = {
     "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "qcs_fit_mle_mixed_dev": (_i, [_vp, _P(QcsRoiSet), _i, _i, _vp]),
+    "qcs_extract_mixed_dev": (_i, [_vp, _vp, _i, _i, _i, _P(QcsRoiSet), _vp, _vp]),
+    "qcs_loc_mle_frames_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _i, _vp, _vp]),
     "qcs_fit_plain_stats_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _P(C.c_longlong), _vp]),
     "qcs_graph_create_mle": (_i, [_vp, _vp, _vp, _i, _i, _vp, _vp, _vp, _P(_vp)]),
     "qcs_drift_correct_dev": (_i, [_vp, _vp, _ll, _i, _vp, _vp, _i, _P(_i), _vp, _vp]),
@@ -220,6 +224,7 @@ PROTOTYPES = {
     "qcs_consec_reset": (_i, [_vp]),
     "qcs_consec_destroy": (_i, [_vp]),
     "qcs_graph_launch": (_i, [_vp, _vp]),
+    "qcs_graph_create_frames": (_i, [_vp, _vp, _i, _vp, _i, _i, _vp, _i, _vp, _vp, _P(_vp)]),
     "qcs_loc_stats_dev": (_i, [_vp, _vp, _i, _P(QcsLocStats), _vp]),
     "qcs_autotune_mle_dev": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _P(QcsAutotuneOpts), _P(QcsAutotuneEntry), _i, _P(_i), _vp]),
     "qcs_graph_destroy": (_i, [_vp]),
@@ -440,11 +445,35 @@ class Context:
               "qcs_fit_mle_dev")
 
     def fit_mle_mixed_dev(self, sets, kmax, stream=None):
-        """sets: list of (roi_size, d_rois, n, d_loc, d_sel or None)."""
+        """sets: list of (roi_size, d_rois, n, d_loc, d_sel or None[, d_n or None])."""
         arr = (QcsRoiSet * len(sets))()
-        for i, (R, d_rois, n, d_loc, d_sel) in enumerate(sets):
-            arr[i] = QcsRoiSet(R, n, d_rois.data_ptr(), d_loc.data_ptr(), d_sel.data_ptr() if d_sel is not None else None)
+        for i, st in enumerate(sets):
+            R, d_rois, n, d_loc, d_sel = st[:5]
+            d_n = st[5] if len(st) > 5 else None
+            arr[i] = QcsRoiSet(R, n, d_rois.data_ptr(), d_loc.data_ptr(), d_sel.data_ptr() if d_sel is not None else None,
+                               d_n.data_ptr() if d_n is not None else None)
         check(lib().qcs_fit_mle_mixed_dev(self.h, arr, len(sets), kmax, stream), "qcs_fit_mle_mixed_dev")
+
+    def extract_mixed_dev(self, d_frames, nframes, start_frame, cap, set_caps, stream=None):
+        """Mixed-size extraction into new device sets; returns ([(R, d_rois, count, d_n)] for R = 7, 9, 11)."""
+        import torch
+        dev = d_frames.device
+        arr = (QcsRoiSet * 3)()
+        outs = []
+        for k, R in enumerate((7, 9, 11)):
+            c = int(set_caps[k])
+            d_rois = torch.zeros((max(c, 1), roi_words(R)), dtype=torch.int16, device=dev)
+            d_n = torch.full((1,), -1, dtype=torch.int32, device=dev)
+            arr[k] = QcsRoiSet(R, c, d_rois.data_ptr(), None, None, d_n.data_ptr())
+            outs.append((R, d_rois, d_n))
+        cnt = (C.c_int * 3)()
+        check(lib().qcs_extract_mixed_dev(self.h, self._dp(d_frames), nframes, start_frame, cap, arr, cnt, stream),
+              "qcs_extract_mixed_dev")
+        return [(R, d_rois, cnt[k], d_n) for k, (R, d_rois, d_n) in enumerate(outs)]
+
+    def loc_mle_frames_dev(self, d_frames, nframes, start_frame, cap, kmax, d_out, d_n_out, stream=None):
+        check(lib().qcs_loc_mle_frames_dev(self.h, self._dp(d_frames), nframes, start_frame, cap, kmax, self._dp(d_out),
+                                           d_out.shape[0], self._dp(d_n_out), stream), "qcs_loc_mle_frames_dev")
 
     def tuning(self):
         t = QcsTuning()
@@ -510,6 +539,15 @@ class Context:
         st = QcsLocStats()
         check(lib().qcs_loc_stats_dev(self.h, self._dp(d_loc), n, C.byref(st), stream), "qcs_loc_stats_dev")
         return loc_stats_dict(st)
+
+    def graph_frames(self, d_frames, nframes, d_start_frame, cap, kmax, d_out, d_n_out, stream=None):
+        """A captured frame batch (qcs_loc_mle_frames_dev over fixed buffers, first frame number read
+        from the device int d_start_frame at each launch); returns a Graph."""
+        h = C.c_void_p()
+        check(lib().qcs_graph_create_frames(self.h, self._dp(d_frames), nframes, self._dp(d_start_frame), cap, kmax,
+                                            self._dp(d_out), d_out.shape[0], self._dp(d_n_out), stream, C.byref(h)),
+              "qcs_graph_create_frames")
+        return Graph(h)
 
     def graph_mle(self, d_rois, n, kmax, d_loc, d_sel=None, net=None, stream=None):
         """A captured multi-emitter batch (plain, or warm with `net`); returns a Graph."""

@@ -45,6 +45,8 @@ PROTOS = {
     "orc_classify_rois": (None, [_vp, _vp, _i, _i, _i, _i, _vp]),
     "orc_cut_rois": (None, [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]),
     "orc_extract": (_i, [_PP, _vp, _i, _i, _vp, _vp, _i, _P(_i), _vp, _vp]),
+    "orc_mixed_size": (_i, [_vp, _i, _i, _i, _i, _vp]),
+    "orc_extract_mixed": (_i, [_PP, _vp, _i, _i, _i, _P(_vp), _P(_i), _P(_i), _vp, _P(_i), _P(_i)]),
     "orc_spline_interp": (None, [_vp, _vp, _i, _vp, _vp, _i]),
     "orc_wle_psf_width": (None, [_vp, _i, _i, _vp]),
     "orc_wle_nearest": (None, [_vp, _i, _i, _vp]),
@@ -207,6 +209,27 @@ def extract(params, frames, start_frame=1, cap=200000):
     n = lib().orc_extract(C.byref(params), p(frames), frames.shape[0], start_frame, p(rois), p(w), cap,
                           C.byref(found), None, p(var))
     return rois[:n].copy(), w[:n].copy(), found.value, var
+
+
+def mixed_size(mark, h, x, y, wle7):
+    """ROI size of the candidate at (x, y) of the stacked mark image (uint8, rows x w)."""
+    mark = np.ascontiguousarray(mark, np.uint8)
+    w7 = np.ascontiguousarray(wle7, np.float32)
+    return lib().orc_mixed_size(p(mark), mark.shape[1], h, x, y, p(w7))
+
+
+def extract_mixed(params, frames, start_frame=1, cand_cap=200000, caps=(200000, 200000, 200000)):
+    """Mixed-size extraction: ([rois7, rois9, rois11], per-candidate sizes, candidates, found)."""
+    frames = np.ascontiguousarray(frames, np.uint16)
+    bufs = [np.zeros((max(caps[k], 1), Q.roi_words(7 + 2 * k)), np.uint16) for k in range(3)]
+    ptrs = (C.c_void_p * 3)(*[b.ctypes.data for b in bufs])
+    capa = (C.c_int * 3)(*caps)
+    cnt = (C.c_int * 3)()
+    sizes = np.zeros(max(cand_cap, 1), np.int32)
+    nc, nf = C.c_int(), C.c_int()
+    lib().orc_extract_mixed(C.byref(params), p(frames), frames.shape[0], start_frame, cand_cap, ptrs, capa, cnt, p(sizes),
+                            C.byref(nc), C.byref(nf))
+    return [bufs[k][:cnt[k]].copy() for k in range(3)], sizes[:nc.value].copy(), nc.value, nf.value
 
 
 def crlb(params, loc):

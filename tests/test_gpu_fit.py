@@ -14,7 +14,7 @@ def _config0(n, seed=1000, R=7):
     return rois, truth
 
 
-@pytest.mark.parametrize("R", [7, 5, 9])
+@pytest.mark.parametrize("R", [7, 5, 9, 13, 15, 17])
 def test_gs2d_single_bit_exact(gpu, R):
     rois, _ = _config0(4096, seed=1000 + R, R=R)
     params = Q.default_params(QE=1.0, ROISize=R)
@@ -56,7 +56,7 @@ def _multi_rois(R, n, seed, astig=False):
     return rois, truth
 
 
-@pytest.mark.parametrize("R,lt", [(7, 0), (9, 0), (11, 0), (7, 1), (9, 1)])
+@pytest.mark.parametrize("R,lt", [(7, 0), (9, 0), (11, 0), (7, 1), (9, 1), (5, 0), (13, 0), (5, 1), (13, 1)])
 def test_multi_emitter_cascade_bit_exact(gpu, R, lt):
     rois, _ = _multi_rois(R, 3000, 500 + R + 10 * lt, astig=(lt == 1))
     params = Q.default_params(QE=1.0, ROISize=R, LocType=lt, MultiEmitterFitEn=1)
@@ -76,7 +76,7 @@ def test_multi_emitter_cascade_bit_exact(gpu, R, lt):
     assert len(bad) == 0
 
 
-@pytest.mark.parametrize("R", [7, 9])
+@pytest.mark.parametrize("R", [7, 9, 13, 15, 17])
 def test_as3d_single_bit_exact(gpu, R):
     sp = Q.default_sim(width=R, height=R, astig=1, photons_min=1000, photons_max=8000, bg_min=20, bg_max=80)
     rois, _ = Q.sim_rois_host(sp, 900 + R, 4096)
@@ -87,3 +87,15 @@ def test_as3d_single_bit_exact(gpu, R):
     bad = np.argwhere(got.view(np.uint32) != ref.view(np.uint32))
     print("R", R, "accepted", int((ref[:, 0] > 0).sum()), "mismatch", len(bad))
     assert len(bad) == 0
+
+
+def test_multi_emitter_rejects_sizes_above_13(gpu):
+    """The reference has no 2- and 3-emitter kernels for ROI sizes 15 and 17
+    (bfgsMLE_Multi_Emitter_Fit.cu:44-75); the library refuses them instead of skipping stages."""
+    R = 15
+    sp = Q.default_sim(width=R, height=R)
+    rois, _ = Q.sim_rois_host(sp, 5, 8)
+    params = Q.default_params(QE=1.0, ROISize=R, MultiEmitterFitEn=1)
+    with pytest.raises(Q.QcsError) as e:
+        Q.Context(params).fit_rois(rois, O.wle(rois, R))
+    assert e.value.rc == Q.QCS_ERR_ARG
