@@ -194,6 +194,19 @@ int qcs_wle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float *d_wle, void 
 int qcs_fit_dev(qcs_ctx *ctx, const uint16_t *d_rois, const float *d_wle, int n,
                 float *d_loc, int loc_cap, int *n_rows, void *stream);
 
+/* Allocation-free qcs_fit_dev (no counterpart in the reference).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same rows as qcs_fit_dev on the same inputs; the multi-emitter cascade's
+ * scratch comes from a fixed file-scope device array (14 MiB per GPU, held from the first call on
+ * that GPU until the process ends) instead of the context's buffers.  At most
+ * QCS_NOALLOC_MAX_ROIS ROIs per call (QCS_ERR_CAPACITY beyond).  Nothing is waited for: the row
+ * count goes to the device int d_n_rows (may be NULL), and qcs_fit_stats_get is not updated.
+ * The context must belong to the current device.  Calls that overlap on one GPU share the array
+ * and race: serialise them. */
+#define QCS_NOALLOC_MAX_ROIS QCS_MAX_POINT_NUM
+int qcs_fit_dev_noalloc(qcs_ctx *ctx, const uint16_t *d_rois, const float *d_wle, int n,
+                        float *d_loc, int loc_cap, int *d_n_rows, void *stream);
+
 /* Fit-stage counters of the last qcs_fit_dev call (FitRatio_* of bfgs_top.cu:68-173). */
 typedef struct qcs_fit_stats {
     int   n_in, n_single, n_2e, n_3e, n_rejected, n_rows;

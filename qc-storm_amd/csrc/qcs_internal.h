@@ -87,6 +87,10 @@ void cascade_carve(void *base, int n, CascadeBufs &b);
 int fit_cascade_launch(const qcs_params &p, Timer *tm, const uint16_t *d_rois, const float *d_wle, int n, float *d_loc,
                        int loc_cap, const CascadeBufs &b, hipStream_t s);
 
+// file-scope scratch of the allocation-free entry points (noalloc.hip): resolves the calling
+// device's copy once per device (the address that host code passes to kernels)
+int noalloc_scratch_init(int device);
+
 // rendering helpers (post.hip): splat rows into the SR image(s); max of (int) image value into
 // *d_out (which must start at 0)
 int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);

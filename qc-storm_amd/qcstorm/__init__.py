@@ -25,6 +25,7 @@ REC_FLOATS = 12
 WLE_FLOATS = 6
 TRUTH_FLOATS = 8
 MAX_POINT_NUM = 20480 * 5
+NOALLOC_MAX_ROIS = MAX_POINT_NUM
 REC_FIELDS = ("PPho", "X", "Y", "Z", "SigX", "SigY", "TPho", "Bkg", "PSNR", "CrbX", "CrbY", "Frame")
 
 
@@ -202,6 +203,7 @@ PROTOTYPES = {
     "qcs_extract_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _i, _P(_i), _P(_i), _vp]),
     "qcs_wle_dev": (_i, [_vp, _vp, _i, _vp, _vp]),
     "qcs_fit_dev": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i), _vp]),
+    "qcs_fit_dev_noalloc": (_i, [_vp, _vp, _vp, _i, _vp, _i, _vp, _vp]),
     "qcs_fit_stats_get": (_i, [_vp, _P(QcsFitStats)]),
     "qcs_crlb_dev": (_i, [_vp, _vp, _i, _vp]),
     "qcs_timing_enable": (_i, [_vp, _i]),
@@ -435,6 +437,11 @@ class Context:
         check(lib().qcs_fit_dev(self.h, self._dp(d_rois), self._dp(d_wle), n, self._dp(d_loc), cap, C.byref(rows),
                                 stream), "qcs_fit_dev")
         return rows.value
+
+    def fit_dev_noalloc(self, d_rois, d_wle, n, d_loc, d_n_rows, stream=None):
+        """qcs_fit_dev_noalloc: the row count is left in the device int d_n_rows."""
+        check(lib().qcs_fit_dev_noalloc(self.h, self._dp(d_rois), self._dp(d_wle), n, self._dp(d_loc), d_loc.shape[0],
+                                        self._dp(d_n_rows), stream), "qcs_fit_dev_noalloc")
 
     def cfit_dev(self, K, d_rois, n, init_mode, d_init, d_theta, d_info=None, d_ll=None, stream=None):
         check(lib().qcs_cfit_dev(self.h, K, self._dp(d_rois), n, init_mode, self._dp(d_init), self._dp(d_theta),
