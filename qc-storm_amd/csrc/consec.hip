@@ -79,17 +79,21 @@ __global__ void consec_clear_kernel(ConsecAcc *acc)
     if (threadIdx.x == 0) acc->fmin = INT_MAX;
 }
 
-// frames (clamped, as the keys use them) and cells: their ranges in acc
+// frames (clamped, as the keys use them) and cells: their ranges in acc.  Grid-stride over the
+// rows (at most RANGE_BLOCKS work-groups), reduced per work-group, one atomic per range and group.
+constexpr int RANGE_BLOCKS = 512;
+
 __global__ void __launch_bounds__(256) consec_range_kernel(const float *__restrict__ a, int N, int cs, ConsecAcc *__restrict__ acc)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int red[4][4];
     int fmin = INT_MAX, fmax = 0, cymax = 0, cxmax = 0;
-    if (i < N) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const float *r = a + (size_t)i * QCS_REC_FLOATS;
-        const int f = f2i(r[QCS_POS_FRME]);
-        fmin = fmax = min(max(f, 0), FRAME_LIMIT - 1);
-        cymax = cell_of(f2i(r[QCS_POS_YPOS]), cs);
-        cxmax = cell_of(f2i(r[QCS_POS_XPOS]), cs);
+        const int f = min(max(f2i(r[QCS_POS_FRME]), 0), FRAME_LIMIT - 1);
+        fmin = min(fmin, f);
+        fmax = max(fmax, f);
+        cymax = max(cymax, cell_of(f2i(r[QCS_POS_YPOS]), cs));
+        cxmax = max(cxmax, cell_of(f2i(r[QCS_POS_XPOS]), cs));
     }
     for (int off = 32; off > 0; off >>= 1) {
         fmin = min(fmin, __shfl_xor(fmin, off, 64));
@@ -97,7 +101,21 @@ __global__ void __launch_bounds__(256) consec_range_kernel(const float *__restri
         cymax = max(cymax, __shfl_xor(cymax, off, 64));
         cxmax = max(cxmax, __shfl_xor(cxmax, off, 64));
     }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
+        red[w][0] = fmin;
+        red[w][1] = fmax;
+        red[w][2] = cymax;
+        red[w][3] = cxmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            fmin = min(fmin, red[k][0]);
+            fmax = max(fmax, red[k][1]);
+            cymax = max(cymax, red[k][2]);
+            cxmax = max(cxmax, red[k][3]);
+        }
         atomicMin(&acc->fmin, fmin);
         atomicMax(&acc->fmax, fmax);
         atomicMax(&acc->cymax, cymax);
@@ -344,7 +362,7 @@ int qcs_consec_fit_dev(qcs_consec *c, const float *d_rows, int n, int is_end, fl
     ConsecAcc *acc = c->acc.as<ConsecAcc>();
     const dim3 g((N + 255) / 256), b(256);
     hipLaunchKernelGGL(consec_clear_kernel, dim3(1), dim3(64), 0, s, acc);
-    hipLaunchKernelGGL(consec_range_kernel, g, b, 0, s, cat, N, cs, acc);
+    hipLaunchKernelGGL(consec_range_kernel, dim3(std::min((N + 255) / 256, RANGE_BLOCKS)), b, 0, s, cat, N, cs, acc);
     QCS_HIP(hipGetLastError());
     // the key fields' widths from the call's ranges
     QCS_HIP(hipMemcpyAsync(c->h_acc.ptr, acc, sizeof(ConsecAcc), hipMemcpyDeviceToHost, s));
