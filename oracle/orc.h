@@ -11,8 +11,9 @@
  * Arithmetic: FP32 with every multiply-add written as an explicit fmaf (built with
  * -ffp-contract=off), IEEE division and sqrt.  Transcendentals: the reference's __expf/__logf
  * cannot be reproduced; the oracle offers
- *   ORC_TMODE_DET  -- orc_qexpf/orc_qlogf, deterministic FP32 routines (<= 2 ulp, pinned
- *                     against double-precision exp/log by the tests);
+ *   ORC_TMODE_DET  -- orc_qexpf/orc_qlogf (and the fit loss's table log orc_tlog_u),
+ *                     deterministic FP32 routines (<= 2 ulp, pinned against
+ *                     double-precision exp/log by the tests);
  *   ORC_TMODE_LIBM -- the C library's expf/logf.
  * and two evaluations of the Gaussian model inside the fit loss:
  *   ORC_MODEL_SEPARABLE -- exp(-s dx^2) * exp(-s dy^2) (one exp per row/column);
@@ -43,6 +44,12 @@ float orc_qexpf(float x);
 float orc_qlogf(float x);
 void  orc_qexpf_v(const float *x, float *y, long n);
 void  orc_qlogf_v(const float *x, float *y, long n);
+/* table-driven log of the fit loss: orc_tlog_u(x, c) = log x + c (c added before the final
+ * e ln2 step, as the HIP loss does); orc_tlogf(x) = orc_tlog_u(x, 0); the 32 table pairs */
+float orc_tlog_u(float x, float c);
+float orc_tlogf(float x);
+void  orc_tlogf_v(const float *x, float *y, long n);
+void  orc_tlog_table(uint32_t out[64]);
 
 /* ---------------- extraction (DLL/SubregionExtraction) ---------------- */
 void  orc_filter_taps(int roi_size, int loc_type, int multi, float taps[5]);

@@ -9,6 +9,7 @@
 
 float orc_exp32(float x);
 float orc_log32(float x);
+float orc_log32_u(float x, float c);
 int   orc_model_separable(void);
 
 /* loss evaluations of the calling thread (the schedule's cost; read by the tests) */
@@ -27,6 +28,7 @@ long orc_loss_evals(int reset)
 #define FMA(a,b,c)  fmaf((a), (b), (c))
 #define EXPR(x)     orc_exp32(x)
 #define LOGR(x)     orc_log32(x)
+#define LOGU(x,c)   orc_log32_u((x), (c))
 #define SQRTR(x)    sqrtf(x)
 #define FABSR(x)    fabsf(x)
 #define SEPARABLE() orc_model_separable()
@@ -37,6 +39,7 @@ long orc_loss_evals(int reset)
 #undef FMA
 #undef EXPR
 #undef LOGR
+#undef LOGU
 #undef SQRTR
 #undef FABSR
 #undef SEPARABLE
@@ -58,6 +61,7 @@ long orc_loss_evals(int reset)
 #define FMA(a,b,c)  fma((a), (b), (c))
 #define EXPR(x)     exp(x)
 #define LOGR(x)     log(x)
+#define LOGU(x,c)   (log(x) + (c))
 #define SQRTR(x)    sqrt(x)
 #define FABSR(x)    fabs(x)
 #define SEPARABLE() 0

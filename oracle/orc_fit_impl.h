@@ -2,7 +2,8 @@
  * orc_fit_impl.h -- BFGS Poisson-MLE fitters of the reference, restated for the CPU oracle
  * (TEST INFRASTRUCTURE).  Included twice: once with REAL = float (FP32 oracle) and once with
  * REAL = double (the same schedule in FP64).  The including file defines
- *   REAL, SUF(name), FMA(a,b,c), EXPR(x), LOGR(x), SQRTR(x), SEPARABLE(), C(x) (literal cast)
+ *   REAL, SUF(name), FMA(a,b,c), EXPR(x), LOGR(x), LOGU(x,c) (log x + c, fused where the
+ *   arithmetic mode fuses it), SQRTR(x), SEPARABLE(), C(x) (literal cast)
  *
  * Reference files (DLL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/bfgsMLE_dll):
  *   shared BFGS machinery  DLL/bfgsMLE_Loc_LD/bfgsMLE_core.h:51-238
@@ -60,20 +61,23 @@ static void SUF(get_desc)(int model, SUF(mdesc) *m)
     }
 }
 
-/* Per-ROI fit context: photo-electron pixels q, their logs, and the WLE weights. */
+/* Per-ROI fit context: photo-electron pixels q, their logs, c1 = 1 - log q, and the WLE weights. */
 typedef struct SUF(roictx) {
     int R;
-    REAL q[MAXPIX], lq[MAXPIX];
+    REAL q[MAXPIX], lq[MAXPIX], c1[MAXPIX];
     REAL w[MAXPIX];              /* literal weights: exp(-(sx1 dx^2 + sy1 dy^2)) */
     REAL wx[MAXR], wy[MAXR];     /* separable weights: w = wy[r] * wx[c] */
     int weighted;
 } SUF(roictx);
 
 /*
- * Poisson deviance with WLE weights, sum over pixels in row-major order of
+ * Poisson deviance with WLE weights, sum over pixels of
  *   w * (I - q - q (log I - log q)),  I = sum_k A_k exp(-(sx dx^2 + sy dy^2)) + B,
  * with the in-place clamping of the argument vector the reference performs
  * (bfgsMLE_GS2D_core.h:458-518; AS3D :484-544; 2E :518-592; 3E :510-589; AS3D-2E :569-644).
+ * Literal mode sums in row-major order per pixel as the reference writes it.  Separable mode
+ * (the HIP library's arithmetic) forms the pixel term as fma(-q, LOGU(I, c1), I) with
+ * c1 = 1 - log q, i.e. I - q (1 + log I - log q), and sums sum_r wy[r] sum_c wx[c] t(r, c).
  */
 static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
 {
@@ -102,6 +106,7 @@ static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
             }
         }
         for (r = 0; r < R; r++) {
+            REAL acc = C(0.0);
             for (c = 0; c < R; c++) {
                 REAL I;
                 if (m->K == 1) {
@@ -112,10 +117,10 @@ static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
                     I = I + b;
                 }
                 const int pi = r * R + c;
-                const REAL q = rc->q[pi];
-                REAL t = FMA(-q, LOGR(I) - rc->lq[pi], I - q);
-                total = rc->weighted ? FMA(rc->wy[r] * rc->wx[c], t, total) : total + t;
+                const REAL t = FMA(-rc->q[pi], LOGU(I, rc->c1[pi]), I);
+                acc = rc->weighted ? FMA(rc->wx[c], t, acc) : acc + t;
             }
+            total = rc->weighted ? FMA(rc->wy[r], acc, total) : total + acc;
         }
     } else {
         for (r = 0; r < R; r++) {
@@ -259,6 +264,7 @@ static void SUF(load_roi)(const qcs_params *p, const uint16_t *roi, int R, SUF(r
         REAL v = ((REAL)roi[i] - off) * kadc;
         rc->q[i] = (v > C(1.0)) ? v : C(1.0);
         rc->lq[i] = LOGR(rc->q[i]);
+        rc->c1[i] = C(1.0) - rc->lq[i];
     }
 }
 

@@ -14,6 +14,15 @@
  *           log(1+f) = f + f^2 P(f) with P a degree-8 polynomial fitted by least squares on
  *           Chebyshev nodes of [sqrt(1/2)-1, sqrt(2)-1] (max error 0.82 ulp of log1p),
  *           plus e ln2 (split constant).
+ *   table log (the fit loss's logarithm, orc_tlogf / orc_tlog_u): the same reduction to
+ *           m in [sqrt(1/2), sqrt(2)); the top five bits of the reduced mantissa select one of
+ *           32 intervals, each with a pair (i_j, t_j): i_j a float near 1/(interval centre),
+ *           t_j = -log(i_j) rounded to float, i_j picked among nearby floats so that -log(i_j)
+ *           is within 3e-4 ulp of t_j (accurate tables; the interval holding 1 has (1, 0)).
+ *           r = fma(m, i_j, -1), log1p(r) = r + r^2 (-1/2 + r (1/3 - r/4)) (|r| < 2^-6 + 4e-4,
+ *           truncation < 2e-10), log x = fma(e, ln2_f, t_j + log1p(r)).  The loss adds its
+ *           data term c before the last fma: orc_tlog_u(x, c) = log x + c.  Max error of
+ *           orc_tlogf 1.25 ulp over all positive normal floats.
  *
  * Every step is an IEEE-754 single operation or an integer operation on the bit pattern, so any IEEE
  * implementation evaluates them bit-identically.  The tests pin both routines against
@@ -84,10 +93,54 @@ float orc_qlogf(float x)
     return fmaf(ef, LN2_HI_F, fmaf(ef, LN2_LO_F, r));
 }
 
+static const uint32_t TLOG_TAB[64] = {
+    0x3fb31609u, 0xbeabf2bcu, 0x3faf4117u, 0xbea0dfd0u, 0x3fab8713u, 0xbe95dea0u, 0x3fa801dcu, 0xbe8b4090u,
+    0x3fa4ad84u, 0xbe8100f9u, 0x3fa168b9u, 0xbe6d7a43u, 0x3f9e4786u, 0xbe596d6du, 0x3f9b5080u, 0xbe46104bu,
+    0x3f986dc5u, 0xbe32dbf9u, 0x3f95a207u, 0xbe1fe70du, 0x3f92e464u, 0xbe0cf8f1u, 0x3f905c56u, 0xbdf657d0u,
+    0x3f8dd751u, 0xbdd24853u, 0x3f8b658bu, 0xbdaeae36u, 0x3f89230du, 0xbd8d356du, 0x3f86cde9u, 0xbd54260eu,
+    0x3f84a776u, 0xbd1249f4u, 0x3f82936au, 0xbca3377au, 0x3f800000u, 0x00000000u, 0x3f7a0ec6u, 0x3cc0651fu,
+    0x3f72b5b7u, 0x3d5a5c7cu, 0x3f6bb147u, 0x3da94424u, 0x3f652255u, 0x3de30fcdu, 0x3f5eecc3u, 0x3e0da9b7u,
+    0x3f59059fu, 0x3e2924a9u, 0x3f5362ddu, 0x3e441625u, 0x3f4e0ad7u, 0x3e5e4e59u, 0x3f49012au, 0x3e77a781u,
+    0x3f442f94u, 0x3e88403au, 0x3f3f8f6fu, 0x3e947796u, 0x3f3b3bffu, 0x3ea02977u, 0x3f3712c1u, 0x3eabab45u,
+};
+#define LN2_F 0.693147182464599609375f
+
+void orc_tlog_table(uint32_t out[64]) { memcpy(out, TLOG_TAB, sizeof TLOG_TAB); }
+
+float orc_tlog_u(float x, float c)
+{
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int esh = 0;
+    if (x < 1.17549435082228751e-38f) { x = x * 8388608.0f; esh = -23; }
+    uint32_t bx, mb;
+    memcpy(&bx, &x, sizeof bx);
+    const uint32_t ix = bx - 0x3f3504f3u;
+    mb = bx - (ix & 0xff800000u);
+    float m, inv, t;
+    memcpy(&m, &mb, sizeof m);                          /* m in [sqrt(1/2), sqrt(2)) */
+    const uint32_t j = (ix >> 18) & 31u;
+    memcpy(&inv, &TLOG_TAB[2 * j], sizeof inv);
+    memcpy(&t, &TLOG_TAB[2 * j + 1], sizeof t);
+    const float r = fmaf(m, inv, -1.0f);
+    const float r2 = r * r;
+    float p = fmaf(r, -0.25f, 0.3333333432674407958984375f);
+    p = fmaf(p, r, -0.5f);
+    const float lr = fmaf(p, r2, r);
+    const float w = (t + lr) + c;
+    const float ef = (float)((((int32_t)ix) >> 23) + esh);
+    return fmaf(ef, LN2_F, w);
+}
+
+float orc_tlogf(float x) { return orc_tlog_u(x, 0.0f); }
+void  orc_tlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_tlogf(x[i]); }
+
 void orc_qexpf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_qexpf(x[i]); }
 void orc_qlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_qlogf(x[i]); }
 
 /* transcendentals used by the FP32 oracle functions */
 float orc_exp32(float x) { return g_tmode == ORC_TMODE_DET ? orc_qexpf(x) : expf(x); }
 float orc_log32(float x) { return g_tmode == ORC_TMODE_DET ? orc_qlogf(x) : logf(x); }
+float orc_log32_u(float x, float c) { return g_tmode == ORC_TMODE_DET ? orc_tlog_u(x, c) : logf(x) + c; }
 int   orc_model_separable(void) { return g_model == ORC_MODEL_SEPARABLE; }

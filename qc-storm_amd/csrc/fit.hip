@@ -87,40 +87,26 @@ FitConst make_fit_const(const qcs_params &p)
 // ---------------------------------------------------------------------------------------------
 // Weighted Poisson deviance, model I = sum_k A_k exp(-(sx_k dx^2 + sy_k dy^2)) + B evaluated
 // separably (one exp per emitter and ROI row / column).  Clamps p in place as the reference's
-// loss does with its argument array.  SAFE = false uses the positive-normal fast path of the
-// logarithm and reports in `bad` whether any model value was zero, negative, subnormal or not
-// finite; the caller then re-evaluates with SAFE = true (full special-value handling), which
-// yields the value the CPU oracle computes.
+// loss does with its argument array.  The pixel term is w (I - q - q (log I - log q)), formed as
+// fma(-q, log I + c, I) with c = 1 - log q precomputed per ROI and the log fused with c
+// (logtab_u); the weighted sum runs as sum_r wy[r] sum_c wx[c] t(r, c).
+//
+// The range check makes every model value a positive normal float before the pixel loop: with
+// the background B in [2^-126, 2^124], every amplitude at most 2^124, positions below 2^60 in
+// magnitude and finite widths, each exp term lies in [0, 1] and I in [B, 2^126].  A lane that
+// fails it (a line-search probe gone far out) takes logtab_u_any, whose special-value handling
+// gives the value the CPU oracle computes.
 template <int MODEL, int R, bool FAST, bool SAFE>
-__device__ __forceinline__ float loss_model(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                            const float (&wy)[R], float (&p)[Traits<MODEL>::P], bool &bad)
+__device__ __forceinline__ float pixel_sum(const float (&q)[R * R], const float (&c1)[R * R], const float (&wx)[R],
+                                           const float (&wy)[R], const float (&ex)[Traits<MODEL>::K][R],
+                                           const float (&ay)[Traits<MODEL>::K][R], float b, const float2 *lt)
 {
     using T = Traits<MODEL>;
-    constexpr int P = T::P, K = T::K;
-#pragma unroll
-    for (int i = 0; i < P; i++) p[i] = (p[i] < T::CLAMP) ? T::CLAMP : p[i];
-    const float b = p[T::IB] * 64.0f;
-    float ex[K][R], ay[K][R];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const float a = p[T::ia(k)] * 128.0f;
-        const float sx = p[T::isx(k)] * 0.1f;
-        const float sy = p[T::isy(k)] * 0.1f;
-#pragma unroll
-        for (int c = 0; c < R; c++) {
-            const float dx = ((float)c + 0.5f) - p[T::ix(k)];
-            ex[k][c] = vexp_nonpos<FAST>((-(dx * dx)) * sx);
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const float dy = ((float)r + 0.5f) - p[T::iy(k)];
-            ay[k][r] = a * vexp_nonpos<FAST>((-(dy * dy)) * sy);
-        }
-    }
+    constexpr int K = T::K;
     float total = 0.0f;
-    bool nonnormal = false;
 #pragma unroll
     for (int r = 0; r < R; r++) {
+        float acc = 0.0f;
 #pragma unroll
         for (int c = 0; c < R; c++) {
             float I;
@@ -132,31 +118,50 @@ __device__ __forceinline__ float loss_model(const float (&q)[R * R], const float
                 for (int k = 1; k < K; k++) I = __builtin_fmaf(ay[k][r], ex[k][c], I);
                 I = I + b;
             }
-            float LI;
-            if (SAFE || FAST) {
-                LI = vlog<FAST>(I);
-            } else {
-                LI = qlogf_normal(I);
-                nonnormal = nonnormal || !is_pos_normal(I);
-            }
-            const float qq = q[r * R + c];
-            const float t = __builtin_fmaf(-qq, LI - lq[r * R + c], I - qq);
-            if (T::WEIGHTED) total = __builtin_fmaf(wy[r] * wx[c], t, total);
-            else total = total + t;
+            const float cc = c1[r * R + c];
+            float u;
+            if (FAST) u = flogf(I) + cc;
+            else if (SAFE) u = logtab_u_any(I, cc, lt);
+            else u = logtab_u(I, cc, lt);
+            const float t = __builtin_fmaf(-q[r * R + c], u, I);
+            acc = T::WEIGHTED ? __builtin_fmaf(wx[c], t, acc) : acc + t;
         }
+        total = T::WEIGHTED ? __builtin_fmaf(wy[r], acc, total) : total + acc;
     }
-    bad = nonnormal;
     return total;
 }
 
 template <int MODEL, int R, bool FAST>
-__device__ __forceinline__ float loss_eval(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                          const float (&wy)[R], float (&p)[Traits<MODEL>::P])
+__device__ __forceinline__ float loss_eval(const float (&q)[R * R], const float (&c1)[R * R], const float (&wx)[R],
+                                          const float (&wy)[R], float (&p)[Traits<MODEL>::P], const float2 *lt)
 {
-    bool bad;
-    float L = loss_model<MODEL, R, FAST, false>(q, lq, wx, wy, p, bad);
-    if (!FAST && bad) L = loss_model<MODEL, R, FAST, true>(q, lq, wx, wy, p, bad);
-    return L;
+    using T = Traits<MODEL>;
+    constexpr int P = T::P, K = T::K;
+#pragma unroll
+    for (int i = 0; i < P; i++) p[i] = (p[i] < T::CLAMP) ? T::CLAMP : p[i];
+    const float b = p[T::IB] * 64.0f;
+    bool inrange = (b >= 1.17549435082228751e-38f) && (b <= 0x1p124f);
+    float ex[K][R], ay[K][R];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const float a = p[T::ia(k)] * 128.0f;
+        const float sx = p[T::isx(k)] * 0.1f;
+        const float sy = p[T::isy(k)] * 0.1f;
+        inrange = inrange && (a <= 0x1p124f) && (sx <= 0x1p124f) && (sy <= 0x1p124f) &&
+                  (__builtin_fabsf(p[T::ix(k)]) <= 0x1p60f) && (__builtin_fabsf(p[T::iy(k)]) <= 0x1p60f);
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            const float dx = ((float)c + 0.5f) - p[T::ix(k)];
+            ex[k][c] = vexp_nonpos<FAST>((-(dx * dx)) * sx);
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float dy = ((float)r + 0.5f) - p[T::iy(k)];
+            ay[k][r] = a * vexp_nonpos<FAST>((-(dy * dy)) * sy);
+        }
+    }
+    if (FAST || inrange) return pixel_sum<MODEL, R, FAST, false>(q, c1, wx, wy, ex, ay, b, lt);
+    return pixel_sum<MODEL, R, FAST, true>(q, c1, wx, wy, ex, ay, b, lt);
 }
 
 // ConstructD0 (bfgsMLE_core.h:87-182).  D is the lane's inverse-Hessian approximation, stored in
@@ -219,8 +224,9 @@ enum { PH_G0 = 0, PH_G1 = 1, PH_L0 = 2, PH_L1 = 3, PH_BS = 4 };
 // evaluations (eleven halvings of [1e-5, 1]), step, clamp >= 0.01, and -- except after the
 // last iteration -- gradient, BFGS update and new direction.
 template <int MODEL, int R, bool FAST>
-__device__ __forceinline__ void bfgs_fit(const float (&q)[R * R], const float (&lq)[R * R], const float (&wx)[R],
-                                            const float (&wy)[R], float (&x)[Traits<MODEL>::P], float *cold)
+__device__ __forceinline__ void bfgs_fit(const float (&q)[R * R], const float (&c1)[R * R], const float (&wx)[R],
+                                            const float (&wy)[R], float (&x)[Traits<MODEL>::P], float *cold,
+                                            const float2 *lt)
 {
     constexpr int P = Traits<MODEL>::P, IT = Traits<MODEL>::ITERS;
     constexpr int NEVAL = 2 * P + IT * 12 + (IT - 1) * 2 * P;
@@ -254,7 +260,7 @@ __device__ __forceinline__ void bfgs_fit(const float (&q)[R * R], const float (&
             for (int k = 0; k < P; k++) pr[k] = __builtin_fmaf(d[k], step, x[k]);
         }
 
-        const float L = loss_eval<MODEL, R, FAST>(q, lq, wx, wy, pr);
+        const float L = loss_eval<MODEL, R, FAST>(q, c1, wx, wy, pr, lt);
 
         if (phase == PH_G0) {
 #pragma unroll
@@ -332,10 +338,11 @@ __device__ __forceinline__ void bfgs_fit(const float (&q)[R * R], const float (&
 }
 
 // ---------------------------------------------------------------------------------------------
-// Per-ROI setup shared by the one-emitter kernels: photo-electrons, logs, header.
+// Per-ROI setup shared by the fitting kernels: photo-electrons q, the loss's data term
+// c1 = 1 - log q, header.
 template <int R>
 __device__ __forceinline__ void load_roi(const uint16_t *__restrict__ rec, const FitConst &fc, float (&q)[R * R],
-                                         float (&lq)[R * R], int &xo, int &yo, int &fr)
+                                         float (&c1)[R * R], int &xo, int &yo, int &fr)
 {
     // the record is R*(R+1) 16-bit words; read it as 32-bit words (R*(R+1) is even)
     constexpr int NW = R * (R + 1) / 2;
@@ -349,7 +356,7 @@ __device__ __forceinline__ void load_roi(const uint16_t *__restrict__ rec, const
         const float adu = (float)((i & 1) ? (word >> 16) : (word & 0xffffu));
         const float e = (adu - fc.Offset) * fc.KAdc;     // bfgsMLE_GS2D_core.h:144
         q[i] = (e > 1.0f) ? e : 1.0f;
-        lq[i] = qlogf_normal(q[i]);   // q >= 1
+        c1[i] = 1.0f - qlogf_normal(q[i]);   // q >= 1
     }
     auto word_at = [&](int idx) -> int {
         const uint32_t word = v[idx >> 1];
@@ -417,14 +424,16 @@ fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
 {
     constexpr int P = Traits<MODEL>::P;
     constexpr int HALF = R / 2;
+    __shared__ float2 ltab[32];
+    log_tab_load(ltab);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = *n_dev;
     if (i >= n) return;
     const int roi = idx ? idx[i] : i;
 
-    float q[R * R], lq[R * R];
+    float q[R * R], c1[R * R];
     int xo, yo, fr;
-    load_roi<R>(rois + (size_t)roi * (R * (R + 1)), fc, q, lq, xo, yo, fr);
+    load_roi<R>(rois + (size_t)roi * (R * (R + 1)), fc, q, c1, xo, yo, fr);
 
     const float *wp = wle + (size_t)roi * QCS_WLE_FLOATS;
     const float sl = wp[1], sr = wp[2], su = wp[3], sd = wp[4];
@@ -469,7 +478,7 @@ fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
     }
 
     __shared__ float cold[(P * P + 2 * P) * 64];
-    bfgs_fit<MODEL, R, FAST>(q, lq, wx, wy, x, cold + threadIdx.x);
+    bfgs_fit<MODEL, R, FAST>(q, c1, wx, wy, x, cold + threadIdx.x, ltab);
 
     float row[QCS_REC_FLOATS];
     bool ok;

@@ -88,6 +88,79 @@ __device__ __forceinline__ float qlogf(float x)
     return y;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Table-driven logarithm of the fitting loss.  x = m 2^e with m in [sqrt(1/2), sqrt(2)) as in
+// qlogf_normal; the top five bits of the reduced mantissa select one of 32 intervals j, whose
+// table pair is (i_j, t_j) with i_j a float near 1 / (interval centre) and t_j = -log(i_j)
+// rounded to float.  i_j was chosen among the floats near the centre's reciprocal so that
+// -log(i_j) lies within 3e-4 ulp of t_j (accurate tables); interval 18 holds 1.0 and has the
+// exact pair (1, 0).  Then r = m i_j - 1 (one fma, |r| < 2^-6 + 4e-4),
+// log(m) = t_j + log1p(r) with log1p(r) = r + r^2 (-1/2 + r (1/3 - r/4)), and
+// log(x) = e ln2 + log(m) with ln2 rounded to float in one fma (|log x| >= |e| ln2 / 2, so the
+// constant's error stays below 0.07 ulp).  Maximum error 1.25 ulp over all positive normal
+// floats.  The table lives in LDS (log_tab_load); the loss folds its data term c into the sum
+// before the e ln2 fma (logtab_u).
+static __constant__ uint32_t kLogTab[64] = {
+    0x3fb31609u, 0xbeabf2bcu, 0x3faf4117u, 0xbea0dfd0u, 0x3fab8713u, 0xbe95dea0u, 0x3fa801dcu, 0xbe8b4090u,
+    0x3fa4ad84u, 0xbe8100f9u, 0x3fa168b9u, 0xbe6d7a43u, 0x3f9e4786u, 0xbe596d6du, 0x3f9b5080u, 0xbe46104bu,
+    0x3f986dc5u, 0xbe32dbf9u, 0x3f95a207u, 0xbe1fe70du, 0x3f92e464u, 0xbe0cf8f1u, 0x3f905c56u, 0xbdf657d0u,
+    0x3f8dd751u, 0xbdd24853u, 0x3f8b658bu, 0xbdaeae36u, 0x3f89230du, 0xbd8d356du, 0x3f86cde9u, 0xbd54260eu,
+    0x3f84a776u, 0xbd1249f4u, 0x3f82936au, 0xbca3377au, 0x3f800000u, 0x00000000u, 0x3f7a0ec6u, 0x3cc0651fu,
+    0x3f72b5b7u, 0x3d5a5c7cu, 0x3f6bb147u, 0x3da94424u, 0x3f652255u, 0x3de30fcdu, 0x3f5eecc3u, 0x3e0da9b7u,
+    0x3f59059fu, 0x3e2924a9u, 0x3f5362ddu, 0x3e441625u, 0x3f4e0ad7u, 0x3e5e4e59u, 0x3f49012au, 0x3e77a781u,
+    0x3f442f94u, 0x3e88403au, 0x3f3f8f6fu, 0x3e947796u, 0x3f3b3bffu, 0x3ea02977u, 0x3f3712c1u, 0x3eabab45u,
+};
+constexpr float kLn2F = 0.693147182464599609375f;   // ln 2 rounded to float
+
+// Copy the table into LDS; every thread of the block calls it (blockDim.x >= 32).
+__device__ __forceinline__ void log_tab_load(float2 *lt)
+{
+    const int l = threadIdx.x;
+    if (l < 32) lt[l] = make_float2(__uint_as_float(kLogTab[2 * l]), __uint_as_float(kLogTab[2 * l + 1]));
+    __syncthreads();
+}
+
+// log(x) + c for positive normal x, where c is small against log x or cancels it
+__device__ __forceinline__ float logtab_u(float x, float c, const float2 *lt)
+{
+    const uint32_t bx = __float_as_uint(x);
+    const uint32_t ix = bx - 0x3f3504f3u;
+    const float m = __uint_as_float(bx - (ix & 0xff800000u));
+    const float2 tv = lt[(ix >> 18) & 31u];
+    const float r = __builtin_fmaf(m, tv.x, -1.0f);
+    const float r2 = r * r;
+    float p = __builtin_fmaf(r, -0.25f, 0.3333333432674407958984375f);
+    p = __builtin_fmaf(p, r, -0.5f);
+    const float lr = __builtin_fmaf(p, r2, r);
+    const float w = (tv.y + lr) + c;
+    const float ef = (float)(((int)ix) >> 23);
+    return __builtin_fmaf(ef, kLn2F, w);
+}
+
+// logtab_u for every x: subnormals are pre-scaled by 2^23; zero gives -inf, negative and NaN
+// arguments NaN, +inf gives +inf
+__device__ __forceinline__ float logtab_u_any(float x, float c, const float2 *lt)
+{
+    const bool sub = (x > 0.0f) && (x < 1.17549435082228751e-38f);
+    const float xs = sub ? x * 8388608.0f : x;
+    const uint32_t bx = __float_as_uint(xs);
+    const uint32_t ix = bx - 0x3f3504f3u;
+    const float m = __uint_as_float(bx - (ix & 0xff800000u));
+    const float2 tv = lt[(ix >> 18) & 31u];
+    const float r = __builtin_fmaf(m, tv.x, -1.0f);
+    const float r2 = r * r;
+    float p = __builtin_fmaf(r, -0.25f, 0.3333333432674407958984375f);
+    p = __builtin_fmaf(p, r, -0.5f);
+    const float lr = __builtin_fmaf(p, r2, r);
+    const float w = (tv.y + lr) + c;
+    const float ef = (float)((((int)ix) >> 23) - (sub ? 23 : 0));
+    float u = __builtin_fmaf(ef, kLn2F, w);
+    u = (x == 0.0f) ? -__builtin_inff() : u;
+    u = (x < 0.0f || x != x) ? __builtin_nanf("") : u;
+    u = (x == __builtin_inff()) ? x : u;
+    return u;
+}
+
 __device__ __forceinline__ float fexpf(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 __device__ __forceinline__ float flogf(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 

@@ -36,6 +36,10 @@ PROTOS = {
     "orc_qlogf": (_f, [_f]),
     "orc_qexpf_v": (None, [_vp, _vp, C.c_long]),
     "orc_qlogf_v": (None, [_vp, _vp, C.c_long]),
+    "orc_tlogf": (_f, [_f]),
+    "orc_tlog_u": (_f, [_f, _f]),
+    "orc_tlogf_v": (None, [_vp, _vp, C.c_long]),
+    "orc_tlog_table": (None, [_vp]),
     "orc_filter_taps": (None, [_i, _i, _i, _vp]),
     "orc_remove_background": (None, [_vp, _vp, _i, _i]),
     "orc_image_variance": (_f, [_vp, _i, _i]),

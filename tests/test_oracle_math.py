@@ -41,6 +41,71 @@ def test_qlogf_specials():
     assert np.isneginf(l(0.0)) and np.isnan(l(-1.0)) and np.isposinf(l(float("inf"))) and l(1.0) == 0.0
 
 
+def _tlog(x):
+    y = np.zeros_like(x)
+    O.lib().orc_tlogf_v(O.p(x), O.p(y), x.size)
+    return y
+
+
+def test_tlogf_accuracy():
+    """The fit loss's table log against double-precision log: every float of [1/4, 4) (all 32
+    intervals in the binades around 1, where the table and polynomial terms dominate), a
+    geometric sweep of the whole normal range, and subnormals."""
+    x = np.arange(0x3e800000, 0x40800000, dtype=np.uint32).view(np.float32)
+    y = _tlog(x)
+    exact = np.log(x.astype(np.float64))
+    nz = exact != 0.0
+    err = np.abs(y[nz].astype(np.float64) - exact[nz]) / _ulp(exact[nz])
+    print("tlogf max ulp on [1/4, 4)", err.max())
+    assert err.max() <= 1.3 and y[x == 1.0][0] == 0.0
+    x = np.concatenate([np.geomspace(1.2e-38, 3e38, 2_000_001), np.geomspace(1e-45, 1e-38, 10_001)]).astype(np.float32)
+    x = x[(x > 0) & (x != 1.0)]
+    y = _tlog(x)
+    exact = np.log(x.astype(np.float64))
+    err = np.abs(y.astype(np.float64) - exact) / _ulp(exact)
+    print("tlogf max ulp, full range", err.max())
+    assert err.max() <= 1.3
+
+
+def test_tlog_table_entries():
+    """Each pair (i_j, t_j): i_j within 0.1% of the reciprocal of its interval's centre, t_j the
+    float nearest -log(i_j) to within 1e-3 ulp, the pair (1, 0) on the interval holding 1, and
+    |m i_j - 1| < 2^-6 + 4e-4 over the interval's ends."""
+    tab = np.zeros(64, np.uint32)
+    O.lib().orc_tlog_table(O.p(tab))
+    inv, t = tab[0::2].view(np.float32).astype(np.float64), tab[1::2].view(np.float32).astype(np.float64)
+    k = np.uint32(0x3f3504f3)
+    for j in range(32):
+        lo = np.array([k + (j << 18)], np.uint32).view(np.float32)[0].astype(np.float64)
+        hi = np.array([k + ((j + 1) << 18) - 1], np.uint32).view(np.float32)[0].astype(np.float64)
+        if lo <= 1.0 <= hi:
+            assert inv[j] == 1.0 and t[j] == 0.0 and j == 18
+            continue
+        assert abs(inv[j] * (lo + hi) / 2 - 1.0) < 1e-3, j
+        assert abs(t[j] + np.log(inv[j])) <= 1e-3 * np.spacing(np.float32(abs(t[j]))), j
+        assert max(abs(lo * inv[j] - 1.0), abs(hi * inv[j] - 1.0)) < 2.0 ** -6 + 4e-4, j
+
+
+def test_tlog_u_adds_the_data_term():
+    """orc_tlog_u(x, c) = log x + c with c = 1 - log q (the loss's data term) for model values x
+    near and far from q: within 1 ulp of log x plus 1 ulp of the result, and c = 0 is tlogf."""
+    rng = np.random.default_rng(5)
+    q = rng.uniform(1.0, 5000.0, 20000).astype(np.float32)
+    x = (q * rng.uniform(0.25, 4.0, q.size)).astype(np.float32)
+    c = (1.0 - np.log(q.astype(np.float64))).astype(np.float32)
+    u = np.array([O.lib().orc_tlog_u(float(a), float(b)) for a, b in zip(x, c)], np.float32)
+    exact = np.log(x.astype(np.float64)) + c.astype(np.float64)
+    bound = _ulp(np.log(x.astype(np.float64))) + _ulp(exact)
+    assert np.all(np.abs(u.astype(np.float64) - exact) <= bound)
+    assert np.array_equal(np.array([O.lib().orc_tlog_u(float(a), 0.0) for a in x[:200]], np.float32), _tlog(x[:200]))
+
+
+def test_tlogf_specials():
+    l = O.lib().orc_tlogf
+    assert np.isneginf(l(0.0)) and np.isnan(l(-1.0)) and np.isposinf(l(float("inf"))) and l(1.0) == 0.0
+    assert np.isnan(l(float("nan"))) and np.isneginf(O.lib().orc_tlog_u(0.0, 3.0))
+
+
 def test_philox_known_answers():
     # Random123 known-answer vectors for Philox4x32-10
     import ctypes as C
