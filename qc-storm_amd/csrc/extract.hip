@@ -150,25 +150,36 @@ __device__ __forceinline__ void batch_thresholds(const ExtractConst &c, const un
 // past the wave, and a strip at the image border takes its clamped columns by readlane.  hc stays
 // in a lane-local LDS ring, sv in registers.
 // MODE 0 computes tv only, over the variance region, into tvbuf; MODE 1 is the full pass.
+// Wave shifts by one lane (DPP wave_shr:1 / wave_shl:1).  Bound control writes 0 where the
+// source lane is outside the wave (lane 0 for prev, lane 63 for next), so no old value is needed.
+__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ float lane_prev(float v) { return __int_as_float(lane_prev(__float_as_int(v))); }
+__device__ __forceinline__ float lane_next(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
+
 constexpr int SEGW = 56;             // output columns per wave
 constexpr int RAWW = 72;             // raw row width in LDS
 constexpr int VAR_MAX = 512;         // region side bound (2 * 256)
 
-template <int MODE, bool SMOOTH>
-__global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
-                                                  const unsigned long long *__restrict__ var_sums,
-                                                  uint16_t *__restrict__ tvbuf,
-                                                  unsigned long long *__restrict__ cand_mask,
-                                                  unsigned long long *__restrict__ mark_mask,
-                                                  int *__restrict__ row_count)
+constexpr int WALK_RS = 4;   // image rows per step
+constexpr int WALK_NR = 16;  // raw ring rows (a step needs rows r-4 .. r+RS+3); power of two
+
+// The walk of one strip.  BORDER: a lane of 1..62 has an H-pass column clamped to the image
+// (the strips at the left and right image edges); only those carry the clamping code.
+template <int MODE, bool SMOOTH, bool BORDER>
+__device__ __forceinline__ void walk_body(const uint16_t *__restrict__ frames, const ExtractConst &c,
+                                          const unsigned long long *__restrict__ var_sums,
+                                          uint16_t *__restrict__ tvbuf,
+                                          unsigned long long *__restrict__ cand_mask,
+                                          unsigned long long *__restrict__ mark_mask,
+                                          int *__restrict__ row_count, uint16_t (*raw)[RAWW], float (*hcs)[64])
 {
-    constexpr int RS = 4;    // image rows per step
-    constexpr int NR = 16;   // raw ring rows (a step needs rows r-4 .. r+RS+3); power of two
-    __shared__ uint16_t raw[NR][RAWW];
-    __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
+    constexpr int RS = WALK_RS;
+    constexpr int NR = WALK_NR;
 
     const int b = blockIdx.z;
     const int BH = batch_rows(c, b);
+    const int seg = blockIdx.x;
     const int W = c.W;
     const uint16_t *img = frames + (size_t)b * c.B * c.H * W;
     const int l = threadIdx.x;
@@ -187,7 +198,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
         rb = ys;          // steps compute tv rows r .. r+3
         rlast = ye - 1;
     } else {
-        x0 = blockIdx.x * SEGW;
+        x0 = seg * SEGW;
         ys = blockIdx.y * c.walk_rows;
         if (ys >= BH) return;
         ye = min(ys + c.walk_rows, BH);
@@ -234,13 +245,12 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     for (int t = 0; t < 5; t++) hci[t] = max(min(clampi(xc - 2 + t, 0, W - 1) - (x0 - 4), 63), 0);
     // lanes of image columns 0 and W-1 when the strip reaches a border (the H pass clamps to them)
     const int lane_lo = max(min(0 - (x0 - 4), 63), 0), lane_hi = max(min(W - 1 - (x0 - 4), 63), 0);
-    const bool border = x0 - 4 + 1 < 2 || x0 - 4 + 62 > W - 3;   // uniform: some lane 1..62 clamps
     const int half = c.R / 2;
     const bool out_lane = (l >= 4 && l < 4 + SEGW);
     const bool det_col = out_lane && xc < W && xc >= half && xc < W - half;
     const int ylo = max(ys - 1, 0), yhi = min(ye, BH - 1);   // sv rows this wave needs
-    unsigned long long *cmask = cand_mask + (size_t)b * c.B * c.H * c.nseg + blockIdx.x;
-    unsigned long long *mmask = mark_mask + (size_t)b * c.B * c.H * c.nseg + blockIdx.x;
+    unsigned long long *cmask = cand_mask + (size_t)b * c.B * c.H * c.nseg + seg;
+    unsigned long long *mmask = mark_mask + (size_t)b * c.B * c.H * c.nseg + seg;
     int *rcount = row_count + (size_t)b * c.B * c.H;
     const int nseg = c.nseg, H = c.H;
     int ryd = ((rb - 3) % H + H) % H;   // frame row of detection row r-3 for the current step
@@ -251,9 +261,10 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
     bool have_cols = false;   // uniform
     // the lane's V-smoothed rows r-4 .. r+RS-3: [0, 2) from the previous step, [2, RS+2) this
     // step's V pass; detection takes lanes l-1 and l+1 of them by DPP wave shifts
-    int svr[RS + 2];
+    float svr[RS + 2];
+    int lor[RS + 2];
 #pragma unroll
-    for (int i = 0; i < RS + 2; i++) svr[i] = 0;
+    for (int i = 0; i < RS + 2; i++) { svr[i] = 0.0f; lor[i] = 0; }
 
     // One step of 4 rows.  EDGE: rows near the batch or walk ends (bounds checks, clamped rows,
     // dynamic ring slots); otherwise every row is in range and PH fixes the ring slots.
@@ -315,8 +326,7 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
         int tvk[RS];
 #pragma unroll
         for (int k = 0; k < RS; k++) {
-            const int bm = __builtin_amdgcn_update_dpp(0, bkv[k], 0x138, 0xf, 0xf, false);   // lane l-1
-            const int bp = __builtin_amdgcn_update_dpp(0, bkv[k], 0x130, 0xf, 0xf, false);   // lane l+1
+            const int bm = lane_prev(bkv[k]), bp = lane_next(bkv[k]);
             const int bl = li == l ? bkv[k] : bm, br = ri == l ? bkv[k] : bp;
             const int rawc = raw[rs(k)][l + 4];
             const int t = rawc - (bl + bkv[k] + br + 23) / 24;
@@ -340,33 +350,33 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
             if (EDGE && (r + k < 0 || r + k >= BH)) continue;   // uniform
             float hc;
             if constexpr (SMOOTH) {
-                const int v0 = tvk[k];
-                const int m1 = __builtin_amdgcn_update_dpp(0, v0, 0x138, 0xf, 0xf, false);   // lane l-1
-                const int m2 = __builtin_amdgcn_update_dpp(0, m1, 0x138, 0xf, 0xf, false);   // lane l-2
-                const int p1 = __builtin_amdgcn_update_dpp(0, v0, 0x130, 0xf, 0xf, false);   // lane l+1
-                const int p2 = __builtin_amdgcn_update_dpp(0, p1, 0x130, 0xf, 0xf, false);   // lane l+2
-                int src[5] = {m2, m1, v0, p1, p2};
-                if (border) {   // uniform
-                    const int vl = __builtin_amdgcn_readlane(v0, lane_lo), vr = __builtin_amdgcn_readlane(v0, lane_hi);
+                const float v0 = (float)tvk[k];
+                const float m1 = lane_prev(v0), m2 = lane_prev(m1);   // lanes l-1, l-2
+                const float p1 = lane_next(v0), p2 = lane_next(p1);   // lanes l+1, l+2
+                float src[5] = {m2, m1, v0, p1, p2};
+                if constexpr (BORDER) {
+                    const float vl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0), lane_lo));
+                    const float vr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v0), lane_hi));
 #pragma unroll
                     for (int t = 0; t < 5; t++) src[t] = hci[t] == lane_lo ? vl : (hci[t] == lane_hi ? vr : src[t]);
                 }
                 float acc = 0.0f;
 #pragma unroll
-                for (int t = 0; t < 5; t++) acc = __builtin_fmaf((float)src[t], tap[t], acc);
+                for (int t = 0; t < 5; t++) acc = __builtin_fmaf(src[t], tap[t], acc);
                 acc = acc > 65535.0f ? 65535.0f : acc;
-                hc = (float)(int)acc;
+                hc = __builtin_truncf(acc);   // acc >= 0: the uint16 store's truncation
             } else {
                 hc = (float)tvk[k];
             }
             hcs[h8(k)][l] = hc;
         }
-        // V pass: sv rows r-2 .. r+1 (hc rows clamped to the batch)
+        // V pass: sv rows r-2 .. r+1 (hc rows clamped to the batch), held as integral floats, and
+        // whether each is at most th1
 #pragma unroll
         for (int k = 0; k < RS; k++) {
             const int y = r - 2 + k;
             if (EDGE && (y < ylo || y > yhi)) continue;   // uniform
-            int sv;
+            float sv;
             if constexpr (SMOOTH) {
                 float acc = 0.0f;
 #pragma unroll
@@ -377,38 +387,34 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
                     acc = __builtin_fmaf(hv, tap[t], acc);
                 }
                 acc = acc > 65535.0f ? 65535.0f : acc;
-                sv = (int)acc;
+                sv = __builtin_truncf(acc);
             } else {
-                sv = (int)hcs[h8(k - 2)][l];
+                sv = hcs[h8(k - 2)][l];
             }
             svr[2 + k] = sv;
+            lor[2 + k] = (sv > th1) ? 0 : 1;
         }
         // detection rows r-3 .. r (sv rows r-4 .. r+1); the shifts run on the whole wave, ahead
-        // of the per-lane tests
+        // of the per-lane tests.  The eight-neighbour count s8 >= 7 of the reference is "at most one
+        // neighbour <= th1": with cv >= th > th1 the centre is not one of them, so it is the 3 x 3
+        // count of the low flags (column sums, then lanes l-1, l, l+1) being at most one.
 #pragma unroll
         for (int k = 0; k < RS; k++) {
             const int d = r - 3 + k;
             int ry = ryd + k;
             ry = ry >= H ? ry - H : ry;
             if (EDGE && (d < ys || d >= ye)) continue;   // uniform
-            const int p01 = svr[k], cv = svr[k + 1], p21 = svr[k + 2];
-            const int p00 = __builtin_amdgcn_update_dpp(0, p01, 0x138, 0xf, 0xf, false);   // lane l-1
-            const int p02 = __builtin_amdgcn_update_dpp(0, p01, 0x130, 0xf, 0xf, false);   // lane l+1
-            const int p10 = __builtin_amdgcn_update_dpp(0, cv, 0x138, 0xf, 0xf, false);
-            const int p12 = __builtin_amdgcn_update_dpp(0, cv, 0x130, 0xf, 0xf, false);
-            const int p20 = __builtin_amdgcn_update_dpp(0, p21, 0x138, 0xf, 0xf, false);
-            const int p22 = __builtin_amdgcn_update_dpp(0, p21, 0x130, 0xf, 0xf, false);
-            bool cand = false, mark = false;
-            if (det_col && ry >= half && ry < H - half) {
-                if ((float)cv >= th) {
-                    const int s8 = ((float)p00 > th1) + ((float)p01 > th1) + ((float)p02 > th1) + ((float)p10 > th1) +
-                                   ((float)p12 > th1) + ((float)p20 > th1) + ((float)p21 > th1) + ((float)p22 > th1);
-                    const int sin = (cv > p01) + (cv >= p12) + (cv >= p21) + (cv > p10);
-                    const int sout = (cv > p00) + (cv > p02) + (cv > p20) + (cv > p22);
-                    mark = (s8 >= 7 && sin == 4 && sout >= 3);
-                    cand = mark && (sout == 4);
-                }
-            }
+            const float p01 = svr[k], cv = svr[k + 1], p21 = svr[k + 2];
+            const float p00 = lane_prev(p01), p02 = lane_next(p01);
+            const float p10 = lane_prev(cv), p12 = lane_next(cv);
+            const float p20 = lane_prev(p21), p22 = lane_next(p21);
+            const int lc = lor[k] + lor[k + 1] + lor[k + 2];
+            const int low9 = lane_prev(lc) + lc + lane_next(lc);
+            const bool sin4 = (cv > p01) && (cv >= p12) && (cv >= p21) && (cv > p10);
+            const bool a = cv > p00, bb = cv > p02, cc = cv > p20, dd = cv > p22;
+            const bool sout3 = (a && bb && (cc || dd)) || (cc && dd && (a || bb));
+            const bool mark = det_col && ry >= half && ry < H - half && cv >= th && low9 <= 1 && sin4 && sout3;
+            const bool cand = mark && a && bb && cc && dd;
             const unsigned long long cm = (__ballot(cand) >> 4) & ((1ULL << SEGW) - 1);
             const unsigned long long mm = (__ballot(mark) >> 4) & ((1ULL << SEGW) - 1);
             if (l == 0) {
@@ -419,6 +425,8 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
         }
         svr[0] = svr[RS];
         svr[1] = svr[RS + 1];
+        lor[0] = lor[RS];
+        lor[1] = lor[RS + 1];
     };
 
     using E = std::true_type;
@@ -440,6 +448,23 @@ __global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ f
         ryd += RS;
         ryd = ryd >= H ? ryd - H : ryd;
     }
+}
+
+template <int MODE, bool SMOOTH>
+__global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
+                                                  const unsigned long long *__restrict__ var_sums,
+                                                  uint16_t *__restrict__ tvbuf,
+                                                  unsigned long long *__restrict__ cand_mask,
+                                                  unsigned long long *__restrict__ mark_mask,
+                                                  int *__restrict__ row_count)
+{
+    __shared__ uint16_t raw[WALK_NR][RAWW];
+    __shared__ float hcs[8][64];   // lane-local ring of H-smoothed rows
+    const int xs = (int)blockIdx.x * SEGW;
+    if (MODE == 1 && (xs - 4 + 1 < 2 || xs - 4 + 62 > c.W - 3))   // uniform
+        walk_body<MODE, SMOOTH, true>(frames, c, var_sums, tvbuf, cand_mask, mark_mask, row_count, raw, hcs);
+    else
+        walk_body<MODE, SMOOTH, false>(frames, c, var_sums, tvbuf, cand_mask, mark_mask, row_count, raw, hcs);
 }
 
 // integer sums of the variance samples of each batch from the region's background-removed values
