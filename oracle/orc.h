@@ -50,6 +50,11 @@ float orc_tlog_u(float x, float c);
 float orc_tlogf(float x);
 void  orc_tlogf_v(const float *x, float *y, long n);
 void  orc_tlog_table(uint32_t out[64]);
+/* binade table log of the one-emitter losses: orc_tlogb_u(x, c) = log x + c; the 1024 pairs */
+float orc_tlogb_u(float x, float c);
+float orc_tlogbf(float x);
+void  orc_tlogbf_v(const float *x, float *y, long n);
+void  orc_tlogb_table(uint32_t out[2048]);
 
 /* ---------------- extraction (DLL/SubregionExtraction) ---------------- */
 void  orc_filter_taps(int roi_size, int loc_type, int multi, float taps[5]);

@@ -10,6 +10,7 @@
 float orc_exp32(float x);
 float orc_log32(float x);
 float orc_log32_u(float x, float c);
+float orc_log32_ub(float x, float c);
 int   orc_model_separable(void);
 
 /* loss evaluations of the calling thread (the schedule's cost; read by the tests) */
@@ -29,6 +30,7 @@ long orc_loss_evals(int reset)
 #define EXPR(x)     orc_exp32(x)
 #define LOGR(x)     orc_log32(x)
 #define LOGU(x,c)   orc_log32_u((x), (c))
+#define LOGUB(x,c)  orc_log32_ub((x), (c))
 #define SQRTR(x)    sqrtf(x)
 #define FABSR(x)    fabsf(x)
 #define SEPARABLE() orc_model_separable()
@@ -40,6 +42,7 @@ long orc_loss_evals(int reset)
 #undef EXPR
 #undef LOGR
 #undef LOGU
+#undef LOGUB
 #undef SQRTR
 #undef FABSR
 #undef SEPARABLE
@@ -62,6 +65,7 @@ long orc_loss_evals(int reset)
 #define EXPR(x)     exp(x)
 #define LOGR(x)     log(x)
 #define LOGU(x,c)   (log(x) + (c))
+#define LOGUB(x,c)  (log(x) + (c))
 #define SQRTR(x)    sqrt(x)
 #define FABSR(x)    fabs(x)
 #define SEPARABLE() 0

@@ -2,8 +2,9 @@
  * orc_fit_impl.h -- BFGS Poisson-MLE fitters of the reference, restated for the CPU oracle
  * (TEST INFRASTRUCTURE).  Included twice: once with REAL = float (FP32 oracle) and once with
  * REAL = double (the same schedule in FP64).  The including file defines
- *   REAL, SUF(name), FMA(a,b,c), EXPR(x), LOGR(x), LOGU(x,c) (log x + c, fused where the
- *   arithmetic mode fuses it), SQRTR(x), SEPARABLE(), C(x) (literal cast)
+ *   REAL, SUF(name), FMA(a,b,c), EXPR(x), LOGR(x), LOGU(x,c) and LOGUB(x,c) (log x + c, fused
+ *   where the arithmetic mode fuses it: the multi- and one-emitter losses' table logs),
+ *   SQRTR(x), SEPARABLE(), C(x) (literal cast)
  *
  * Reference files (DLL = MLEbfgs/bfgs_MLE_dll x64/bfgsMLE_dll/bfgsMLE_dll):
  *   shared BFGS machinery  DLL/bfgsMLE_Loc_LD/bfgsMLE_core.h:51-238
@@ -77,7 +78,8 @@ typedef struct SUF(roictx) {
  * (bfgsMLE_GS2D_core.h:458-518; AS3D :484-544; 2E :518-592; 3E :510-589; AS3D-2E :569-644).
  * Literal mode sums in row-major order per pixel as the reference writes it.  Separable mode
  * (the HIP library's arithmetic) forms the pixel term as fma(-q, LOGU(I, c1), I) with
- * c1 = 1 - log q, i.e. I - q (1 + log I - log q), and sums sum_r wy[r] sum_c wx[c] t(r, c).
+ * c1 = 1 - log q, i.e. I - q (1 + log I - log q), and sums sum_r wy[r] sum_c wx[c] t(r, c);
+ * one-emitter models take the binade table log (LOGUB), multi-emitter ones the 32-pair one.
  */
 static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
 {
@@ -117,7 +119,8 @@ static REAL SUF(loss)(const SUF(mdesc) *m, const SUF(roictx) *rc, REAL *p)
                     I = I + b;
                 }
                 const int pi = r * R + c;
-                const REAL t = FMA(-rc->q[pi], LOGU(I, rc->c1[pi]), I);
+                const REAL u = (m->K == 1) ? LOGUB(I, rc->c1[pi]) : LOGU(I, rc->c1[pi]);
+                const REAL t = FMA(-rc->q[pi], u, I);
                 acc = rc->weighted ? FMA(rc->wx[c], t, acc) : acc + t;
             }
             total = rc->weighted ? FMA(rc->wy[r], acc, total) : total + acc;

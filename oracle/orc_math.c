@@ -22,7 +22,13 @@
  *           r = fma(m, i_j, -1), log1p(r) = r + r^2 (-1/2 + r (1/3 - r/4)) (|r| < 2^-6 + 4e-4,
  *           truncation < 2e-10), log x = fma(e, ln2_f, t_j + log1p(r)).  The loss adds its
  *           data term c before the last fma: orc_tlog_u(x, c) = log x + c.  Max error of
- *           orc_tlogf 1.25 ulp over all positive normal floats.
+ *           orc_tlogf 1.25 ulp over all positive normal floats.  The multi-emitter losses use it.
+ *   binade table log (the one-emitter losses, orc_tlogb_u): g = (int)(b - 0x3f360000) >> 18 of
+ *           the bit pattern b numbers the binade e and one of 32 intervals of x = m 2^e,
+ *           m in [0.7109375, 1.421875); for e in -10..21 the pair (i, t) at g + 320 carries
+ *           e ln2 inside t = -log(i), so log x + c = (t + c) + log1p(fma(x, i, -1)); outside
+ *           that range x is scaled into binade 0 and e ln2 added in one fma.  Max error of
+ *           orc_tlogbf 1.2 ulp.
  *
  * Every step is an IEEE-754 single operation or an integer operation on the bit pattern, so any IEEE
  * implementation evaluates them bit-identically.  The tests pin both routines against
@@ -134,6 +140,54 @@ float orc_tlog_u(float x, float c)
 }
 
 float orc_tlogf(float x) { return orc_tlog_u(x, 0.0f); }
+
+/* binade table log of the one-emitter losses: the table is the library's data file
+ * (qc-storm_amd/csrc/qcs_logtab.inc, written by logtab_gen.py; its entries are pinned against
+ * their definition by the tests) */
+static const uint32_t TLOGB_TAB[2048] = {
+#include "../qc-storm_amd/csrc/qcs_logtab.inc"
+};
+#define TLOGB_K   0x3f360000u
+#define TLOGB_LO  0x1.6cp-11f   /* 0.7109375 * 2^-10 */
+#define TLOGB_HI  0x1.6cp21f    /* 0.7109375 * 2^22 */
+
+void orc_tlogb_table(uint32_t out[2048]) { memcpy(out, TLOGB_TAB, sizeof TLOGB_TAB); }
+
+static float tlogb_core(float x, float c)
+{
+    uint32_t bx;
+    memcpy(&bx, &x, sizeof bx);
+    const int g = (((int32_t)(bx - TLOGB_K)) >> 18) + 320;   /* binade and interval of x */
+    float inv, t;
+    memcpy(&inv, &TLOGB_TAB[2 * g], sizeof inv);
+    memcpy(&t, &TLOGB_TAB[2 * g + 1], sizeof t);
+    const float r = fmaf(x, inv, -1.0f);
+    const float r2 = r * r;
+    float p = fmaf(r, -0.25f, 0.3333333432674407958984375f);
+    p = fmaf(p, r, -0.5f);
+    const float lr = fmaf(p, r2, r);
+    return (t + c) + lr;
+}
+
+float orc_tlogb_u(float x, float c)
+{
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    if (x >= TLOGB_LO && x < TLOGB_HI) return tlogb_core(x, c);      /* binades -10..21 */
+    int esh = 0;
+    if (x < 1.17549435082228751e-38f) { x = x * 8388608.0f; esh = -23; }
+    uint32_t bx, mb;
+    memcpy(&bx, &x, sizeof bx);
+    const uint32_t ix = bx - TLOGB_K;
+    mb = bx - (ix & 0xff800000u);                                      /* x 2^-e in binade 0 */
+    float xr;
+    memcpy(&xr, &mb, sizeof xr);
+    return fmaf((float)((((int32_t)ix) >> 23) + esh), LN2_F, tlogb_core(xr, c));
+}
+
+float orc_tlogbf(float x) { return orc_tlogb_u(x, 0.0f); }
+void  orc_tlogbf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_tlogbf(x[i]); }
 void  orc_tlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_tlogf(x[i]); }
 
 void orc_qexpf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_qexpf(x[i]); }
@@ -143,4 +197,5 @@ void orc_qlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++
 float orc_exp32(float x) { return g_tmode == ORC_TMODE_DET ? orc_qexpf(x) : expf(x); }
 float orc_log32(float x) { return g_tmode == ORC_TMODE_DET ? orc_qlogf(x) : logf(x); }
 float orc_log32_u(float x, float c) { return g_tmode == ORC_TMODE_DET ? orc_tlog_u(x, c) : logf(x) + c; }
+float orc_log32_ub(float x, float c) { return g_tmode == ORC_TMODE_DET ? orc_tlogb_u(x, c) : logf(x) + c; }
 int   orc_model_separable(void) { return g_model == ORC_MODEL_SEPARABLE; }

@@ -91,11 +91,13 @@ FitConst make_fit_const(const qcs_params &p)
 // fma(-q, log I + c, I) with c = 1 - log q precomputed per ROI and the log fused with c
 // (logtab_u); the weighted sum runs as sum_r wy[r] sum_c wx[c] t(r, c).
 //
-// The range check makes every model value a positive normal float before the pixel loop: with
-// the background B in [2^-126, 2^124], every amplitude at most 2^124, positions below 2^60 in
-// magnitude and finite widths, each exp term lies in [0, 1] and I in [B, 2^126].  A lane that
-// fails it (a line-search probe gone far out) takes logtab_u_any, whose special-value handling
-// gives the value the CPU oracle computes.
+// The range check bounds every model value before the pixel loop: with positions below 2^60 in
+// magnitude and finite widths each exp term lies in [0, 1], so I lies in [B, sum of the
+// amplitudes + B].  Multi-emitter models need B in [2^-126, 2^124] and amplitudes at most 2^124
+// (I a positive normal float); the one-emitter models use the binade table log and need B and A
+// within [2^-10.5, 2^18.5] so that I stays inside the table.  A lane that fails it (a line-search
+// probe gone far out, a background at zero) takes the _any form of the log, whose range and
+// special-value handling gives the value the CPU oracle computes.
 template <int MODEL, int R, bool FAST, bool SAFE>
 __device__ __forceinline__ float pixel_sum(const float (&q)[R * R], const float (&c1)[R * R], const float (&wx)[R],
                                            const float (&wy)[R], const float (&ex)[Traits<MODEL>::K][R],
@@ -121,8 +123,8 @@ __device__ __forceinline__ float pixel_sum(const float (&q)[R * R], const float 
             const float cc = c1[r * R + c];
             float u;
             if (FAST) u = flogf(I) + cc;
-            else if (SAFE) u = logtab_u_any(I, cc, lt);
-            else u = logtab_u(I, cc, lt);
+            else if (K == 1) u = SAFE ? logtabb_u_any(I, cc, lt) : logtabb_u(I, cc, lt);
+            else u = SAFE ? logtab_u_any(I, cc, lt) : logtab_u(I, cc, lt);
             const float t = __builtin_fmaf(-q[r * R + c], u, I);
             acc = T::WEIGHTED ? __builtin_fmaf(wx[c], t, acc) : acc + t;
         }
@@ -140,14 +142,17 @@ __device__ __forceinline__ float loss_eval(const float (&q)[R * R], const float 
 #pragma unroll
     for (int i = 0; i < P; i++) p[i] = (p[i] < T::CLAMP) ? T::CLAMP : p[i];
     const float b = p[T::IB] * 64.0f;
-    bool inrange = (b >= 1.17549435082228751e-38f) && (b <= 0x1p124f);
+    // one-emitter models: every model value inside the binade table, I in [b, a + b]
+    constexpr float BLO = (K == 1) ? kLogTabBLo : 1.17549435082228751e-38f;
+    constexpr float AHI = (K == 1) ? kLogTabBHi * 0.125f : 0x1p124f;
+    bool inrange = (b >= BLO) && (b <= AHI);
     float ex[K][R], ay[K][R];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const float a = p[T::ia(k)] * 128.0f;
         const float sx = p[T::isx(k)] * 0.1f;
         const float sy = p[T::isy(k)] * 0.1f;
-        inrange = inrange && (a <= 0x1p124f) && (sx <= 0x1p124f) && (sy <= 0x1p124f) &&
+        inrange = inrange && (a <= AHI) && (sx <= 0x1p124f) && (sy <= 0x1p124f) &&
                   (__builtin_fabsf(p[T::ix(k)]) <= 0x1p60f) && (__builtin_fabsf(p[T::iy(k)]) <= 0x1p60f);
 #pragma unroll
         for (int c = 0; c < R; c++) {
@@ -417,15 +422,15 @@ __device__ __forceinline__ float zpoly(const float (&pc)[5], float d, float corr
 // Up to 7x7 the register budget is held to two waves per SIMD: the exact-math kernel then spills a
 // few registers outside its loss loop but runs 1.6x faster than at its natural one wave.
 template <int MODEL, int R, bool FAST>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(R <= 7 ? 2 : 1, R <= 7 ? 2 : 10)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(R <= 7 ? 2 : 1, R <= 7 ? 2 : 10)))
 fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
                                                     const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
                                                     float *__restrict__ loc, int *__restrict__ flag, FitConst fc)
 {
     constexpr int P = Traits<MODEL>::P;
     constexpr int HALF = R / 2;
-    __shared__ float2 ltab[32];
-    log_tab_load(ltab);
+    __shared__ float2 ltab[1024];
+    log_tabb_load(ltab);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = *n_dev;
     if (i >= n) return;
@@ -477,8 +482,8 @@ fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
         wle_weights<R>(wsx, wsy, mlefit, wx, wy);
     }
 
-    __shared__ float cold[(P * P + 2 * P) * 64];
-    bfgs_fit<MODEL, R, FAST>(q, c1, wx, wy, x, cold + threadIdx.x, ltab);
+    __shared__ float cold[2][(P * P + 2 * P) * 64];   // per wave of the two
+    bfgs_fit<MODEL, R, FAST>(q, c1, wx, wy, x, &cold[threadIdx.x >> 6][threadIdx.x & 63], ltab);
 
     float row[QCS_REC_FLOATS];
     bool ok;
@@ -607,13 +612,13 @@ static int launch_1e_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, co
                        const int *n_dev, float *loc, int *flag, const FitConst &fc)
 {
     switch (R) {
-    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 11: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 11, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 13: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 13, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 15: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 15, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 17: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 17, FAST>), grid, dim3(64), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 11: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 11, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 13: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 13, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 15: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 15, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 17: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 17, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
     default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the fitter (odd, 5..17)", R);
     }
     QCS_HIP(hipGetLastError());
@@ -625,7 +630,7 @@ int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wl
 {
     if (n <= 0) return QCS_OK;
     const FitConst fc = make_fit_const(p);
-    const dim3 grid((n + 63) / 64);
+    const dim3 grid((n + 127) / 128);   // two waves per work-group share the log table
     const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
     const bool fast = (p.FitMath == QCS_MATH_FAST);
     if (as3d)

@@ -2,8 +2,8 @@
 //
 // EXACT: explicit IEEE-754 single-precision operation sequences (Cody-Waite reduced exp with a
 // degree-7 Taylor polynomial; log with the range reduction on the bit pattern, f = m - 1 and
-// f + f^2 P(f), P of degree 8), so every IEEE implementation -- this GPU and the host --
-// evaluates them bit-identically.  These kernels
+// f + f^2 P(f), P of degree 8; the fitting losses' table-driven logs below), so every IEEE
+// implementation -- this GPU and the host -- evaluates them bit-identically.  These kernels
 // are compiled with -ffp-contract=off: only the fmaf calls written here are fused.
 // FAST: the gfx950 transcendental unit (v_exp_f32 / v_log_f32, base 2).
 #pragma once
@@ -89,8 +89,9 @@ __device__ __forceinline__ float qlogf(float x)
 }
 
 // ---------------------------------------------------------------------------------------------
-// Table-driven logarithm of the fitting loss.  x = m 2^e with m in [sqrt(1/2), sqrt(2)) as in
-// qlogf_normal; the top five bits of the reduced mantissa select one of 32 intervals j, whose
+// Table-driven logarithm of the multi-emitter fitting losses.  x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)) as in qlogf_normal; the top five bits of the reduced mantissa select one
+// of 32 intervals j, whose
 // table pair is (i_j, t_j) with i_j a float near 1 / (interval centre) and t_j = -log(i_j)
 // rounded to float.  i_j was chosen among the floats near the centre's reciprocal so that
 // -log(i_j) lies within 3e-4 ulp of t_j (accurate tables); interval 18 holds 1.0 and has the
@@ -155,6 +156,71 @@ __device__ __forceinline__ float logtab_u_any(float x, float c, const float2 *lt
     const float w = (tv.y + lr) + c;
     const float ef = (float)((((int)ix) >> 23) - (sub ? 23 : 0));
     float u = __builtin_fmaf(ef, kLn2F, w);
+    u = (x == 0.0f) ? -__builtin_inff() : u;
+    u = (x < 0.0f || x != x) ? __builtin_nanf("") : u;
+    u = (x == __builtin_inff()) ? x : u;
+    return u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Binade table log of the one-emitter loss (qcs_logtab.inc, written by logtab_gen.py).  For a
+// positive float x with bits b, g = (int)(b - 0x3f360000) >> 18 numbers its interval: binade
+// e = g >> 5 of x = m 2^e, m in [0.7109375, 1.421875), and one of 32 bit-pattern steps of m.
+// Binades -10..21, x in [kLogTabBLo, kLogTabBHi), have their own pair (i, t) at g + 320, with
+// t = -log(i) to within 4e-4 ulp (accurate tables) and i near the reciprocal of the interval's
+// centre; the interval around 1 has (1, 0).  There log x + c = (t + c) + log1p(r),
+// r = fma(x, i, -1), |r| < 2^-6 + 4e-4, with the same degree-4 log1p as above: the binade's
+// e ln2 is part of t, so the loss needs no exponent extraction, conversion or e ln2 FMA.
+// Outside that range x is scaled by 2^-e into binade 0 and e ln2 is added in one FMA.
+// Max error of log x alone: 1.2 ulp in range, 0.6 ulp outside.
+static __constant__ uint32_t kLogTabB[2048] = {
+#include "qcs_logtab.inc"
+};
+constexpr uint32_t kLogTabBK = 0x3f360000u;
+constexpr float kLogTabBLo = 0x1.6cp-11f;   // 0.7109375 * 2^-10
+constexpr float kLogTabBHi = 0x1.6cp21f;    // 0.7109375 * 2^22
+
+// Copy the 8 KB table into LDS; every thread of the block calls it.
+__device__ __forceinline__ void log_tabb_load(float2 *lt)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(kLogTabB);
+    uint4 *dst = reinterpret_cast<uint4 *>(lt);
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ int logtabb_index(uint32_t bx) { return (((int)(bx - kLogTabBK)) >> 18) + 320; }
+
+__device__ __forceinline__ float logtabb_core(float x, float2 tv, float c)
+{
+    const float r = __builtin_fmaf(x, tv.x, -1.0f);
+    const float r2 = r * r;
+    float p = __builtin_fmaf(r, -0.25f, 0.3333333432674407958984375f);
+    p = __builtin_fmaf(p, r, -0.5f);
+    const float lr = __builtin_fmaf(p, r2, r);
+    return (tv.y + c) + lr;
+}
+
+// log(x) + c for x in [kLogTabBLo, kLogTabBHi)
+__device__ __forceinline__ float logtabb_u(float x, float c, const float2 *lt)
+{
+    return logtabb_core(x, lt[logtabb_index(__float_as_uint(x))], c);
+}
+
+// logtabb_u for every x: outside the table's range x is scaled into binade 0 (subnormals first
+// by 2^23); zero gives -inf, negative and NaN arguments NaN, +inf gives +inf.  The table index
+// stays inside binade 0 for every bit pattern outside the range.
+__device__ __forceinline__ float logtabb_u_any(float x, float c, const float2 *lt)
+{
+    const bool inr = (x >= kLogTabBLo) && (x < kLogTabBHi);
+    const bool sub = (x > 0.0f) && (x < 1.17549435082228751e-38f);
+    const float xs = sub ? x * 8388608.0f : x;
+    const uint32_t bx = __float_as_uint(xs);
+    const uint32_t ix = bx - kLogTabBK;
+    const float xr = inr ? x : __uint_as_float(bx - (ix & 0xff800000u));
+    const float w = logtabb_core(xr, lt[logtabb_index(__float_as_uint(xr))], c);
+    const float ef = (float)((((int)ix) >> 23) - (sub ? 23 : 0));
+    float u = inr ? w : __builtin_fmaf(ef, kLn2F, w);
     u = (x == 0.0f) ? -__builtin_inff() : u;
     u = (x < 0.0f || x != x) ? __builtin_nanf("") : u;
     u = (x == __builtin_inff()) ? x : u;

@@ -40,6 +40,10 @@ PROTOS = {
     "orc_tlog_u": (_f, [_f, _f]),
     "orc_tlogf_v": (None, [_vp, _vp, C.c_long]),
     "orc_tlog_table": (None, [_vp]),
+    "orc_tlogbf": (_f, [_f]),
+    "orc_tlogb_u": (_f, [_f, _f]),
+    "orc_tlogbf_v": (None, [_vp, _vp, C.c_long]),
+    "orc_tlogb_table": (None, [_vp]),
     "orc_filter_taps": (None, [_i, _i, _i, _vp]),
     "orc_remove_background": (None, [_vp, _vp, _i, _i]),
     "orc_image_variance": (_f, [_vp, _i, _i]),

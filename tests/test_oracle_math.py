@@ -106,6 +106,75 @@ def test_tlogf_specials():
     assert np.isnan(l(float("nan"))) and np.isneginf(O.lib().orc_tlog_u(0.0, 3.0))
 
 
+def _tlogb(x):
+    y = np.zeros_like(x)
+    O.lib().orc_tlogbf_v(O.p(x), O.p(y), x.size)
+    return y
+
+
+def test_tlogbf_accuracy():
+    """The one-emitter losses' binade table log against double-precision log: every float of
+    [1/4, 4), every 7th float of [2^-11, 2^22) (the table's binades and their edges), a
+    geometric sweep of the whole range (the scaled path outside the table) and subnormals."""
+    for lo, hi, step, bound in ((0x3E800000, 0x40800000, 1, 1.25), (0x3A000000, 0x4A800000, 7, 1.25)):
+        x = np.arange(lo, hi, step, dtype=np.uint32).view(np.float32)
+        y = _tlogb(x)
+        exact = np.log(x.astype(np.float64))
+        nz = exact != 0.0
+        err = np.abs(y[nz].astype(np.float64) - exact[nz]) / _ulp(exact[nz])
+        print("tlogbf max ulp on [%g, %g)" % (x[0], x[-1]), err.max())
+        assert err.max() <= bound and y[x == 1.0].tolist() in ([], [0.0])
+    x = np.concatenate([np.geomspace(1.2e-38, 3e38, 2_000_001), np.geomspace(1e-45, 1e-38, 10_001)]).astype(np.float32)
+    x = x[(x > 0) & (x != 1.0)]
+    y = _tlogb(x)
+    exact = np.log(x.astype(np.float64))
+    err = np.abs(y.astype(np.float64) - exact) / _ulp(exact)
+    print("tlogbf max ulp, full range", err.max())
+    assert err.max() <= 1.25
+
+
+def test_tlogb_table_entries():
+    """Each pair (i, t) of binade e, interval j: i within 0.1% of the reciprocal of the interval's
+    centre, t the float nearest -log(i) to within 1e-3 ulp, |x i - 1| < 2^-6 + 4e-4 over the
+    interval's ends, and the interval holding 1 (binade 0, interval 18) exactly (1, 0)."""
+    tab = np.zeros(2048, np.uint32)
+    O.lib().orc_tlogb_table(O.p(tab))
+    inv, t = tab[0::2].view(np.float32).astype(np.float64), tab[1::2].view(np.float32).astype(np.float64)
+    k = 0x3F360000
+    for idx in range(1024):
+        g = idx - 320
+        lo_b = (k + (g << 18)) & 0xFFFFFFFF
+        lo = float(np.array([lo_b], np.uint32).view(np.float32)[0])
+        hi = float(np.array([lo_b + (1 << 18) - 1], np.uint32).view(np.float32)[0])
+        if lo <= 1.0 <= hi:
+            assert inv[idx] == 1.0 and t[idx] == 0.0 and idx == 320 + 18
+            continue
+        assert abs(inv[idx] * (lo + hi) / 2 - 1.0) < 1e-3, idx
+        assert abs(t[idx] + np.log(inv[idx])) <= 1e-3 * np.spacing(np.float32(abs(t[idx]))), idx
+        assert max(abs(lo * inv[idx] - 1.0), abs(hi * inv[idx] - 1.0)) < 2.0 ** -6 + 4e-4, idx
+
+
+def test_tlogb_u_adds_the_data_term():
+    """orc_tlogb_u(x, c) = log x + c with c = 1 - log q for model values near and far from q,
+    inside and outside the table's binades: within 1 ulp of log x plus 1 ulp of the result;
+    c = 0 is tlogbf."""
+    rng = np.random.default_rng(6)
+    q = rng.uniform(1.0, 5000.0, 20000).astype(np.float32)
+    x = (q * np.exp(rng.uniform(-12.0, 12.0, q.size))).astype(np.float32)
+    c = (1.0 - np.log(q.astype(np.float64))).astype(np.float32)
+    u = np.array([O.lib().orc_tlogb_u(float(a), float(b)) for a, b in zip(x, c)], np.float32)
+    exact = np.log(x.astype(np.float64)) + c.astype(np.float64)
+    bound = _ulp(np.log(x.astype(np.float64))) + _ulp(exact)
+    assert np.all(np.abs(u.astype(np.float64) - exact) <= bound)
+    assert np.array_equal(np.array([O.lib().orc_tlogb_u(float(a), 0.0) for a in x[:300]], np.float32), _tlogb(x[:300]))
+
+
+def test_tlogbf_specials():
+    l = O.lib().orc_tlogbf
+    assert np.isneginf(l(0.0)) and np.isnan(l(-1.0)) and np.isposinf(l(float("inf"))) and l(1.0) == 0.0
+    assert np.isnan(l(float("nan"))) and np.isneginf(O.lib().orc_tlogb_u(0.0, 3.0))
+
+
 def test_philox_known_answers():
     # Random123 known-answer vectors for Philox4x32-10
     import ctypes as C
