@@ -35,6 +35,8 @@ triples) through the frames -> rows path of the converged fitter (mixed 7/9/11 R
 work-queue fits, FP64 selection and CRLB), one HIP graph replay per 50-frame batch; frames/s,
 ROIs/s, accuracy of the first batch against the simulated emitters, and the learned warm start
 against the plain path on the 7 x 7 ROIs those frames yield (median of 5 timed calls each).
+`as3d_frames`: 500 astigmatic-3D 1024 x 1024 frames (9 x 9 ROIs, batches of 4 frames) through the
+same pipeline as the headline step, frames/s.
 `rois_100k`: the 100k seeded 7 x 7 ROIs of the single-emitter reference case, the oracle on every
 usable host core against the library, rows compared bit for bit.  `session_path`: the plug-ins'
 drop-in path, host frames through the online session with the result file, H2D included.
@@ -306,13 +308,21 @@ def multi_emitter_leg(n, seed, log):
     ctx.close()
     net.close()
     # evaluations: device-counted loss-and-gradient evaluations of the plain path (pixel evaluations
-    # = evaluations x 49); about 45 FLOP per pixel per evaluation (loss and 3K + 2 gradient terms)
+    # = evaluations x 49).  Per pixel a K-emitter evaluation costs 23 + 14 (K - 1) FLOP (FMA = 2:
+    # the model, the deviance term and the 3K + 2 gradient sums) and 3 transcendentals (exp of the
+    # row term aside); the count does not split by K, so the roofline below prices every
+    # evaluation at K = 1 and is a lower bound.
     evals = stp["evals"]
+    flop_lb = 23.0 * 49 * evals
+    ach_lb = flop_lb / (plain_ms * 1e-3) / 1e12
     return {"rois": n, "plain_fits_per_s": n / (plain_ms * 1e-3), "warm_fits_per_s": n / (warm_ms * 1e-3),
             "warm_speedup": plain_ms / warm_ms, "nn_ms": tw["nn_ms"], "plain_ms": plain_ms, "warm_ms": warm_ms,
             "timed_calls": len(plain_all), "plain_ms_all": plain_all, "warm_ms_all": warm_all,
             "plain_evaluations": int(evals), "plain_pixel_evaluations_per_s": evals * 49 / (plain_ms * 1e-3),
             "plain_iterations_mean": stp["iters"] / n,
+            "plain_roofline_lower_bound": {"bound": "valu", "achieved": ach_lb, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                           "frac": ach_lb / PEAK_FP32_TFLOPS, "flop_per_pixel_evaluation": 23,
+                                           "pixel_evaluations": int(evals) * 49},
             "plain": rp, "warm": rw, "rmse_ratio_warm_vs_plain": rw["rmse_px"] / rp["rmse_px"],
             "rmse_over_crlb_plain": rp["rmse_over_crlb"], "rmse_over_crlb_warm": rw["rmse_over_crlb"],
             "rmse_over_crlb_count_correct_plain": rp["rmse_over_crlb_count_correct"],
@@ -552,6 +562,51 @@ def session_leg(frames_dev, params, F, log, group=1):
             "host_bytes_per_s": F * h.shape[1] * h.shape[2] * 2 / dt}
 
 
+def as3d_frames_leg(F, stream, log, steps=3):
+    """The astigmatic-3D workload from frames: F simulated 1024 x 1024 frames (0.3 emitters/um^2,
+    z uniform in +-500 nm, photons 1000-8000, background 20-80), batches of 4 frames, through
+    extraction, the 9 x 9 elliptical one-emitter fit + 15 x 15 CRLB, frame-sorted compaction and
+    rendering (the same pipeline as the headline step); frames/s over `steps` timed passes."""
+    import torch
+    W = H = 1024
+    params = Q.default_params(ImageWidth=W, ImageHigh=H, ROISize=9, LocType=Q.LOCTYPE_AS3D, MultiEmitterFitEn=0,
+                              WLEEn=1, PixelSize=100.0, PixelZoom=10.0, QE=1.0, TotalFrameNum=F, BatchedImageNum=4,
+                              p1_XGY=-111.11, p1_XLY=-111.11, ZDepthCorrFactor=1.0)
+    Q.lib().qcs_params_update_sr_size(C.byref(params))
+    sp = Q.default_sim(width=W, height=H, density_per_px=0.003, astig=1, photons_min=1000.0, photons_max=8000.0,
+                       bg_min=20.0, bg_max=80.0, qe=1.0)
+    sh = C.c_void_p(stream.cuda_stream)
+    frames = torch.empty((F, H, W), dtype=torch.int16, device="cuda")
+    with torch.cuda.stream(stream):
+        for f0 in range(0, F, 50):
+            nf = min(50, F - f0)
+            Q.check(Q.lib().qcs_sim_frames_dev(C.byref(sp), 1002, 1 + f0, nf, C.c_void_p(frames[f0].data_ptr()), sh),
+                    "qcs_sim_frames_dev")
+    stream.synchronize()
+    pipe = Pipeline(params, frames, F, 1, stream, torch)
+    pipe.step()
+    torch.cuda.synchronize()
+    pipe.timing_enable()
+    t0 = time.perf_counter()
+    fits = rows = 0
+    for _ in range(steps):
+        n, m = pipe.step()
+        fits += n
+        rows += m
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tm = pipe.timing()
+    pipe.close()
+    return {"frames": F, "frame": [W, H], "roi": 9, "batch_frames": 4, "ms_per_pass": dt * 1e3 / steps,
+            "frames_per_s": F * steps / dt, "fits_per_s": fits / dt, "rois_per_pass": fits // steps,
+            "localizations_per_pass": rows // steps,
+            "fit_kernel_fits_per_s": tm["fit_1e_rois"] / (tm["fit_1e_ms"] * 1e-3),
+            "extract_ms_per_pass": tm["extract_ms"] / steps, "fit_ms_per_pass": tm["fit_1e_ms"] / steps,
+            "crlb_ms_per_pass": tm["crlb_ms"] / steps,
+            "data": "simulator seed 1002: 0.3 emitters/um^2, z U(-500, 500) nm, sigma_x/y = 1.2 sqrt(1 + ((z -+ 250)/400)^2), "
+                    "photons log-U(1000, 8000), background U(20, 80)"}
+
+
 class Pipeline:
     """The full-frame pipeline over F resident frames.  With lanes > 1 the frames are split into
     contiguous groups, each extracted, fitted and compacted by its own context on its own stream
@@ -620,6 +675,84 @@ class Pipeline:
             L["ctx"].close()
 
 
+def output_side_legs(pipe, params, dev, sh, W, H, F, torch):
+    """Statistics, consecutive-frame merging and drift correction of the step's localizations;
+    returns their three JSON blocks."""
+    # statistics of the step's localizations (histograms, means, activation density), one call over
+    # the frame-sorted rows of lane 0 (all frames when lanes == 1)
+    log("bench: statistics leg")
+    L0 = pipe.lanes[0]
+    st_rows = L0["m"]
+    pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
+    st_ms = (time.perf_counter() - t0) * 1e3 / reps
+    statistics = {"rows": int(st_rows), "ms": st_ms, "rows_per_s": st_rows / (st_ms * 1e-3),
+                  "valid": st["valid"], "density_total": st["density_total"],
+                  "loc_density_2d_per_um2": st["loc_density_2d"], "mean_total_photon": st["mean_total_photon"],
+                  "mean_loc_prec_nm": st["mean_loc_prec"]}
+
+    # consecutive-frame merging of the step's localizations (one call that gives everything back,
+    # 100 nm threshold), then removal of the merged-away rows
+    log("bench: consecutive-frame merging leg")
+    params_c = Q.QcsParams.from_buffer_copy(params)
+    params_c.ConsecFitEn = 1
+    params_c.ConsecFit_DistanceTh_nm = 100.0
+    ctx_c = Q.Context(params_c)
+    cons = Q.Consec(ctx_c)
+    c_out = torch.empty((st_rows, Q.REC_FLOATS), dtype=torch.float32, device=dev)
+    c_kept = torch.empty_like(c_out)
+    cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cm, ratio = cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
+    c_ms = (time.perf_counter() - t0) * 1e3 / reps
+    ck = ctx_c.compact_dev(c_out, cm, c_kept, sh)
+    torch.cuda.synchronize()
+    consec = {"rows": int(st_rows), "ms": c_ms, "rows_per_s": st_rows / (c_ms * 1e-3), "threshold_nm": 100.0,
+              "rows_after_merge": int(ck), "ontime_ratio": [float(x) for x in ratio]}
+
+    # drift correction by SR-image cross-correlation: a fixed structure (25,000 points, 12,500
+    # localizations per frame) drifting over 200 frames of 2048 x 2048 px, groups of 50 frames
+    log("bench: drift-correction leg")
+    d_rows_h, d_dx, d_dy = drift_workload(W, H, F, 25000, 77)
+    params_d = Q.QcsParams.from_buffer_copy(params)
+    ctx_d = Q.Context(params_d)
+    d_rows = torch.from_numpy(d_rows_h).to(dev)
+    corr = min(50, max(F // 4, 1))   # 50-frame groups at the default 200 frames
+    ctx_d.drift_correct_dev(d_rows.clone(), d_rows_h.shape[0], corr, stream=sh)   # warm-up (allocates the SR images)
+    ctx_d.timing_enable(True)
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        work = d_rows.clone()
+        torch.cuda.synchronize()
+        total_d, dfx, dfy, dgxy = ctx_d.drift_correct_dev(work, d_rows_h.shape[0], corr, stream=sh)
+    dr_ms = (time.perf_counter() - t0) * 1e3 / reps
+    td = ctx_d.timing()
+    cc_launches = max(td["drift_cc_launches"], 1)
+    cc_ms = td["drift_cc_ms"] / cc_launches
+    cc_flop = 2.0 * 47 * 47 * td["drift_cc_pixels"] / cc_launches
+    cc_tflops = cc_flop / (cc_ms * 1e-3) / 1e12
+    from_frames = [(1 + k * (corr // 2), k * (corr // 2) + corr) for k in range(dgxy.shape[0])]
+    mdx = np.array([d_dx[a - 1:b].mean() for a, b in from_frames])
+    mdy = np.array([d_dy[a - 1:b].mean() for a, b in from_frames])
+    drift = {"rows": int(d_rows_h.shape[0]), "frames": int(total_d), "corr_frames": corr, "groups": int(dgxy.shape[0]),
+             "ms": dr_ms, "rows_per_s": d_rows_h.shape[0] / (dr_ms * 1e-3),
+             "cc_kernel": {"avg_launch_ms": cc_ms, "launches_per_call": cc_launches // reps,
+                           "sr_pixels": td["drift_cc_pixels"] // cc_launches,
+                           "roofline": {"bound": "mfma", "achieved": cc_tflops, "peak": 157.3, "unit": "TFLOP/s",
+                                        "frac": cc_tflops / 157.3, "flop_per_launch": cc_flop}},
+             "group_shift_err_px": float(max(np.abs(dgxy[:, 0] - (mdx - mdx[0])).max(),
+                                             np.abs(dgxy[:, 1] - (mdy - mdy[0])).max()))}
+    ctx_d.timing_enable(False)
+    keep = {"cons": cons, "ctx_c": ctx_c, "ctx_d": ctx_d, "params_c": params_c, "td": td}
+    return statistics, consec, drift, keep
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -636,14 +769,21 @@ def main():
     ap.add_argument("--hd-frames", type=int, default=1000,
                     help="1024x1024 high-density frames of the converged frames leg (0 skips it)")
     ap.add_argument("--hd-batch", type=int, default=50, help="frames per HIP-graph batch of the high-density leg")
+    ap.add_argument("--as3d-frames", type=int, default=500,
+                    help="1024x1024 frames of the astigmatic-3D frames leg (0 skips it)")
     ap.add_argument("--session-frames", type=int, default=200,
                     help="host frames of the drop-in session-path leg (0 skips it)")
     ap.add_argument("--no-fast-leg", action="store_true", help="skip the fast-math pipeline leg")
     ap.add_argument("--lanes", type=int, default=1,
                     help="frame groups processed concurrently (own context, stream and host thread each)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="time only the headline step (no other legs, no CPU baseline): for profiling its kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.headline_only:
+        args.no_fast_leg = args.no_cpu_baseline = True
+        args.multi_rois = args.as3d_rois = args.hd_frames = args.session_frames = args.as3d_frames = 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the localization path runs on a single MI355X "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -768,77 +908,10 @@ def main():
                                   "dxy_px_p99": float(np.percentile(d, 99)) if d.size else 0.0}}
         pipe_f.close()
 
-    # statistics of the step's localizations (histograms, means, activation density), one call over
-    # the frame-sorted rows of lane 0 (all frames when lanes == 1)
-    log("bench: statistics leg")
-    L0 = pipe.lanes[0]
-    st_rows = L0["m"]
-    pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        st = pipe.ctx.loc_stats_dev(L0["out"], st_rows, sh)
-    st_ms = (time.perf_counter() - t0) * 1e3 / reps
-    statistics = {"rows": int(st_rows), "ms": st_ms, "rows_per_s": st_rows / (st_ms * 1e-3),
-                  "valid": st["valid"], "density_total": st["density_total"],
-                  "loc_density_2d_per_um2": st["loc_density_2d"], "mean_total_photon": st["mean_total_photon"],
-                  "mean_loc_prec_nm": st["mean_loc_prec"]}
-
-    # consecutive-frame merging of the step's localizations (one call that gives everything back,
-    # 100 nm threshold), then removal of the merged-away rows
-    log("bench: consecutive-frame merging leg")
-    params_c = Q.QcsParams.from_buffer_copy(params)
-    params_c.ConsecFitEn = 1
-    params_c.ConsecFit_DistanceTh_nm = 100.0
-    ctx_c = Q.Context(params_c)
-    cons = Q.Consec(ctx_c)
-    c_out = torch.empty((st_rows, Q.REC_FLOATS), dtype=torch.float32, device=dev)
-    c_kept = torch.empty_like(c_out)
-    cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        cm, ratio = cons.fit_dev(L0["out"], st_rows, True, c_out, sh)
-    c_ms = (time.perf_counter() - t0) * 1e3 / reps
-    ck = ctx_c.compact_dev(c_out, cm, c_kept, sh)
-    torch.cuda.synchronize()
-    consec = {"rows": int(st_rows), "ms": c_ms, "rows_per_s": st_rows / (c_ms * 1e-3), "threshold_nm": 100.0,
-              "rows_after_merge": int(ck), "ontime_ratio": [float(x) for x in ratio]}
-
-    # drift correction by SR-image cross-correlation: a fixed structure (25,000 points, 12,500
-    # localizations per frame) drifting over 200 frames of 2048 x 2048 px, groups of 50 frames
-    log("bench: drift-correction leg")
-    d_rows_h, d_dx, d_dy = drift_workload(W, H, F, 25000, 77)
-    params_d = Q.QcsParams.from_buffer_copy(params)
-    ctx_d = Q.Context(params_d)
-    d_rows = torch.from_numpy(d_rows_h).to(dev)
-    corr = min(50, max(F // 4, 1))   # 50-frame groups at the default 200 frames
-    ctx_d.drift_correct_dev(d_rows.clone(), d_rows_h.shape[0], corr, stream=sh)   # warm-up (allocates the SR images)
-    ctx_d.timing_enable(True)
-    reps = 2
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        work = d_rows.clone()
-        torch.cuda.synchronize()
-        total_d, dfx, dfy, dgxy = ctx_d.drift_correct_dev(work, d_rows_h.shape[0], corr, stream=sh)
-    dr_ms = (time.perf_counter() - t0) * 1e3 / reps
-    td = ctx_d.timing()
-    cc_launches = max(td["drift_cc_launches"], 1)
-    cc_ms = td["drift_cc_ms"] / cc_launches
-    cc_flop = 2.0 * 47 * 47 * td["drift_cc_pixels"] / cc_launches
-    cc_tflops = cc_flop / (cc_ms * 1e-3) / 1e12
-    from_frames = [(1 + k * (corr // 2), k * (corr // 2) + corr) for k in range(dgxy.shape[0])]
-    mdx = np.array([d_dx[a - 1:b].mean() for a, b in from_frames])
-    mdy = np.array([d_dy[a - 1:b].mean() for a, b in from_frames])
-    drift = {"rows": int(d_rows_h.shape[0]), "frames": int(total_d), "corr_frames": corr, "groups": int(dgxy.shape[0]),
-             "ms": dr_ms, "rows_per_s": d_rows_h.shape[0] / (dr_ms * 1e-3),
-             "cc_kernel": {"avg_launch_ms": cc_ms, "launches_per_call": cc_launches // reps,
-                           "sr_pixels": td["drift_cc_pixels"] // cc_launches,
-                           "roofline": {"bound": "mfma", "achieved": cc_tflops, "peak": 157.3, "unit": "TFLOP/s",
-                                        "frac": cc_tflops / 157.3, "flop_per_launch": cc_flop}},
-             "group_shift_err_px": float(max(np.abs(dgxy[:, 0] - (mdx - mdx[0])).max(),
-                                             np.abs(dgxy[:, 1] - (mdy - mdy[0])).max()))}
-    ctx_d.timing_enable(False)
+    statistics = consec = drift = None
+    keep = {}   # the output-side legs' contexts, used again by the CPU-baseline checks
+    if not args.headline_only:
+        statistics, consec, drift, keep = output_side_legs(pipe, params, dev, sh, W, H, F, torch)
 
     multi = None
     if args.multi_rois > 0:
@@ -863,6 +936,12 @@ def main():
     if args.hd_frames > 0:
         log("bench: high-density frames leg (%d frames 1024x1024, graphs of %d frames)" % (args.hd_frames, args.hd_batch))
         hd = hd_frames_leg(args.hd_frames, args.hd_batch, 1003, log)
+
+    as3d_frames = None
+    if args.as3d_frames > 0:
+        log("bench: astigmatic-3D frames leg (%d frames 1024x1024, 9x9 ROIs)" % args.as3d_frames)
+        as3d_frames = as3d_frames_leg(args.as3d_frames, stream, log)
+        log("bench: astigmatic-3D frames %.0f frames/s, %.3g fits/s" % (as3d_frames["frames_per_s"], as3d_frames["fits_per_s"]))
 
     cpu = None
     parity = None
@@ -897,13 +976,13 @@ def main():
         # consecutive-frame merging: the oracle (the reference's per-row scan, one thread) on the
         # rows of the first 4 frames, against the device call on the same rows
         sub = rows_h[rows_h[:, 11] <= 4]
-        ref_c = O.Consec(params_c)
+        ref_c = O.Consec(keep["params_c"])
         t0 = time.perf_counter()
         want, _ = ref_c.fit(sub, True)
         cpu_ms = (time.perf_counter() - t0) * 1e3
         d_sub = torch.from_numpy(np.ascontiguousarray(sub)).to(dev)
         d_o = torch.empty_like(d_sub)
-        gm, _ = cons.fit_dev(d_sub, sub.shape[0], True, d_o, sh)
+        gm, _ = keep["cons"].fit_dev(d_sub, sub.shape[0], True, d_o, sh)
         got = d_o[:gm].cpu().numpy()
         nan = np.isnan(got)
         consec["cpu_baseline"] = {"value": sub.shape[0] / (cpu_ms * 1e-3), "unit": "rows/s", "cores": 1, "kind": "port",
@@ -922,6 +1001,7 @@ def main():
         drift["cpu_baseline"] = {"value": 4 * 768 * 768 * 47 * 47 / (cpu_ms * 1e-3), "unit": "SR pixel-shift products/s",
                                  "cores": 1, "kind": "port",
                                  "sample": "96x96 px crop, 60 frames, 4 correlations of 768x768 SR px (%.1f s)" % (cpu_ms / 1e3)}
+        td = keep["td"]
         drift["pixel_shift_products_per_s"] = td["drift_cc_pixels"] * 47 * 47 / (td["drift_cc_ms"] * 1e-3)
         d_crop = torch.from_numpy(crop.copy()).to(dev)
         ctx_crop = Q.Context(p_crop)
@@ -942,13 +1022,13 @@ def main():
                        "rois_per_step": n_last, "lanes": args.lanes},
             "roofline": roofline, "cpu_baseline": cpu, "stages": stages, "parity_sample": parity,
             "fast_math": fast, "multi_emitter": multi, "as3d_multi_emitter": as3d, "hd_frames": hd, "rois_100k": c0,
-            "session_path": sess,
+            "session_path": sess, "as3d_frames": as3d_frames,
             "statistics": statistics,
             "consecutive_merge": consec,
             "drift_correction": drift}
-    cons.close()
-    ctx_c.close()
-    ctx_d.close()
+    for k in ("cons", "ctx_c", "ctx_d"):
+        if k in keep:
+            keep[k].close()
     pipe.close()
     print(json.dumps(line), flush=True)
 

@@ -364,7 +364,7 @@ int plain_path(qcs_ctx *ctx, int kmax, const uint16_t *rois, const int *idx, int
 // Network outputs [n][16] (nn.hip): 3 count logits, (x/R, y/R, N/(2 pi scale)) for 3 slots
 // (brightest first), s - 1.3, (b - bhat)/scale.  The warm path keeps the selection rule of the
 // plain path and changes where the fits start and which fits run:
-//   - every ROI's one-emitter fit starts from the brightest slot;
+//   - every ROI's one-emitter fit starts from the heuristic, as on the plain path;
 //   - a count predicted with probability >= pmin caps the ROI's count: predicted single ROIs skip
 //     the multi-emitter fits, predicted doubles the three-emitter fit;
 //   - the others get the two-emitter fit, started from the network's two brightest slots where it
@@ -611,14 +611,12 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m.out, m.norm, n, R, ctx->tune.warm_pmin, m.tw1,
                        m.tw2, m.cs, m.khat, cnt, m.l2w, m.l2a);
     QCS_HIP(hipGetLastError());
-    const CFitConst c1 = cfit_const(p, ctx->tune, 1, CF_INIT_HEUR), c2 = cfit_const(p, ctx->tune, 2, CF_INIT_ADD);
+    const CFitConst c2 = cfit_const(p, ctx->tune, 2, CF_INIT_ADD);
     const dim3 grid((n + 63) / 64), block(64);
-    // one-emitter fits: warm start, fallback from the heuristic
-    if ((rc = cfit_launch(ctx, 1, d_rois, nullptr, n, nullptr, CF_INIT_GIVEN, m.tw1, 5, m.t1, 5, m.info1, m.l1, s))) return rc;
-    hipLaunchKernelGGL((warm_check_kernel<1, 7>), grid, block, 0, s, d_rois, nullptr, n, nullptr, m.t1, m.info1, nullptr, c1,
-                       cnt + WC_F1, m.f1);
-    QCS_HIP(hipGetLastError());
-    if ((rc = cfit_launch(ctx, 1, d_rois, m.f1, n, cnt + WC_F1, CF_INIT_HEUR, nullptr, 0, m.t1, 5, m.info1, m.l1, s))) return rc;
+    // one-emitter fits from the heuristic start, as on the plain path: their time is the tail of
+    // wrong-model fits of multi-emitter ROIs, which a warm start does not shorten, and a plain
+    // start needs no fallback pass
+    if ((rc = cfit_launch(ctx, 1, d_rois, nullptr, n, nullptr, CF_INIT_HEUR, nullptr, 0, m.t1, 5, m.info1, m.l1, s))) return rc;
     // two-emitter fits of the ROIs not confidently single
     if (ctx->tune.queue) {   // both starts in one persistent launch
         CfqSet xs[2];

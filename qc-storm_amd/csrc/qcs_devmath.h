@@ -57,8 +57,6 @@ __device__ __forceinline__ float qlogf_normal(float x)
     return __builtin_fmaf(ef, 0.693145751953125f, __builtin_fmaf(ef, 1.428606765330187e-06f, r));
 }
 
-// true when x is a positive normal number (v_cmp_class)
-__device__ __forceinline__ bool is_pos_normal(float x) { return __builtin_amdgcn_classf(x, 0x100); }
 
 // log(x) for every x: zero, negative, subnormal, infinite and NaN arguments included
 __device__ __forceinline__ float qlogf(float x)
@@ -231,8 +229,6 @@ __device__ __forceinline__ float fexpf(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ float flogf(float x) { return __builtin_amdgcn_logf(x) * 0.693147180559945309f; }
 
 template <bool FAST> __device__ __forceinline__ float vexp_nonpos(float x) { return FAST ? fexpf(x) : qexpf_nonpos(x); }
-// fast-path log: the caller checks is_pos_normal on the argument and recomputes with qlogf if not
-template <bool FAST> __device__ __forceinline__ float vlog_normal(float x) { return FAST ? flogf(x) : qlogf_normal(x); }
-template <bool FAST> __device__ __forceinline__ float vlog(float x) { return FAST ? flogf(x) : qlogf(x); }
+
 
 }  // namespace qcs
