@@ -55,6 +55,9 @@ float orc_tlogb_u(float x, float c);
 float orc_tlogbf(float x);
 void  orc_tlogbf_v(const float *x, float *y, long n);
 void  orc_tlogb_table(uint32_t out[2048]);
+/* the CRLB's narrow-range division by dh = 0.02 and a count of its differences from x / 0.02f */
+float orc_div_dh_fast(float x);
+long long orc_div_dh_scan(uint32_t b0, uint32_t b1);
 
 /* ---------------- extraction (DLL/SubregionExtraction) ---------------- */
 void  orc_filter_taps(int roi_size, int loc_type, int multi, float taps[5]);

@@ -190,6 +190,31 @@ float orc_tlogbf(float x) { return orc_tlogb_u(x, 0.0f); }
 void  orc_tlogbf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_tlogbf(x[i]); }
 void  orc_tlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_tlogf(x[i]); }
 
+/* The CRLB's division by the step dh = 0.02 as the library's narrow-range form computes it:
+ * y = x * 50 corrected by one fma residual (50 is 1 / 0.02f rounded to float).  Test
+ * infrastructure: orc_div_dh_scan counts the floats with bit patterns in [b0, b1) where it differs
+ * from the IEEE quotient x / 0.02f. */
+float orc_div_dh_fast(float x)
+{
+    const float y = x * 50.0f;
+    return fmaf(fmaf(-y, 0.02f, x), 50.0f, y);
+}
+
+long long orc_div_dh_scan(uint32_t b0, uint32_t b1)
+{
+    long long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (long long b = b0; b < (long long)b1; b++) {
+        const uint32_t u = (uint32_t)b;
+        float x, q, r;
+        memcpy(&x, &u, sizeof x);
+        q = orc_div_dh_fast(x);
+        r = x / 0.02f;
+        if (memcmp(&q, &r, sizeof q) != 0) bad++;
+    }
+    return bad;
+}
+
 void orc_qexpf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_qexpf(x[i]); }
 void orc_qlogf_v(const float *x, float *y, long n) { for (long i = 0; i < n; i++) y[i] = orc_qlogf(x[i]); }
 

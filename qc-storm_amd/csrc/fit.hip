@@ -548,12 +548,58 @@ fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
 // ---------------------------------------------------------------------------------------------
 // Diagonal-Fisher CRLB of x and y (GaussianCRLB_Calc_top, LocPrecCalc_GaussianCRLB.h:64-128):
 // numeric second derivatives of the model and its log with dh = 0.02 on a G x G grid.
-template <int G>
+template <int G, bool NARROW>
 __device__ __forceinline__ float crlb_psf(int r, int c, float peak, float ix, float iy, float x0, float y0, float B)
 {
     const float cy = (float)(r - G / 2), cx = (float)(c - G / 2);
     const float dx = cx - x0, dy = cy - y0;
-    return __builtin_fmaf(peak, qexpf(-__builtin_fmaf(dx * dx, ix, (dy * dy) * iy)), B);
+    const float arg = -__builtin_fmaf(dx * dx, ix, (dy * dy) * iy);
+    return __builtin_fmaf(peak, NARROW ? qexpf_nonpos(arg) : qexpf(arg), B);
+}
+
+// x / 0.02f.  NARROW: as y = x * 50 corrected by one fma residual, which equals the IEEE quotient
+// for x = 0 and every |x| in [1e-30, 1e30] (checked over all floats); the Fisher sums' numerators
+// stay there when every model value lies in [1, 1e20].
+template <bool NARROW>
+__device__ __forceinline__ float div_dh(float x)
+{
+    if (!NARROW) return x / 0.02f;
+    const float y = x * 50.0f;
+    return __builtin_fmaf(__builtin_fmaf(-y, 0.02f, x), 50.0f, y);
+}
+
+// Diagonal Fisher sums of x and y.  NARROW (every model value in [1, 1e20], the argument of each
+// exponential <= 0): qexpf_nonpos, qlogf_normal and div_dh<true> give the same bits as the
+// general forms there.
+template <int G, bool NARROW>
+__device__ __forceinline__ void crlb_fisher(float peak, float ix, float iy, float be, float &fx, float &fy)
+{
+    const float dh = 0.02f;
+    const float xp = 0.0f + dh, xn = (0.0f + dh) - 2.0f * dh;
+    auto lg = [](float m) { return NARROW ? qlogf_normal(m) : qlogf(m); };
+    fx = 0.0f;
+    fy = 0.0f;
+    for (int r = 0; r < G; r++) {
+#pragma unroll
+        for (int c = 0; c < G; c++) {
+            const float m0 = crlb_psf<G, NARROW>(r, c, peak, ix, iy, 0.0f, 0.0f, be);
+            const float l0 = lg(m0);
+            {
+                const float mp = crlb_psf<G, NARROW>(r, c, peak, ix, iy, xp, 0.0f, be);
+                const float mn = crlb_psf<G, NARROW>(r, c, peak, ix, iy, xn, 0.0f, be);
+                const float d2 = div_dh<NARROW>(div_dh<NARROW>(__builtin_fmaf(-2.0f, m0, mp) + mn));
+                const float d2l = div_dh<NARROW>(div_dh<NARROW>(__builtin_fmaf(-2.0f, l0, lg(mp)) + lg(mn)));
+                fx = fx + __builtin_fmaf(-m0, d2l, d2);
+            }
+            {
+                const float mp = crlb_psf<G, NARROW>(r, c, peak, ix, iy, 0.0f, xp, be);
+                const float mn = crlb_psf<G, NARROW>(r, c, peak, ix, iy, 0.0f, xn, be);
+                const float d2 = div_dh<NARROW>(div_dh<NARROW>(__builtin_fmaf(-2.0f, m0, mp) + mn));
+                const float d2l = div_dh<NARROW>(div_dh<NARROW>(__builtin_fmaf(-2.0f, l0, lg(mp)) + lg(mn)));
+                fy = fy + __builtin_fmaf(-m0, d2l, d2);
+            }
+        }
+    }
 }
 
 template <int G>
@@ -573,30 +619,10 @@ __global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n
         const float peak = (1.0f / (((2.0f * pi) * sx) * sy)) * te;
         const float ix = 1.0f / ((2.0f * sx) * sx);
         const float iy = 1.0f / ((2.0f * sy) * sy);
-        const float dh = 0.02f;
-        const float xp = 0.0f + dh, xn = (0.0f + dh) - 2.0f * dh;
-        float fx = 0.0f, fy = 0.0f;
-        for (int r = 0; r < G; r++) {
-#pragma unroll
-            for (int c = 0; c < G; c++) {
-                const float m0 = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, 0.0f, be);
-                const float l0 = qlogf(m0);
-                {
-                    const float mp = crlb_psf<G>(r, c, peak, ix, iy, xp, 0.0f, be);
-                    const float mn = crlb_psf<G>(r, c, peak, ix, iy, xn, 0.0f, be);
-                    const float d2 = ((__builtin_fmaf(-2.0f, m0, mp) + mn) / dh) / dh;
-                    const float d2l = ((__builtin_fmaf(-2.0f, l0, qlogf(mp)) + qlogf(mn)) / dh) / dh;
-                    fx = fx + __builtin_fmaf(-m0, d2l, d2);
-                }
-                {
-                    const float mp = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, xp, be);
-                    const float mn = crlb_psf<G>(r, c, peak, ix, iy, 0.0f, xn, be);
-                    const float d2 = ((__builtin_fmaf(-2.0f, m0, mp) + mn) / dh) / dh;
-                    const float d2l = ((__builtin_fmaf(-2.0f, l0, qlogf(mp)) + qlogf(mn)) / dh) / dh;
-                    fy = fy + __builtin_fmaf(-m0, d2l, d2);
-                }
-            }
-        }
+        float fx, fy;
+        // model values lie in [be, peak + be] with be > 1
+        if (peak + be < 1e20f) crlb_fisher<G, true>(peak, ix, iy, be, fx, fy);
+        else crlb_fisher<G, false>(peak, ix, iy, be, fx, fy);
         fx = (fx == 0.0f) ? 1.0f : fx;
         fy = (fy == 0.0f) ? 1.0f : fy;
         stdx = sqrtf(1.0f / fx) * PS;

@@ -44,6 +44,8 @@ PROTOS = {
     "orc_tlogb_u": (_f, [_f, _f]),
     "orc_tlogbf_v": (None, [_vp, _vp, C.c_long]),
     "orc_tlogb_table": (None, [_vp]),
+    "orc_div_dh_fast": (_f, [_f]),
+    "orc_div_dh_scan": (C.c_longlong, [C.c_uint32, C.c_uint32]),
     "orc_filter_taps": (None, [_i, _i, _i, _vp]),
     "orc_remove_background": (None, [_vp, _vp, _i, _i]),
     "orc_image_variance": (_f, [_vp, _i, _i]),

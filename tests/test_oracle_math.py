@@ -175,6 +175,29 @@ def test_tlogbf_specials():
     assert np.isnan(l(float("nan"))) and np.isneginf(O.lib().orc_tlogb_u(0.0, 3.0))
 
 
+def _bits(v):
+    return int(np.array([v], np.float32).view(np.uint32)[0])
+
+
+def test_crlb_division_form_equals_ieee_quotient():
+    """The CRLB's narrow-range division x * 50 + fma residual equals x / 0.02f for every float
+    with |x| in [1e-30, 1e30] (the form is odd in x, so the positive half decides), and differs
+    outside that range, where the kernel keeps the division."""
+    lib = O.lib()
+    assert lib.orc_div_dh_scan(_bits(1e-30), _bits(1e30) + 1) == 0
+    assert lib.orc_div_dh_fast(0.0) == 0.0 and lib.orc_div_dh_fast(-0.0) == 0.0
+    # the range is not vacuous: the form fails somewhere below it and above it
+    assert lib.orc_div_dh_scan(1, _bits(1e-30)) > 0
+    assert lib.orc_div_dh_scan(_bits(1e30) + 1, _bits(float("inf")) + 1) > 0
+    # spot values against numpy's IEEE float32 quotient, negative side included
+    rng = np.random.default_rng(5)
+    x = (rng.uniform(-30, 30, 2000).astype(np.float64))
+    x = (np.sign(rng.uniform(-1, 1, 2000)) * 10.0 ** x).astype(np.float32)
+    want = x / np.float32(0.02)
+    got = np.array([lib.orc_div_dh_fast(float(v)) for v in x], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 def test_philox_known_answers():
     # Random123 known-answer vectors for Philox4x32-10
     import ctypes as C
