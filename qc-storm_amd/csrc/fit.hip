@@ -419,10 +419,11 @@ __device__ __forceinline__ float zpoly(const float (&pc)[5], float d, float corr
 // One-emitter fitting kernel.  Lane i fits ROI idx[i] (or i when idx is null) and writes its
 // 12-float row; flag[roi] (if non-null) is set to 1 when a rejected fit qualifies for the
 // two-emitter stage (valid background and SNR > 6, bfgsMLE_GS2D_core.h:279-297).
-// Up to 7x7 the register budget is held to two waves per SIMD: the exact-math kernel then spills a
-// few registers outside its loss loop but runs 1.6x faster than at its natural one wave.
+// Up to 9x9 the register budget is held to two waves per SIMD (256 VGPRs).  The 7x7 exact-math
+// kernel fits without spills; at 9x9 the kernel spills 30 (2D) or 65 (elliptical) registers outside
+// its loss loop and still runs 1.5-1.6x faster than at its natural one wave.
 template <int MODEL, int R, bool FAST>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(R <= 7 ? 2 : 1, R <= 7 ? 2 : 10)))
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(R <= 9 ? 2 : 1, R <= 9 ? 2 : 10)))
 fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
                                                     const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
                                                     float *__restrict__ loc, int *__restrict__ flag, FitConst fc)

@@ -145,78 +145,87 @@ __global__ void __launch_bounds__(256) render_bin_kernel(const float *__restrict
         }
 }
 
-// one work-group per tile: LDS accumulation, then img += tile (and zimg += z tile in 3D)
-__global__ void __launch_bounds__(256) render_tile_kernel(const float *__restrict__ loc, RenderConst c, const int *__restrict__ tcnt,
-                                                          const int *__restrict__ toff, const int *__restrict__ list,
-                                                          float *__restrict__ img, float *__restrict__ zimg)
+// One work-group of RBLK threads per tile: LDS accumulation, then img += tile (and zimg += z tile
+// with ZON).  Each wave draws one listed localization at a time, one footprint pixel per lane
+// (the row's fields are wave-uniform).  An interior tile of an aligned image (rows 16-byte aligned,
+// width a multiple of 4) reads its image pixels, four per thread, before drawing, so the HBM
+// latency of the read-modify-write overlaps the LDS work.
+constexpr int RBLK = 256;
+constexpr int RVEC = RTILE * RTILE / (4 * RBLK);   // float4 image words per thread
+template <bool ZON>
+__global__ void __launch_bounds__(RBLK) render_tile_kernel(const float *__restrict__ loc, RenderConst c, const int *__restrict__ tcnt,
+                                                           const int *__restrict__ toff, const int *__restrict__ list,
+                                                           float *__restrict__ img, float *__restrict__ zimg)
 {
     __shared__ float acc[RTILE * RTILE];
-    __shared__ float zacc[RTILE * RTILE];
+    __shared__ float zacc[ZON ? RTILE * RTILE : 1];
     const int t = blockIdx.x;
     const int cnt = tcnt[t];
     if (cnt == 0) return;   // uniform over the work-group
-    const bool zon = c.is3d && zimg;
     const int x0 = (t % c.tiles_x) * RTILE, y0 = (t / c.tiles_x) * RTILE;
-    for (int p = threadIdx.x; p < RTILE * RTILE; p += blockDim.x) {
+    const bool aligned = ((reinterpret_cast<uintptr_t>(img) | (ZON ? reinterpret_cast<uintptr_t>(zimg) : 0)) & 15) == 0;
+    const bool vec = aligned && (c.srw & 3) == 0 && x0 + RTILE <= c.srw && y0 + RTILE <= c.srh;   // uniform
+    float4 v[RVEC], u[RVEC];
+    if (vec) {
+#pragma unroll
+        for (int j = 0; j < RVEC; j++) {
+            const int p = 4 * (threadIdx.x + j * RBLK);
+            const size_t g = (size_t)(y0 + p / RTILE) * c.srw + (x0 + p % RTILE);
+            v[j] = *reinterpret_cast<const float4 *>(img + g);
+            if (ZON) u[j] = *reinterpret_cast<const float4 *>(zimg + g);
+        }
+    }
+    for (int p = threadIdx.x; p < RTILE * RTILE; p += RBLK) {
         acc[p] = 0.0f;
-        if (zon) zacc[p] = 0.0f;
+        if (ZON) zacc[p] = 0.0f;
     }
     __syncthreads();
     const int *lst = list + toff[t];
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int a = lane / 7, b = lane - 7 * (lane / 7);   // footprint pixel of this lane (lanes < 49)
+    for (int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); k < cnt; k += RBLK / 64) {
         Splat q;
         render_prep(loc + (size_t)lst[k] * QCS_REC_FLOATS, c, q);   // listed, hence drawn
         if (q.one) {
-            atomicAdd(&acc[(q.iy - y0) * RTILE + (q.ix - x0)], q.pp);
+            if (lane == 0) atomicAdd(&acc[(q.iy - y0) * RTILE + (q.ix - x0)], q.pp);
             continue;
         }
-#pragma unroll
-        for (int a = 0; a < 7; a++) {
-            const int py = q.iy - 3 + a - y0;
-            if (py < 0 || py >= RTILE) continue;
-            const float dy = (float)a - q.fy;
-#pragma unroll
-            for (int b = 0; b < 7; b++) {
-                const int px = q.ix - 3 + b - x0;
-                if (px < 0 || px >= RTILE) continue;
-                const float dx = (float)b - q.fx;
-                const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * q.rs);
-                atomicAdd(&acc[py * RTILE + px], q.pp * psf);
-                if (zon) atomicAdd(&zacc[py * RTILE + px], (q.z * q.pp) * psf);
-            }
+        const int py = q.iy - 3 + a - y0, px = q.ix - 3 + b - x0;
+        if (lane < 49 && py >= 0 && py < RTILE && px >= 0 && px < RTILE) {
+            const float dy = (float)a - q.fy, dx = (float)b - q.fx;
+            const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * q.rs);
+            atomicAdd(&acc[py * RTILE + px], q.pp * psf);
+            if (ZON) atomicAdd(&zacc[py * RTILE + px], (q.z * q.pp) * psf);
         }
     }
     __syncthreads();
-    // the tile is owned by this work-group: plain coalesced read-modify-write, four pixels per
-    // thread when rows start 16-byte aligned (aligned images, width a multiple of 4) and the
-    // tile lies inside the image
-    const bool aligned = ((reinterpret_cast<uintptr_t>(img) | (zon ? reinterpret_cast<uintptr_t>(zimg) : 0)) & 15) == 0;
-    if (aligned && (c.srw & 3) == 0 && x0 + RTILE <= c.srw && y0 + RTILE <= c.srh) {   // uniform
-        for (int p = 4 * threadIdx.x; p < RTILE * RTILE; p += 4 * blockDim.x) {
+    // the tile is owned by this work-group: plain coalesced writes
+    if (vec) {
+#pragma unroll
+        for (int j = 0; j < RVEC; j++) {
+            const int p = 4 * (threadIdx.x + j * RBLK);
             const size_t g = (size_t)(y0 + p / RTILE) * c.srw + (x0 + p % RTILE);
-            float4 v = *reinterpret_cast<const float4 *>(img + g);
-            v.x += acc[p];
-            v.y += acc[p + 1];
-            v.z += acc[p + 2];
-            v.w += acc[p + 3];
-            *reinterpret_cast<float4 *>(img + g) = v;
-            if (zon) {
-                float4 u = *reinterpret_cast<const float4 *>(zimg + g);
-                u.x += zacc[p];
-                u.y += zacc[p + 1];
-                u.z += zacc[p + 2];
-                u.w += zacc[p + 3];
-                *reinterpret_cast<float4 *>(zimg + g) = u;
+            v[j].x += acc[p];
+            v[j].y += acc[p + 1];
+            v[j].z += acc[p + 2];
+            v[j].w += acc[p + 3];
+            *reinterpret_cast<float4 *>(img + g) = v[j];
+            if (ZON) {
+                u[j].x += zacc[p];
+                u[j].y += zacc[p + 1];
+                u[j].z += zacc[p + 2];
+                u[j].w += zacc[p + 3];
+                *reinterpret_cast<float4 *>(zimg + g) = u[j];
             }
         }
         return;
     }
-    for (int p = threadIdx.x; p < RTILE * RTILE; p += blockDim.x) {
+    for (int p = threadIdx.x; p < RTILE * RTILE; p += RBLK) {
         const int gy = y0 + p / RTILE, gx = x0 + p % RTILE;
         if (gx >= c.srw || gy >= c.srh) continue;
         const size_t g = (size_t)gy * c.srw + gx;
         img[g] += acc[p];
-        if (zon) zimg[g] += zacc[p];
+        if (ZON) zimg[g] += zacc[p];
     }
 }
 
@@ -265,7 +274,10 @@ int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *
     hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, nullptr, nullptr, nullptr);
     if ((rc = scan_exclusive(tcnt, (int)ntile, toff, tsum, s))) return rc;
     hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, toff, cursor, list);
-    hipLaunchKernelGGL(render_tile_kernel, dim3((unsigned)ntile), dim3(256), 0, s, d_loc, c, tcnt, toff, list, d_img, d_zimg);
+    if (c.is3d && d_zimg)
+        hipLaunchKernelGGL(render_tile_kernel<true>, dim3((unsigned)ntile), dim3(RBLK), 0, s, d_loc, c, tcnt, toff, list, d_img, d_zimg);
+    else
+        hipLaunchKernelGGL(render_tile_kernel<false>, dim3((unsigned)ntile), dim3(RBLK), 0, s, d_loc, c, tcnt, toff, list, d_img, nullptr);
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
