@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
     const bool valid = j < n;
     const int roi = valid ? (idx ? idx[j] : j) : 0;
     float *qs = qsh + lane;
-    float *D = Dsh + lane;
+    DLds D{Dsh + lane};
     {
         const uint16_t *src = rois + (size_t)roi * WR;
         for (int k = 0; k < RR; k++) {
@@ -69,12 +69,12 @@ __global__ void __launch_bounds__(64) cfit_kernel(const uint16_t *__restrict__ r
     float L = cf_lossgrad<KK, R, true>(qs, th, g);
     // starting inverse Hessian: the inverse Fisher information at the initial point (identity if
     // it is not positive definite; the first step is then normalised)
-    bool first = !cf_fisher_inverse<KK, R>(th, D);
+    bool first = !cf_fisher_inverse<KK, R>(th, D.p);
     if (first) {
 #pragma unroll
         for (int i = 0; i < P; i++)
 #pragma unroll
-            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P)] = (i == jj) ? 1.0f : 0.0f;
     }
     int iters = 0, flags = 0, evals = 1, pin_run = 0;
     bool done = !valid;

@@ -61,10 +61,33 @@ __device__ __forceinline__ float cf_scale(int i)
 // packed upper-triangular index of (i, j), i <= j
 __device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
 
+// Where a lane keeps its observed photons q and its packed inverse Hessian D.  Every D access
+// uses a compile-time index (unrolled loops), so the entries a DSplit keeps in registers stay
+// there.
+struct QLds {   // q of pixel i at p[i * 64] (lane-interleaved floats)
+    const float *p;
+    __device__ __forceinline__ float operator()(int i) const { return p[i * 64]; }
+};
+struct QLds16 {   // raw counts at p[i * 64], converted on use exactly as cf_load_q converts them
+    const uint16_t *p;
+    float offset, kadc, inv_qe;
+    __device__ __forceinline__ float operator()(int i) const { return fmaxf(((float)p[i * 64] - offset) * kadc, 1.0f) * inv_qe; }
+};
+struct DLds {   // D[i] at p[i * 64]
+    float *p;
+    __device__ __forceinline__ float &operator[](int i) const { return p[i * 64]; }
+};
+template <int NREG>
+struct DSplit {   // D[i] in registers for i < NREG, else at p[(i - NREG) * 64]
+    float r[NREG > 0 ? NREG : 1];
+    float *p;
+    __device__ __forceinline__ float &operator[](int i) { return (NREG > 0 && i < NREG) ? r[i] : p[(i - NREG) * 64]; }
+};
+
 // loss sum((mu - q) - q ln(mu / q)) (the Poisson negative log-likelihood up to a constant) and,
 // if G, its gradient with respect to theta (physical units)
-template <int KK, int R, bool G>
-__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<KK>::P], float (&g)[CP<KK>::P])
+template <int KK, int R, bool G, class QA>
+__device__ __forceinline__ float cf_lossgrad_q(const QA &qa, const float (&th)[CP<KK>::P], float (&g)[CP<KK>::P])
 {
     using M = CM<KK>;
     constexpr int K = M::K;
@@ -105,7 +128,7 @@ __device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[
                     mu += gk[k];
                 }
                 mu = fmaxf(mu, 1e-6f);
-                const float q = qs[(r * R + c) * 64];
+                const float q = qa(r * R + c);
                 // deviance form: terms are O(1), so the FP32 sum resolves loss changes of ~1e-4
                 L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
                 if (G) {
@@ -173,7 +196,7 @@ __device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[
                     mu += gk[k];
                 }
                 mu = fmaxf(mu, 1e-6f);
-                const float q = qs[(r * R + c) * 64];
+                const float q = qa(r * R + c);
                 L += (mu - q) - q * __logf(mu * __builtin_amdgcn_rcpf(q));
                 if (G) {
                     const float w = 1.0f - q * __builtin_amdgcn_rcpf(mu);
@@ -203,6 +226,12 @@ __device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[
         }
         return L;
     }
+}
+
+template <int KK, int R, bool G>
+__device__ __forceinline__ float cf_lossgrad(const float *qs, const float (&th)[CP<KK>::P], float (&g)[CP<KK>::P])
+{
+    return cf_lossgrad_q<KK, R, G>(QLds{qs}, th, g);
 }
 
 template <int KK, int R>
@@ -693,6 +722,20 @@ __device__ __forceinline__ void cf_load_q(const uint16_t *rois, int roi, const C
     }
 }
 
+// raw counts of ROI roi into the lane's LDS column (converted on use, QLds16)
+template <int R>
+__device__ __forceinline__ void cf_load_q16(const uint16_t *rois, int roi, uint16_t *qs)
+{
+    constexpr int RR = R * R, WR = R * (R + 1);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(rois + (size_t)roi * WR);
+#pragma unroll
+    for (int w = 0; w < (RR + 1) / 2; w++) {
+        const uint32_t u = src[w];
+        qs[(2 * w) * 64] = (uint16_t)(u & 0xffffu);
+        if (2 * w + 1 < RR) qs[(2 * w + 1) * 64] = (uint16_t)(u >> 16);
+    }
+}
+
 // A parameter held at its box bound (cf_project) whose descent direction -gu points further out:
 // it is left out of the quasi-Newton step (active set), so the line search is not defeated by a
 // projection that cancels part of the step.
@@ -714,8 +757,8 @@ __device__ __forceinline__ bool cf_pinned(const float (&th)[CP<KK>::P], int i, f
 // non-positive slope (this includes a zero free gradient).  Otherwise du, slope = du . gu < 0 and
 // dmax = max |du| are set; a non-descent direction restarts from the gradient with D = I (first
 // = true), and with D = I the step is limited to 0.1 scaled units.
-template <int KK, int R>
-__device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const float (&g)[CP<KK>::P], float *D, bool &first,
+template <int KK, int R, class DA>
+__device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const float (&g)[CP<KK>::P], DA &D, bool &first,
                                              float tol, bool active_set, float (&du)[CP<KK>::P], float &slope, float &dmax)
 {
     constexpr int P = CP<KK>::P;
@@ -732,7 +775,7 @@ __device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const
         float a = 0.0f;
 #pragma unroll
         for (int jj = 0; jj < P; jj++)
-            if (!pin[jj]) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], gu[jj], a);
+            if (!pin[jj]) a = fmaf(D[i <= jj ? tri(i, jj, P) : tri(jj, i, P)], gu[jj], a);
         du[i] = pin[i] ? 0.0f : -a;
         slope = fmaf(du[i], gu[i], slope);
     }
@@ -751,7 +794,7 @@ __device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const
 #pragma unroll
         for (int i = 0; i < P; i++)
 #pragma unroll
-            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P)] = (i == jj) ? 1.0f : 0.0f;
         first = true;
     }
     if (first) {   // identity inverse Hessian: limit the first step to 0.1 scaled units
@@ -776,8 +819,8 @@ __device__ __forceinline__ bool cf_direction(const float (&th)[CP<KK>::P], const
 
 // BFGS update of the packed inverse Hessian after an accepted step from th to tn with gradients
 // g -> gn (physical units); with D = I (first) D is first rescaled to (s.y / y.y) I.
-template <int KK>
-__device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const float (&th)[CP<KK>::P], const float (&tn)[CP<KK>::P],
+template <int KK, class DA>
+__device__ __forceinline__ void cf_bfgs_update(DA &D, bool &first, const float (&th)[CP<KK>::P], const float (&tn)[CP<KK>::P],
                                                const float (&g)[CP<KK>::P], const float (&gn)[CP<KK>::P])
 {
     constexpr int P = CP<KK>::P;
@@ -796,7 +839,7 @@ __device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const floa
 #pragma unroll
         for (int i = 0; i < P; i++)
 #pragma unroll
-            for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? sc : 0.0f;
+            for (int jj = i; jj < P; jj++) D[tri(i, jj, P)] = (i == jj) ? sc : 0.0f;
         first = false;
     }
     // D <- D - rho (s u^T + u s^T) + (rho^2 y.u + rho) s s^T, u = D y
@@ -806,7 +849,7 @@ __device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const floa
     for (int i = 0; i < P; i++) {
         float a = 0.0f;
 #pragma unroll
-        for (int jj = 0; jj < P; jj++) a = fmaf(D[(i <= jj ? tri(i, jj, P) : tri(jj, i, P)) * 64], y[jj], a);
+        for (int jj = 0; jj < P; jj++) a = fmaf(D[i <= jj ? tri(i, jj, P) : tri(jj, i, P)], y[jj], a);
         u[i] = a;
         yu = fmaf(y[i], a, yu);
     }
@@ -815,8 +858,8 @@ __device__ __forceinline__ void cf_bfgs_update(float *D, bool &first, const floa
     for (int i = 0; i < P; i++)
 #pragma unroll
         for (int jj = i; jj < P; jj++) {
-            float *e = &D[tri(i, jj, P) * 64];
-            *e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), *e));
+            float &e = D[tri(i, jj, P)];
+            e = fmaf(cs * s[i], s[jj], fmaf(-rho, fmaf(s[i], u[jj], u[i] * s[jj]), e));
         }
 }
 

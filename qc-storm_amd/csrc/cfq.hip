@@ -21,6 +21,7 @@
 // of the one-pass kernel, so both give bit-identical parameters, iteration counts and flags (the
 // evaluation counts differ by the saved gradient evaluations).
 #include <string.h>
+#include <type_traits>
 #include "cfit_dev.h"
 
 namespace qcs {
@@ -75,13 +76,36 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
     }
 }
 
-template <int KK, int R>
-__device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitConst &c, float *qsh, float *Dsh)
+// Storage of the persistent kernel for a model key and largest ROI size, chosen so that more
+// single-wave work-groups fit in a CU: QU16 keeps the observed image as raw counts (2 B per pixel
+// in LDS, converted on use), the first NREG packed inverse-Hessian entries live in registers, and
+// WAVES is the register budget's target in waves per SIMD.  The arithmetic is the same in every
+// layout.  Only the 2D three-emitter 7x7 kernel gains from it (5 -> 8 work-groups per CU,
+// 4.78 -> 4.25 ms on 2^20 ROIs); the others were measured at higher occupancy and did not (the
+// register-held entries cost moves, the converted counts cost 5 instructions per pixel).
+template <int KK, int RMAX> struct CqLayout {
+    static constexpr bool QU16 = false;
+    static constexpr int NREG = 0, WAVES = 1;
+};
+template <> struct CqLayout<3, 7> {   // 24.5 + 55 floats of LDS per lane: 8 work-groups per CU
+    static constexpr bool QU16 = true;
+    static constexpr int NREG = 11, WAVES = 2;
+};
+template <class LY> using CqQ = typename std::conditional<LY::QU16, uint16_t, float>::type;
+
+template <int KK, int R, class LY>
+__device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitConst &c, CqQ<LY> *qsh, float *Dsh)
 {
     using Q = CQ<KK>;
     constexpr int P = Q::P;
     const int lane = threadIdx.x;
-    float *qs = qsh + lane, *D = Dsh + lane;
+    CqQ<LY> *qs = qsh + lane;
+    DSplit<LY::NREG> D;
+    D.p = Dsh + lane;
+    const auto qa = [&]() {
+        if constexpr (LY::QU16) return QLds16{qs, c.offset, c.kadc, c.inv_qe};
+        else return QLds{qs};
+    }();
     const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
     int st = CQ_IDLE, roi = 0, iters = 0, flags = 0, evals = 0, t = 0, pin_run = 0;
     bool first = false, dry = n <= 0;
@@ -116,7 +140,8 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         const int j = base + __popcll(m & ((1ull << lane) - 1ull));
         if (j >= n) return;
         roi = S.idx ? S.idx[j] : j;
-        cf_load_q<R>(S.rois, roi, c, qs);
+        if constexpr (LY::QU16) cf_load_q16<R>(S.rois, roi, qs);
+        else cf_load_q<R>(S.rois, roi, c, qs);
         const float4 *rec = reinterpret_cast<const float4 *>(S.rec + (size_t)j * Q::W);
 #pragma unroll
         for (int v = 0; v < Q::W / 4; v++) {
@@ -129,7 +154,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
                 else if (i == Q::OL) L = e[u];
                 else if (i < Q::OF) g[(i > Q::OL && i < Q::OF) ? i - Q::OG : 0] = e[u];
                 else if (i == Q::OF) first = e[u] != 0.0f;
-                else if (i < Q::OD + Q::NS) D[((i >= Q::OD && i < Q::OD + Q::NS) ? i - Q::OD : 0) * 64] = e[u];
+                else if (i < Q::OD + Q::NS) D[(i >= Q::OD && i < Q::OD + Q::NS) ? i - Q::OD : 0] = e[u];
             }
         }
         iters = 0;
@@ -162,7 +187,7 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
 #pragma unroll
         for (int i = 0; i < P; i++) tn[i] = fmaf(alpha, du[i] * cf_scale<KK>(i), th[i]);
         cf_project<KK, R>(tn);
-        const float Lt = cf_lossgrad<KK, R, true>(qs, tn, gt);
+        const float Lt = cf_lossgrad_q<KK, R, true>(qa, tn, gt);
         evals++;
         if (Lt == Lt && Lt < L && Lt <= fmaf(1e-4f * alpha, slope, L)) {
             // accepted: BFGS update of the inverse Hessian with s = step, y = gradient change
@@ -208,19 +233,21 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
 }
 
 template <int KK, int RMAX>
-__global__ void __launch_bounds__(64) cfq_kernel(CfqArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(CqLayout<KK, RMAX>::WAVES))) cfq_kernel(CfqArgs a)
 {
-    __shared__ float qsh[RMAX * RMAX * 64];
-    __shared__ float Dsh[CP<KK>::NS * 64];
+    using LY = CqLayout<KK, RMAX>;
+    constexpr int NDL = CP<KK>::NS - LY::NREG;
+    __shared__ CqQ<LY> qsh[RMAX * RMAX * 64];
+    __shared__ float Dsh[(NDL > 0 ? NDL : 1) * 64];
     for (int si = 0; si < a.nsets; si++) {
         const int R = a.set[si].R;
         if constexpr (RMAX >= 11) {
-            if (R == 11) cfq_run<KK, 11>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+            if (R == 11) cfq_run<KK, 11, LY>(a.set[si], a.heads + si, a.c, qsh, Dsh);
         }
         if constexpr (RMAX >= 9) {
-            if (R == 9) cfq_run<KK, 9>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+            if (R == 9) cfq_run<KK, 9, LY>(a.set[si], a.heads + si, a.c, qsh, Dsh);
         }
-        if (R == 7) cfq_run<KK, 7>(a.set[si], a.heads + si, a.c, qsh, Dsh);
+        if (R == 7) cfq_run<KK, 7, LY>(a.set[si], a.heads + si, a.c, qsh, Dsh);
     }
 }
 
