@@ -157,6 +157,13 @@ __device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_mov_dp
 __device__ __forceinline__ float lane_prev(float v) { return __int_as_float(lane_prev(__float_as_int(v))); }
 __device__ __forceinline__ float lane_next(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
 
+// two 16-bit lanes of a 32-bit register
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t w) { return __builtin_bit_cast(u16x2, w); }
+__device__ __forceinline__ u16x2 swap16(u16x2 v) { return __builtin_shufflevector(v, v, 1, 0); }
+__device__ __forceinline__ u16x2 pmin16(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+constexpr u16x2 kOnes16 = {1, 1};
+
 constexpr int SEGW = 56;             // output columns per wave
 constexpr int RAWW = 72;             // raw row width in LDS
 constexpr int VAR_MAX = 512;         // region side bound (2 * 256)
@@ -258,6 +265,7 @@ __device__ __forceinline__ void walk_body(const uint16_t *__restrict__ frames, c
     // interior step after another step keeps the previous step's seven newest rows of each and
     // reads four from LDS (a walk's first step may be interior: then all eleven are read)
     int colR[RS + 7], colL[RS + 7];
+    u16x2 CR[RS + 6], CL[RS + 6];
     bool have_cols = false;   // uniform
     // the lane's V-smoothed rows r-4 .. r+RS-3: [0, 2) from the previous step, [2, RS+2) this
     // step's V pass; detection takes lanes l-1 and l+1 of them by DPP wave shifts
@@ -289,12 +297,19 @@ __device__ __forceinline__ void walk_body(const uint16_t *__restrict__ frames, c
         }
         __syncthreads();
         // background sums of rows r .. r+3
+        // CR[m] / CL[m]: the 16-bit pairs (col[m], col[m+1]) of the two columns
+        auto pack = [&](int m) {
+            CR[m] = as_u16x2((uint32_t)colR[m] | ((uint32_t)colR[m + 1] << 16));
+            CL[m] = as_u16x2((uint32_t)colL[m] | ((uint32_t)colL[m + 1] << 16));
+        };
         if (EDGE || !have_cols) {
 #pragma unroll
             for (int i = 0; i < RS + 7; i++) {
                 colR[i] = raw[rs(i - 4)][l + 8];
                 colL[i] = raw[rs(i - 3)][l];
             }
+#pragma unroll
+            for (int m = 0; m < RS + 6; m++) pack(m);
             have_cols = true;
         } else {
 #pragma unroll
@@ -307,20 +322,34 @@ __device__ __forceinline__ void walk_body(const uint16_t *__restrict__ frames, c
                 colR[i] = raw[rs(i - 4)][l + 8];
                 colL[i] = raw[rs(i - 3)][l];
             }
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                CR[m] = CR[m + RS];
+                CL[m] = CL[m + RS];
+            }
+#pragma unroll
+            for (int m = 6; m < RS + 6; m++) pack(m);
         }
+        // Two terms per instruction: term i takes rA[l+i], colR[k+i], rB[l+8-i] and colL[k+7-i],
+        // so terms 2j and 2j+1 read the 16-bit pairs (rA[l+2j], rA[l+2j+1]) and
+        // (colR[k+2j], colR[k+2j+1]) in order and (rB[l+7-2j], rB[l+8-2j]) and
+        // (colL[k+6-2j], colL[k+7-2j]) with their halves swapped.  Packed 16-bit minima, then a
+        // dot product with (1, 1) adds both halves into the 32-bit sum.
         int bkv[RS];
 #pragma unroll
         for (int k = 0; k < RS; k++) {
-            const uint16_t *rA = raw[rs(k - 4)], *rB = raw[rs(k + 4)];
-            int bk = 0;
+            uint4 A, B;   // rA[l .. l+7], rB[l+1 .. l+8]
+            __builtin_memcpy(&A, &raw[rs(k - 4)][l], sizeof A);
+            __builtin_memcpy(&B, &raw[rs(k + 4)][l + 1], sizeof B);
+            const uint32_t aw[4] = {A.x, A.y, A.z, A.w}, bw[4] = {B.w, B.z, B.y, B.x};
+            uint32_t bk = 0;
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
-                const int l1 = rA[l + i], l2 = colR[k + i];
-                const int l3 = rB[l + 8 - i], l4 = colL[k + 7 - i];
-                const int m1 = l1 < l2 ? l1 : l2, m2 = l3 < l4 ? l3 : l4;
-                bk += m1 < m2 ? m1 : m2;
+            for (int j = 0; j < 4; j++) {
+                const u16x2 m1 = pmin16(as_u16x2(aw[j]), CR[k + 2 * j]);
+                const u16x2 m2 = pmin16(swap16(as_u16x2(bw[j])), swap16(CL[k + 6 - 2 * j]));
+                bk = __builtin_amdgcn_udot2(pmin16(m1, m2), kOnes16, bk, false);
             }
-            bkv[k] = bk;
+            bkv[k] = (int)bk;
         }
         // the neighbours' sums by DPP wave shifts: li is l - 1 or l, ri is l or l + 1
         int tvk[RS];
@@ -450,8 +479,11 @@ __device__ __forceinline__ void walk_body(const uint16_t *__restrict__ frames, c
     }
 }
 
+// The full pass is held to five waves per SIMD (at most 102 VGPRs, no spills): its steps are chains
+// of dependent LDS round trips, and the fifth wave hides a little more of them (1% faster than
+// four; six spill).
 template <int MODE, bool SMOOTH>
-__global__ void __launch_bounds__(64) walk_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE == 1 ? 5 : 1))) walk_kernel(const uint16_t *__restrict__ frames, ExtractConst c,
                                                   const unsigned long long *__restrict__ var_sums,
                                                   uint16_t *__restrict__ tvbuf,
                                                   unsigned long long *__restrict__ cand_mask,
