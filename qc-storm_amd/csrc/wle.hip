@@ -234,29 +234,47 @@ __global__ void __launch_bounds__(256) wle_nn_sorted_kernel(const uint16_t *__re
 // Same sweep over the extraction's (x, y-in-batch, batch) list, which is in raster order: the
 // frame of an entry is batch * B + y / H, and two entries share a frame when they share the
 // batch and the frame's row range.
-__global__ void __launch_bounds__(256) wle_nn_xyb_kernel(const int3 *__restrict__ xyb, const int *__restrict__ n_dev,
-                                                         int B, int H, float *__restrict__ wle)
+// A work-group first copies its entries and NN_HALO entries on either side into LDS; a sweep
+// that runs past that window reads the rest from HBM.
+constexpr int NN_BLOCK = 256, NN_HALO = 512;
+__global__ void __launch_bounds__(NN_BLOCK) wle_nn_xyb_kernel(const int3 *__restrict__ xyb, const int *__restrict__ n_dev,
+                                                              int B, int H, float *__restrict__ wle)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int3 win[NN_BLOCK + 2 * NN_HALO];
     const int n = *n_dev;
+    const int i0 = blockIdx.x * NN_BLOCK;
+    if (i0 >= n) return;   // uniform over the work-group
+    const int w0 = max(i0 - NN_HALO, 0), w1 = min(i0 + NN_BLOCK + NN_HALO, n);
+    for (int j = w0 + (int)threadIdx.x; j < w1; j += NN_BLOCK) win[j - w0] = xyb[j];
+    __syncthreads();
+    const int i = i0 + threadIdx.x;
     if (i >= n) return;
-    const int3 a = xyb[i];
+    const int3 a = win[i - w0];
     const int fy0 = a.y - a.y % H;   // first batch row of a's frame
     float best = 100000000.0f;
-    for (int j = i + 1; j < n; j++) {
-        const int3 q = xyb[j];
+    // forward: false once a neighbour ends the sweep
+    auto fwd = [&](const int3 q) {
         const int dy = q.y - a.y;
-        if (q.z != a.z || q.y - fy0 >= H || (float)(dy * dy) > best) break;
+        if (q.z != a.z || q.y - fy0 >= H || (float)(dy * dy) > best) return false;
         const int dx = q.x - a.x;
         best = fminf(best, (float)(dx * dx + dy * dy));
-    }
-    for (int j = i - 1; j >= 0; j--) {
-        const int3 q = xyb[j];
+        return true;
+    };
+    auto bwd = [&](const int3 q) {
         const int dy = a.y - q.y;
-        if (q.z != a.z || q.y < fy0 || (float)(dy * dy) > best) break;
+        if (q.z != a.z || q.y < fy0 || (float)(dy * dy) > best) return false;
         const int dx = q.x - a.x;
         best = fminf(best, (float)(dx * dx + dy * dy));
-    }
+        return true;
+    };
+    int j = i + 1;
+    bool go = true;
+    for (; go && j < w1; j++) go = fwd(win[j - w0]);
+    for (; go && j < n; j++) go = fwd(xyb[j]);
+    j = i - 1;
+    go = true;
+    for (; go && j >= w0; j--) go = bwd(win[j - w0]);
+    for (; go && j >= 0; j--) go = bwd(xyb[j]);
     wle[(size_t)i * QCS_WLE_FLOATS + QCS_WLE_NEARDIST] = sqrtf(best);
     (void)B;
 }
@@ -316,7 +334,7 @@ static int wle_launch_impl(const qcs_params &p, const uint16_t *d_rois, int n, c
     }
     QCS_HIP(hipGetLastError());
     if (sorted == 2)
-        hipLaunchKernelGGL(wle_nn_xyb_kernel, g256, dim3(256), 0, s, d_xyb, n_dev, B, H, d_wle);
+        hipLaunchKernelGGL(wle_nn_xyb_kernel, dim3((n + NN_BLOCK - 1) / NN_BLOCK), dim3(NN_BLOCK), 0, s, d_xyb, n_dev, B, H, d_wle);
     else if (sorted)
         hipLaunchKernelGGL(wle_nn_sorted_kernel, g256, dim3(256), 0, s, d_rois, n, n_dev, R, d_wle);
     else
