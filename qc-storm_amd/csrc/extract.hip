@@ -624,25 +624,39 @@ __global__ void batch_kernel(ExtractConst c, const int *__restrict__ row_off, co
 }
 
 // ---- 5. candidate list in raster order -------------------------------------------------------
-__global__ void __launch_bounds__(256) emit_kernel(ExtractConst c, const unsigned long long *__restrict__ cand_mask,
-                                                   const int *__restrict__ row_off, const int *__restrict__ row_count,
-                                                   int nrows, const int *__restrict__ info, int3 *__restrict__ xyb)
+// One wave per stacked row: lane s takes segment s's candidate mask (coalesced loads), and a
+// wave prefix sum of the masks' popcounts gives each segment its first slot, so the row's
+// candidates come out in raster order.
+constexpr int EMIT_WAVES = 4;   // rows per work-group
+__global__ void __launch_bounds__(64 * EMIT_WAVES) emit_kernel(ExtractConst c, const unsigned long long *__restrict__ cand_mask,
+                                                               const int *__restrict__ row_off, const int *__restrict__ row_count,
+                                                               int nrows, const int *__restrict__ info, int3 *__restrict__ xyb)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= nrows || row_count[row] == 0) return;
+    const int row = blockIdx.x * EMIT_WAVES + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= nrows || row_count[row] == 0) return;   // uniform over the wave
     const int b = row / (c.B * c.H);
     const int r0 = b * c.B * c.H;
     const int kept = info[2 + 3 * b + 1], out0 = info[2 + 3 * b + 2];
-    int local = row_off[row] - row_off[r0];
+    int base = row_off[row] - row_off[r0];
     const int y = row - r0;   // batch-relative row
-    for (int sgi = 0; sgi < c.nseg; sgi++) {
-        unsigned long long m = cand_mask[(size_t)row * c.nseg + sgi];
+    for (int s0 = 0; s0 < c.nseg; s0 += 64) {
+        const int sgi = s0 + lane;
+        unsigned long long m = (sgi < c.nseg) ? cand_mask[(size_t)row * c.nseg + sgi] : 0ull;
+        const int cnt = __popcll(m);
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        int local = base + incl - cnt;
         while (m) {
             const int bit = __ffsll((long long)m) - 1;
             m &= m - 1;
             if (local < kept) xyb[out0 + local] = make_int3(sgi * SEGW + bit, y, b);
             local++;
         }
+        base += __shfl(incl, 63, 64);
     }
 }
 
@@ -865,8 +879,8 @@ static int extract_candidates(qcs_ctx *ctx, const qcs_params &p, const uint16_t 
     hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(SCAN_BLOCK), 0, s, tile_sums, ntiles);
     hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, row_off, (int)nrows, tile_sums);
     hipLaunchKernelGGL(batch_kernel, dim3(1), dim3(64), 0, s, c, row_off, tile_sums, (int)nrows, ntiles, info);
-    hipLaunchKernelGGL(emit_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, s, c, cand, row_off, row_count,
-                       (int)nrows, info, xyb);
+    hipLaunchKernelGGL(emit_kernel, dim3((unsigned)((nrows + EMIT_WAVES - 1) / EMIT_WAVES)), dim3(64 * EMIT_WAVES), 0, s, c, cand,
+                       row_off, row_count, (int)nrows, info, xyb);
     QCS_HIP(hipGetLastError());
     run.info = info;
     run.xyb = xyb;
