@@ -780,6 +780,8 @@ def main():
                     help="time only the headline step (no other legs, no CPU baseline): for profiling its kernels")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
+    ap.add_argument("--leg-only", choices=("multi_emitter", "as3d_multi_emitter"),
+                    help="run only this multi-emitter leg and print its block: for profiling its kernels")
     args = ap.parse_args()
     if args.headline_only:
         args.no_fast_leg = args.no_cpu_baseline = True
@@ -805,6 +807,13 @@ def main():
                           "launches": max(args.steps, 1)}), flush=True)
         return
     Q.lib()  # raises when libqcstorm.so is missing: there is no fallback
+    if args.leg_only:
+        if args.leg_only == "multi_emitter":
+            out = multi_emitter_leg(args.multi_rois, 1003, log)
+        else:
+            out = as3d_multi_emitter_leg(args.as3d_rois, 1004, log)
+        print(json.dumps({args.leg_only: out}), flush=True)
+        return
     stream = torch.cuda.Stream(device=dev)
     sh = C.c_void_p(stream.cuda_stream)
 
