@@ -109,14 +109,16 @@ __device__ __forceinline__ float cf_lossgrad_q(const QA &qa, const float (&th)[C
 #pragma unroll
         for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = 0.0f;
         // rows stay a loop: a full unroll lets the scheduler hoist every row's LDS loads and
-        // exponentials, which costs registers (occupancy) for no gain
+        // exponentials, which costs registers (occupancy) for no gain.  The gradient terms that
+        // depend on the row only (dy, dy^2) multiply the row's sums of w g_k once per row.
 #pragma unroll 1
         for (int r = 0; r < R; r++) {
-            float ay[K], dy[K];
+            float ay[K], dy[K], sw[K], sx2[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 dy[k] = ((float)r + 0.5f) - th[3 * k + 2];
                 ay[k] = th[3 * k] * nrm * __expf(h * dy[k] * dy[k]);
+                sw[k] = sx2[k] = 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < R; c++) {
@@ -137,12 +139,19 @@ __device__ __forceinline__ float cf_lossgrad_q(const QA &qa, const float (&th)[C
 #pragma unroll
                     for (int k = 0; k < K; k++) {
                         const float wg = w * gk[k];
-                        gN[k] += wg;
+                        sw[k] += wg;
                         gx[k] = fmaf(wg, dx[k][c], gx[k]);
-                        gy[k] = fmaf(wg, dy[k], gy[k]);
-                        gd2 = fmaf(wg, fmaf(dx[k][c], dx[k][c], dy[k] * dy[k]), gd2);
-                        gw += wg;
+                        sx2[k] = fmaf(wg, dx[k][c] * dx[k][c], sx2[k]);
                     }
+                }
+            }
+            if (G) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    gN[k] += sw[k];
+                    gw += sw[k];
+                    gy[k] = fmaf(sw[k], dy[k], gy[k]);
+                    gd2 = fmaf(sw[k], dy[k] * dy[k], gd2 + sx2[k]);
                 }
             }
         }
@@ -180,11 +189,12 @@ __device__ __forceinline__ float cf_lossgrad_q(const QA &qa, const float (&th)[C
         for (int k = 0; k < K; k++) gN[k] = gx[k] = gy[k] = gxx[k] = gyy[k] = 0.0f;
 #pragma unroll 1
         for (int r = 0; r < R; r++) {
-            float ay[K], dy[K];
+            float ay[K], dy[K], sw[K];
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 dy[k] = ((float)r + 0.5f) - th[M::iy(k)];
                 ay[k] = nk[k] * __expf(hy[k] * dy[k] * dy[k]);
+                sw[k] = 0.0f;
             }
 #pragma unroll
             for (int c = 0; c < R; c++) {
@@ -204,12 +214,18 @@ __device__ __forceinline__ float cf_lossgrad_q(const QA &qa, const float (&th)[C
 #pragma unroll
                     for (int k = 0; k < K; k++) {
                         const float wg = w * gk[k];
-                        gN[k] += wg;
+                        sw[k] += wg;
                         gx[k] = fmaf(wg, dx[k][c], gx[k]);
-                        gy[k] = fmaf(wg, dy[k], gy[k]);
                         gxx[k] = fmaf(wg, dx[k][c] * dx[k][c], gxx[k]);
-                        gyy[k] = fmaf(wg, dy[k] * dy[k], gyy[k]);
                     }
+                }
+            }
+            if (G) {   // the row's terms in dy, dy^2 from the row sums of w g_k
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    gN[k] += sw[k];
+                    gy[k] = fmaf(sw[k], dy[k], gy[k]);
+                    gyy[k] = fmaf(sw[k], dy[k] * dy[k], gyy[k]);
                 }
             }
         }
