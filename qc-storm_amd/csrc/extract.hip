@@ -689,6 +689,9 @@ __global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ fr
                                                  const int *__restrict__ sel, const int *__restrict__ sel_n,
                                                  uint16_t *__restrict__ rois)
 {
+    // the CUT_U records are assembled in LDS and leave as one contiguous block (16-byte stores
+    // when the block is 16-byte aligned)
+    __shared__ __align__(16) uint16_t stage[CUT_U * 17 * 18];
     const int n = sel ? *sel_n : info[0];
     const int half = R / 2, WR = R * (R + 1), RR = R * R;
     const int l = threadIdx.x;
@@ -737,9 +740,22 @@ __global__ void __launch_bounds__(64) cut_kernel(const uint16_t *__restrict__ fr
                 }
             }
 #pragma unroll
-            for (int j = 0; j < CUT_U; j++)
-                if (i0 + j < n) rois[(size_t)(i0 + j) * WR + k] = v[j];
+            for (int j = 0; j < CUT_U; j++) stage[j * WR + k] = v[j];
         }
+        __syncthreads();
+        const int nv = min(CUT_U, n - i0);
+        uint16_t *dst = rois + (size_t)i0 * WR;
+        const uintptr_t al = reinterpret_cast<uintptr_t>(dst);
+        if (nv == CUT_U && (al & 15) == 0) {   // uniform; CUT_U records are 16 R (R + 1) bytes
+            for (int t = l; t < CUT_U * WR / 8; t += 64)
+                reinterpret_cast<uint4 *>(dst)[t] = reinterpret_cast<const uint4 *>(stage)[t];
+        } else if ((al & 3) == 0) {   // records are a whole number of 4-byte words (R (R + 1) even)
+            for (int t = l; t < nv * WR / 2; t += 64)
+                reinterpret_cast<uint32_t *>(dst)[t] = reinterpret_cast<const uint32_t *>(stage)[t];
+        } else {
+            for (int t = l; t < nv * WR; t += 64) dst[t] = stage[t];
+        }
+        __syncthreads();   // the next group reuses stage
     }
 }
 
