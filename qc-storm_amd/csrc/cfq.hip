@@ -80,8 +80,9 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
 // single-wave work-groups fit in a CU: QU16 keeps the observed image as raw counts (2 B per pixel
 // in LDS, converted on use), the first NREG packed inverse-Hessian entries live in registers, and
 // WAVES is the register budget's target in waves per SIMD.  The arithmetic is the same in every
-// layout.  Only the 2D three-emitter 7x7 kernel gains from it (5 -> 8 work-groups per CU,
-// 4.78 -> 4.25 ms on 2^20 ROIs); the others were measured at higher occupancy and did not (the
+// layout.  The 2D three-emitter 7x7 kernel gains from it (5 -> 8 work-groups per CU, 4.78 ->
+// 4.25 ms on 2^20 ROIs), and so do the 2D mixed-size launches (high-density frames 9% faster);
+// the elliptical and the other 7x7 kernels were measured at higher occupancy and did not (the
 // register-held entries cost moves, the converted counts cost 5 instructions per pixel).
 template <int KK, int RMAX> struct CqLayout {
     static constexpr bool QU16 = false;
@@ -90,6 +91,19 @@ template <int KK, int RMAX> struct CqLayout {
 template <> struct CqLayout<3, 7> {   // 24.5 + 55 floats of LDS per lane: 8 work-groups per CU
     static constexpr bool QU16 = true;
     static constexpr int NREG = 11, WAVES = 2;
+};
+// mixed-size launches (sets of 7, 9 and 11 pixels, LDS sized for 11): raw counts
+template <> struct CqLayout<1, 11> {   // 60.5 + 15 floats: 8 per CU
+    static constexpr bool QU16 = true;
+    static constexpr int NREG = 0, WAVES = 2;
+};
+template <> struct CqLayout<2, 11> {   // 60.5 + 36 floats: 6 per CU
+    static constexpr bool QU16 = true;
+    static constexpr int NREG = 0, WAVES = 2;
+};
+template <> struct CqLayout<3, 11> {   // 60.5 + 66 floats: 5 per CU
+    static constexpr bool QU16 = true;
+    static constexpr int NREG = 0, WAVES = 2;
 };
 template <class LY> using CqQ = typename std::conditional<LY::QU16, uint16_t, float>::type;
 
