@@ -50,11 +50,11 @@ float orc_tlog_u(float x, float c);
 float orc_tlogf(float x);
 void  orc_tlogf_v(const float *x, float *y, long n);
 void  orc_tlog_table(uint32_t out[64]);
-/* binade table log of the one-emitter losses: orc_tlogb_u(x, c) = log x + c; the 1024 pairs */
+/* binade table log of the one-emitter losses: orc_tlogb_u(x, c) = log x + c; the 4096 pairs */
 float orc_tlogb_u(float x, float c);
 float orc_tlogbf(float x);
 void  orc_tlogbf_v(const float *x, float *y, long n);
-void  orc_tlogb_table(uint32_t out[2048]);
+void  orc_tlogb_table(uint32_t out[8192]);
 /* the CRLB's narrow-range division by dh = 0.02 and a count of its differences from x / 0.02f */
 float orc_div_dh_fast(float x);
 long long orc_div_dh_scan(uint32_t b0, uint32_t b1);

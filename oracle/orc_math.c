@@ -23,12 +23,12 @@
  *           truncation < 2e-10), log x = fma(e, ln2_f, t_j + log1p(r)).  The loss adds its
  *           data term c before the last fma: orc_tlog_u(x, c) = log x + c.  Max error of
  *           orc_tlogf 1.25 ulp over all positive normal floats.  The multi-emitter losses use it.
- *   binade table log (the one-emitter losses, orc_tlogb_u): g = (int)(b - 0x3f360000) >> 18 of
- *           the bit pattern b numbers the binade e and one of 32 intervals of x = m 2^e,
- *           m in [0.7109375, 1.421875); for e in -10..21 the pair (i, t) at g + 320 carries
- *           e ln2 inside t = -log(i), so log x + c = (t + c) + log1p(fma(x, i, -1)); outside
- *           that range x is scaled into binade 0 and e ln2 added in one fma.  Max error of
- *           orc_tlogbf 1.2 ulp.
+ *   binade table log (the one-emitter losses, orc_tlogb_u): g = (int)(b - 0x3f368000) >> 16 of
+ *           the bit pattern b numbers the binade e and one of 128 intervals of x = m 2^e,
+ *           m in [0.712890625, 1.42578125); for e in -10..21 the pair (i, t) at g + 1280 carries
+ *           e ln2 inside t = -log(i), so log x + c = (t + c) + log1p(fma(x, i, -1)) with
+ *           log1p(r) = r + r^2 (-1/2 + r / 3) (|r| < 2^-8 + 4e-4); outside that range x is
+ *           scaled into binade 0 and e ln2 added in one fma.  Max error of orc_tlogbf 1.05 ulp.
  *
  * Every step is an IEEE-754 single operation or an integer operation on the bit pattern, so any IEEE
  * implementation evaluates them bit-identically.  The tests pin both routines against
@@ -144,27 +144,27 @@ float orc_tlogf(float x) { return orc_tlog_u(x, 0.0f); }
 /* binade table log of the one-emitter losses: the table is the library's data file
  * (qc-storm_amd/csrc/qcs_logtab.inc, written by logtab_gen.py; its entries are pinned against
  * their definition by the tests) */
-static const uint32_t TLOGB_TAB[2048] = {
+static const uint32_t TLOGB_TAB[8192] = {
 #include "../qc-storm_amd/csrc/qcs_logtab.inc"
 };
-#define TLOGB_K   0x3f360000u
-#define TLOGB_LO  0x1.6cp-11f   /* 0.7109375 * 2^-10 */
-#define TLOGB_HI  0x1.6cp21f    /* 0.7109375 * 2^22 */
+#define TLOGB_K   0x3f368000u
+#define TLOGB_LO  0x1.6dp-11f   /* 0.712890625 * 2^-10 */
+#define TLOGB_HI  0x1.6dp21f    /* 0.712890625 * 2^22 */
 
-void orc_tlogb_table(uint32_t out[2048]) { memcpy(out, TLOGB_TAB, sizeof TLOGB_TAB); }
+void orc_tlogb_table(uint32_t out[8192]) { memcpy(out, TLOGB_TAB, sizeof TLOGB_TAB); }
 
+/* 128 intervals per binade: |r| < 2^-8 + 4e-4 and log1p(r) = r + r^2 (-1/2 + r / 3) */
 static float tlogb_core(float x, float c)
 {
     uint32_t bx;
     memcpy(&bx, &x, sizeof bx);
-    const int g = (((int32_t)(bx - TLOGB_K)) >> 18) + 320;   /* binade and interval of x */
+    const int g = (((int32_t)(bx - TLOGB_K)) >> 16) + 1280;   /* binade and interval of x */
     float inv, t;
     memcpy(&inv, &TLOGB_TAB[2 * g], sizeof inv);
     memcpy(&t, &TLOGB_TAB[2 * g + 1], sizeof t);
     const float r = fmaf(x, inv, -1.0f);
     const float r2 = r * r;
-    float p = fmaf(r, -0.25f, 0.3333333432674407958984375f);
-    p = fmaf(p, r, -0.5f);
+    const float p = fmaf(r, 0.3333333432674407958984375f, -0.5f);
     const float lr = fmaf(p, r2, r);
     return (t + c) + lr;
 }

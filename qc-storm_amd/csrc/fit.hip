@@ -422,15 +422,21 @@ __device__ __forceinline__ float zpoly(const float (&pc)[5], float d, float corr
 // Up to 9x9 the register budget is held to two waves per SIMD (256 VGPRs).  The 7x7 exact-math
 // kernel fits without spills; at 9x9 the kernel spills 30 (2D) or 65 (elliptical) registers outside
 // its loss loop and still runs 1.5-1.6x faster than at its natural one wave.
+// Waves per work-group: they share the 32 KB log table in LDS.  With each wave's BFGS state
+// (9 KB for the 2D model, 12 KB elliptical) a CU holds two groups of four 2D waves, or one group
+// of eight elliptical ones, i.e. two waves per SIMD up to 9x9.
+template <int MODEL, int R> constexpr int fit_wg_waves() { return (MODEL == MODEL_AS3D && R <= 9) ? 8 : 4; }
+
 template <int MODEL, int R, bool FAST>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(R <= 9 ? 2 : 1, R <= 9 ? 2 : 10)))
+__global__ void __launch_bounds__((64 * fit_wg_waves<MODEL, R>()))
+__attribute__((amdgpu_waves_per_eu(R <= 9 ? 2 : 1, R <= 9 ? 2 : 10)))
 fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
                                                     const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
                                                     float *__restrict__ loc, int *__restrict__ flag, FitConst fc)
 {
     constexpr int P = Traits<MODEL>::P;
     constexpr int HALF = R / 2;
-    __shared__ float2 ltab[1024];
+    __shared__ float2 ltab[kLogTabBN];
     log_tabb_load(ltab);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = *n_dev;
@@ -483,7 +489,7 @@ fit_1e_kernel(const uint16_t *__restrict__ rois, const float *__restrict__ wle,
         wle_weights<R>(wsx, wsy, mlefit, wx, wy);
     }
 
-    __shared__ float cold[2][(P * P + 2 * P) * 64];   // per wave of the two
+    __shared__ float cold[fit_wg_waves<MODEL, R>()][(P * P + 2 * P) * 64];   // per wave
     bfgs_fit<MODEL, R, FAST>(q, c1, wx, wy, x, &cold[threadIdx.x >> 6][threadIdx.x & 63], ltab);
 
     float row[QCS_REC_FLOATS];
@@ -634,18 +640,27 @@ __global__ void __launch_bounds__(64) crlb_kernel(float *__restrict__ loc, int n
 }
 
 // ---------------------------------------------------------------------------------------------
+template <int MODEL, int R, bool FAST>
+static void launch_1e(hipStream_t s, const uint16_t *rois, const float *wle, const int *idx, int n, const int *n_dev,
+                      float *loc, int *flag, const FitConst &fc)
+{
+    constexpr int T = 64 * fit_wg_waves<MODEL, R>();
+    hipLaunchKernelGGL((fit_1e_kernel<MODEL, R, FAST>), dim3((unsigned)((n + T - 1) / T)), dim3(T), 0, s, rois, wle, idx, n,
+                       n_dev, loc, flag, fc);
+}
+
 template <int MODEL, bool FAST>
-static int launch_1e_r(int R, dim3 grid, hipStream_t s, const uint16_t *rois, const float *wle, const int *idx, int n,
+static int launch_1e_r(int R, hipStream_t s, const uint16_t *rois, const float *wle, const int *idx, int n,
                        const int *n_dev, float *loc, int *flag, const FitConst &fc)
 {
     switch (R) {
-    case 5: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 5, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 7: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 7, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 9: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 9, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 11: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 11, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 13: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 13, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 15: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 15, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
-    case 17: hipLaunchKernelGGL((fit_1e_kernel<MODEL, 17, FAST>), grid, dim3(128), 0, s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 5: launch_1e<MODEL, 5, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 7: launch_1e<MODEL, 7, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 9: launch_1e<MODEL, 9, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 11: launch_1e<MODEL, 11, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 13: launch_1e<MODEL, 13, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 15: launch_1e<MODEL, 15, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
+    case 17: launch_1e<MODEL, 17, FAST>(s, rois, wle, idx, n, n_dev, loc, flag, fc); break;
     default: return fail(QCS_ERR_ARG, "ROISize %d not supported by the fitter (odd, 5..17)", R);
     }
     QCS_HIP(hipGetLastError());
@@ -657,14 +672,13 @@ int fit_launch_1e(const qcs_params &p, const uint16_t *d_rois, const float *d_wl
 {
     if (n <= 0) return QCS_OK;
     const FitConst fc = make_fit_const(p);
-    const dim3 grid((n + 127) / 128);   // two waves per work-group share the log table
     const bool as3d = (p.LocType == QCS_LOCTYPE_AS3D);
     const bool fast = (p.FitMath == QCS_MATH_FAST);
     if (as3d)
-        return fast ? launch_1e_r<MODEL_AS3D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
-                    : launch_1e_r<MODEL_AS3D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
-    return fast ? launch_1e_r<MODEL_GS2D, true>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
-                : launch_1e_r<MODEL_GS2D, false>(p.ROISize, grid, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
+        return fast ? launch_1e_r<MODEL_AS3D, true>(p.ROISize, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
+                    : launch_1e_r<MODEL_AS3D, false>(p.ROISize, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
+    return fast ? launch_1e_r<MODEL_GS2D, true>(p.ROISize, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc)
+                : launch_1e_r<MODEL_GS2D, false>(p.ROISize, s, d_rois, d_wle, d_idx, n, d_n, d_loc, d_flag, fc);
 }
 
 template <int MODEL, bool FAST>

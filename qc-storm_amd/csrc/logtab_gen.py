@@ -1,16 +1,17 @@
 """Write qcs_logtab.inc: the logarithm table of the one-emitter fitting loss.
 
-For a positive float x with bit pattern b, g = (int)(b - K) >> 18 with K = 0x3f360000 numbers
-the interval of x: binade e = g >> 5 of the reduction x = m 2^e, m in [0.7109375, 1.421875),
-and one of 32 equal bit-pattern steps of m.  K's low 18 bits (0x20000) put 1.0 in the middle of
-interval 18 of binade 0, [0.9921875, 1.015625), whose pair is (1, 0): log x = log1p(x - 1) there,
-with x - 1 exact, so there is no cancellation next to 1.  For e in [-10, 21] the table holds, at
-index g + 320, the pair (i, t): i a float close to the reciprocal of the interval's centre, and
-t = -log(i) rounded to float.  Among the 8001 floats nearest that reciprocal, i is the one whose
--log(i) (evaluated in double) lies closest to a float, relative to that float's ulp (accurate
-tables; every entry ends within 4e-4 ulp of -log(i)).
+For a positive float x with bit pattern b, g = (int)(b - K) >> 16 with K = 0x3f368000 numbers
+the interval of x: binade e = g >> 7 of the reduction x = m 2^e, m in [0.712890625, 1.42578125),
+and one of 128 equal bit-pattern steps of m.  K's low 16 bits (0x8000) put 1.0 in the middle of
+interval 73 of binade 0, [0.998046875, 1.00390625), whose pair is (1, 0): log x = log1p(x - 1)
+there, with x - 1 exact, so there is no cancellation next to 1.  For e in [-10, 21] the table
+holds, at index g + 1280, the pair (i, t): i a float close to the reciprocal of the interval's
+centre, and t = -log(i) rounded to float.  Among the 8001 floats nearest that reciprocal, i is
+the one whose -log(i) (evaluated in double) lies closest to a float, relative to that float's
+ulp (accurate tables; every entry ends within 1e-3 ulp of -log(i)).
 
-Then log x = t + log1p(r), r = x i - 1, |r| < 2^-6 + 4e-4.
+Then log x = t + log1p(r), r = x i - 1, |r| < 2^-8 + 4e-4.  K's low 16 bits being 0x8000 and the
+step 2^16 make the byte offset of a pair one add and one shift of the upper half-word.
 
 Run: python3 logtab_gen.py > qcs_logtab.inc  (the committed file is this script's output).
 """
@@ -18,9 +19,10 @@ import sys
 
 import numpy as np
 
-K = 0x3F360000
+K = 0x3F368000
 EMIN, EMAX = -10, 21
-SH = 18
+SH = 16
+PER = 128          # intervals per binade (2^(23 - SH))
 
 
 def _f(bits):
@@ -34,12 +36,12 @@ def _bits(v):
 def table():
     out = []
     for e in range(EMIN, EMAX + 1):
-        for j in range(32):
-            g = e * 32 + j
+        for j in range(PER):
+            g = e * PER + j
             lo_b = (K + (g << SH)) & 0xFFFFFFFF
             lo, hi = _f(lo_b), _f(lo_b + (1 << SH) - 1)
-            if e == 0 and j == 18:
-                assert lo < 1.0 < hi
+            if lo < 1.0 < hi:
+                assert e == 0 and j == 73
                 out.append((_bits(1.0), 0))
                 continue
             b0 = _bits(np.float32(2.0 / (lo + hi)))
@@ -57,7 +59,7 @@ def main():
     rows = table()
     print("/* qcs_logtab.inc -- written by logtab_gen.py: %d (1/c, -log(1/c)) pairs as float bit patterns,"
           % len(rows))
-    print(" * binades %d..%d of x = m 2^e, m in [0.7109375, 1.421875), 32 intervals each */" % (EMIN, EMAX))
+    print(" * binades %d..%d of x = m 2^e, m in [0.712890625, 1.42578125), %d intervals each */" % (EMIN, EMAX, PER))
     words = [w for pair in rows for w in pair]
     for i in range(0, len(words), 8):
         print("    " + ", ".join("0x%08xu" % w for w in words[i:i + 8]) + ",")

@@ -162,39 +162,40 @@ __device__ __forceinline__ float logtab_u_any(float x, float c, const float2 *lt
 
 // ---------------------------------------------------------------------------------------------
 // Binade table log of the one-emitter loss (qcs_logtab.inc, written by logtab_gen.py).  For a
-// positive float x with bits b, g = (int)(b - 0x3f360000) >> 18 numbers its interval: binade
-// e = g >> 5 of x = m 2^e, m in [0.7109375, 1.421875), and one of 32 bit-pattern steps of m.
-// Binades -10..21, x in [kLogTabBLo, kLogTabBHi), have their own pair (i, t) at g + 320, with
-// t = -log(i) to within 4e-4 ulp (accurate tables) and i near the reciprocal of the interval's
+// positive float x with bits b, g = (int)(b - 0x3f368000) >> 16 numbers its interval: binade
+// e = g >> 7 of x = m 2^e, m in [0.712890625, 1.42578125), and one of 128 bit-pattern steps of m.
+// Binades -10..21, x in [kLogTabBLo, kLogTabBHi), have their own pair (i, t) at g + 1280, with
+// t = -log(i) to within 1e-3 ulp (accurate tables) and i near the reciprocal of the interval's
 // centre; the interval around 1 has (1, 0).  There log x + c = (t + c) + log1p(r),
-// r = fma(x, i, -1), |r| < 2^-6 + 4e-4, with the same degree-4 log1p as above: the binade's
-// e ln2 is part of t, so the loss needs no exponent extraction, conversion or e ln2 FMA.
+// r = fma(x, i, -1), |r| < 2^-8 + 4e-4, log1p(r) = r + r^2 (-1/2 + r / 3): the binade's e ln2
+// is part of t, so the loss needs no exponent extraction, conversion or e ln2 FMA, and the byte
+// offset of a pair is one add and one shift of the upper half-word (the table is 32 KB).
 // Outside that range x is scaled by 2^-e into binade 0 and e ln2 is added in one FMA.
-// Max error of log x alone: 1.2 ulp in range, 0.6 ulp outside.
-static __constant__ uint32_t kLogTabB[2048] = {
+// Max error of log x alone: 1.05 ulp in range.
+static __constant__ uint32_t kLogTabB[8192] = {
 #include "qcs_logtab.inc"
 };
-constexpr uint32_t kLogTabBK = 0x3f360000u;
-constexpr float kLogTabBLo = 0x1.6cp-11f;   // 0.7109375 * 2^-10
-constexpr float kLogTabBHi = 0x1.6cp21f;    // 0.7109375 * 2^22
+constexpr uint32_t kLogTabBK = 0x3f368000u;
+constexpr int kLogTabBN = 4096;               // pairs
+constexpr float kLogTabBLo = 0x1.6dp-11f;     // 0.712890625 * 2^-10
+constexpr float kLogTabBHi = 0x1.6dp21f;      // 0.712890625 * 2^22
 
-// Copy the 8 KB table into LDS; every thread of the block calls it.
+// Copy the 32 KB table into LDS; every thread of the block calls it.
 __device__ __forceinline__ void log_tabb_load(float2 *lt)
 {
     const uint4 *src = reinterpret_cast<const uint4 *>(kLogTabB);
     uint4 *dst = reinterpret_cast<uint4 *>(lt);
-    for (int i = threadIdx.x; i < 512; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kLogTabBN / 2; i += blockDim.x) dst[i] = src[i];
     __syncthreads();
 }
 
-__device__ __forceinline__ int logtabb_index(uint32_t bx) { return (((int)(bx - kLogTabBK)) >> 18) + 320; }
+__device__ __forceinline__ int logtabb_index(uint32_t bx) { return (((int)(bx - kLogTabBK)) >> 16) + 1280; }
 
 __device__ __forceinline__ float logtabb_core(float x, float2 tv, float c)
 {
     const float r = __builtin_fmaf(x, tv.x, -1.0f);
     const float r2 = r * r;
-    float p = __builtin_fmaf(r, -0.25f, 0.3333333432674407958984375f);
-    p = __builtin_fmaf(p, r, -0.5f);
+    const float p = __builtin_fmaf(r, 0.3333333432674407958984375f, -0.5f);
     const float lr = __builtin_fmaf(p, r2, r);
     return (tv.y + c) + lr;
 }
