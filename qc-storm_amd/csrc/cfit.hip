@@ -251,16 +251,19 @@ CSelConst csel_const(const qcs_params &p, int kmax)
 // stage 1: K in {0, 1, 2} into sel[roi]; kcap (may be null) caps the count per ROI (a cap of 1:
 // the two-emitter fit was not run).  ROIs at K = 2 with kmax and cap >= 3 are appended to list3.
 // E = 0 selects the 2D model, CF_ELLIPTIC the elliptical one (fit vectors t1, t2 of their sizes).
+constexpr int SEL_WAVES = 16;   // work-group of the selection kernels (one list atomic per work-group)
+
 template <int E, int R>
-__global__ void __launch_bounds__(64) cf_select12_kernel(const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
+__global__ void __launch_bounds__(64 * SEL_WAVES) cf_select12_kernel(const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
                                                          const float *__restrict__ t1, const float *__restrict__ t2,
                                                          const double *__restrict__ l1, const double *__restrict__ l2,
                                                          const int *__restrict__ kcap, CSelConst c, int *__restrict__ sel,
                                                          int *__restrict__ cnt3, int *__restrict__ list3)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
+    __shared__ BlockAppend<SEL_WAVES, 1> ba;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (n_dev) n = min(*n_dev, n);
-    if ((int)(blockIdx.x * 64) >= n) return;   // whole wave beyond the count
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;   // whole work-group beyond the count
     int K = -1, roi = 0, cap = 3;
     if (j < n) {
         roi = idx ? idx[j] : j;
@@ -273,38 +276,66 @@ __global__ void __launch_bounds__(64) cf_select12_kernel(const int *__restrict__
         sel[roi] = K;
     }
     if (c.kmax >= 3) {
-        const int pos = wave_append(cnt3, K == 2 && cap >= 3);
-        if (pos >= 0) list3[pos] = roi;
+        const bool take[1] = {K == 2 && cap >= 3};
+        int pos[1];
+        block_append(ba, cnt3, take, pos);
+        if (pos[0] >= 0) list3[pos[0]] = roi;
     }
 }
 
-// stage 2: K = 3 where it beats K = 2, final count into sel[roi]; rows 3 roi .. 3 roi + 2 (unused
-// ones zero) with the FP64 full-Fisher CRLB
+// stage 2: K = 3 where it beats K = 2, final count into sel[roi], the unused rows of
+// 3 roi .. 3 roi + 2 zeroed, and the ROI appended to the list of its count (klist + (K - 1) cap,
+// counts in kcnt[0..2]): the rows kernels below each evaluate one model, so no wave runs the
+// Fisher matrices of several emitter counts one after the other with lanes masked.
 template <int E, int R>
-__global__ void __launch_bounds__(64) cf_rows_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ idx, int n,
-                                                     const int *__restrict__ n_dev, const float *__restrict__ t1,
-                                                     const float *__restrict__ t2, const float *__restrict__ t3,
-                                                     const double *__restrict__ l2, const double *__restrict__ l3, CSelConst c,
-                                                     int *__restrict__ sel, float *__restrict__ rows)
+__global__ void __launch_bounds__(64 * SEL_WAVES) cf_kpart_kernel(const int *__restrict__ idx, int n, const int *__restrict__ n_dev,
+                                                      const float *__restrict__ t3, const double *__restrict__ l2,
+                                                      const double *__restrict__ l3, CSelConst c, int *__restrict__ sel,
+                                                      float *__restrict__ rows, int *__restrict__ kcnt, int *__restrict__ klist,
+                                                      int cap)
 {
-    __shared__ double Ish[CP<3 + E>::NS * 64];
+    __shared__ BlockAppend<SEL_WAVES, 3> ba;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n_dev) n = min(*n_dev, n);
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;   // whole work-group beyond the count
+    int K = 0, roi = 0;
+    if (j < n) {
+        roi = idx ? idx[j] : j;
+        K = sel[roi];
+        if (K == 2 && c.kmax >= 3 && t3 && cf_valid<3 + E, R>(t3 + (size_t)roi * CM<3 + E>::P, c.n_min) &&
+            l3[roi] - l2[roi] > (double)c.llr_half)
+            K = 3;
+        sel[roi] = K;
+        float *o = rows + (size_t)roi * 3 * QCS_REC_FLOATS;
+        if ((reinterpret_cast<uintptr_t>(rows) & 15) == 0) {   // 48-byte records: 16-byte stores
+            for (int v = K * (QCS_REC_FLOATS / 4); v < 3 * (QCS_REC_FLOATS / 4); v++)
+                reinterpret_cast<float4 *>(o)[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            for (int i = K * QCS_REC_FLOATS; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
+        }
+    }
+    const bool take[3] = {K == 1, K == 2, K == 3};
+    int pos[3];
+    block_append(ba, kcnt, take, pos);
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (pos[k] >= 0) klist[(size_t)k * cap + pos[k]] = roi;
+}
+
+// rows 3 roi .. 3 roi + K - 1 of the listed ROIs of one model (KK) with the FP64 full-Fisher
+// CRLB; the packed Fisher matrix lives in registers, or in LDS for the 16-parameter model
+template <int KK, int R>
+__global__ void __launch_bounds__(64) cf_rows_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ list,
+                                                     const int *__restrict__ cnt, int cap, const float *__restrict__ t,
+                                                     CSelConst c, float *__restrict__ rows)
+{
+    __shared__ double Ish[cf_fisher_in_regs<KK>() ? 1 : CP<KK>::NS * 64];
     const int lane = threadIdx.x;
     const int j = blockIdx.x * 64 + lane;
-    if (n_dev) n = min(*n_dev, n);
-    if (j >= n) return;
-    const int roi = idx ? idx[j] : j;
+    if (j >= min(*cnt, cap)) return;
+    const int roi = list[j];
     const uint16_t *hdr = rois + (size_t)roi * R * (R + 1) + R * R;
-    float *o = rows + (size_t)roi * 3 * QCS_REC_FLOATS;
-    for (int i = 0; i < 3 * QCS_REC_FLOATS; i++) o[i] = 0.0f;
-    int K = sel[roi];
-    if (K == 2 && c.kmax >= 3 && t3 && cf_valid<3 + E, R>(t3 + (size_t)roi * CM<3 + E>::P, c.n_min) &&
-        l3[roi] - l2[roi] > (double)c.llr_half)
-        K = 3;
-    sel[roi] = K;
-    double *I = Ish + lane;
-    if (K == 1) cf_write_rows<1 + E, R>(t1 + (size_t)roi * CM<1 + E>::P, hdr, c, I, o);
-    else if (K == 2) cf_write_rows<2 + E, R>(t2 + (size_t)roi * CM<2 + E>::P, hdr, c, I, o);
-    else if (K == 3) cf_write_rows<3 + E, R>(t3 + (size_t)roi * CM<3 + E>::P, hdr, c, I, o);
+    cf_write_rows<KK, R>(t + (size_t)roi * CM<KK>::P, hdr, c, Ish + lane, rows + (size_t)roi * 3 * QCS_REC_FLOATS);
 }
 
 int select12_launch(const qcs_params &p, int kmax, const int *idx, int n, const int *d_n, const float *t1, const float *t2,
@@ -312,7 +343,7 @@ int select12_launch(const qcs_params &p, int kmax, const int *idx, int n, const 
 {
     if (n <= 0) return QCS_OK;
     const CSelConst c = csel_const(p, kmax);
-    const dim3 grid((n + 63) / 64), block(64);
+    const dim3 grid((n + 64 * SEL_WAVES - 1) / (64 * SEL_WAVES)), block(64 * SEL_WAVES);
     if (p.LocType == QCS_LOCTYPE_AS3D) {
         CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_select12_kernel<CF_ELLIPTIC, RT>), grid, block, 0, s, idx, n, d_n, t1,
                                                      kmax >= 2 ? t2 : nullptr, l1, l2, kcap, c, sel, cnt3, list3))
@@ -324,18 +355,43 @@ int select12_launch(const qcs_params &p, int kmax, const int *idx, int n, const 
     return QCS_OK;
 }
 
+// final counts and rows of the n listed ROIs (all n when idx is null; *d_n caps n when given);
+// kcnt: 3 device ints, klist: 3 n ints of scratch
 int rows_launch(const qcs_params &p, int kmax, const uint16_t *rois, const int *idx, int n, const int *d_n, const float *t1,
-                const float *t2, const float *t3, const double *l2, const double *l3, int *sel, float *rows, hipStream_t s)
+                const float *t2, const float *t3, const double *l2, const double *l3, int *sel, float *rows, int *kcnt,
+                int *klist, hipStream_t s)
 {
     if (n <= 0) return QCS_OK;
     const CSelConst c = csel_const(p, kmax);
     const dim3 grid((n + 63) / 64), block(64);
+    const dim3 sgrid((n + 64 * SEL_WAVES - 1) / (64 * SEL_WAVES)), sblock(64 * SEL_WAVES);
+    fill_async(kcnt, 3, 0, s);
+    const float *tk[3] = {t1, t2, t3};
     if (p.LocType == QCS_LOCTYPE_AS3D) {
-        CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_rows_kernel<CF_ELLIPTIC, RT>), grid, block, 0, s, rois, idx, n, d_n, t1, t2,
-                                                     kmax >= 3 ? t3 : nullptr, l2, l3, c, sel, rows))
+        CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_kpart_kernel<CF_ELLIPTIC, RT>), sgrid, sblock, 0, s, idx, n, d_n,
+                                                     kmax >= 3 ? t3 : nullptr, l2, l3, c, sel, rows, kcnt, klist, n))
+        QCS_HIP(hipGetLastError());
+        CF_DISPATCH_R(p.ROISize, {
+            hipLaunchKernelGGL((cf_rows_kernel<1 + CF_ELLIPTIC, RT>), grid, block, 0, s, rois, klist, kcnt, n, tk[0], c, rows);
+            if (kmax >= 2)
+                hipLaunchKernelGGL((cf_rows_kernel<2 + CF_ELLIPTIC, RT>), grid, block, 0, s, rois, klist + n, kcnt + 1, n, tk[1],
+                                   c, rows);
+            if (kmax >= 3)
+                hipLaunchKernelGGL((cf_rows_kernel<3 + CF_ELLIPTIC, RT>), grid, block, 0, s, rois, klist + 2 * (size_t)n,
+                                   kcnt + 2, n, tk[2], c, rows);
+        })
     } else {
-        CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_rows_kernel<0, RT>), grid, block, 0, s, rois, idx, n, d_n, t1, t2,
-                                                     kmax >= 3 ? t3 : nullptr, l2, l3, c, sel, rows))
+        CF_DISPATCH_R(p.ROISize, hipLaunchKernelGGL((cf_kpart_kernel<0, RT>), sgrid, sblock, 0, s, idx, n, d_n,
+                                                     kmax >= 3 ? t3 : nullptr, l2, l3, c, sel, rows, kcnt, klist, n))
+        QCS_HIP(hipGetLastError());
+        CF_DISPATCH_R(p.ROISize, {
+            hipLaunchKernelGGL((cf_rows_kernel<1, RT>), grid, block, 0, s, rois, klist, kcnt, n, tk[0], c, rows);
+            if (kmax >= 2)
+                hipLaunchKernelGGL((cf_rows_kernel<2, RT>), grid, block, 0, s, rois, klist + n, kcnt + 1, n, tk[1], c, rows);
+            if (kmax >= 3)
+                hipLaunchKernelGGL((cf_rows_kernel<3, RT>), grid, block, 0, s, rois, klist + 2 * (size_t)n, kcnt + 2, n, tk[2],
+                                   c, rows);
+        })
     }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
@@ -348,7 +404,7 @@ int rows_launch(const qcs_params &p, int kmax, const uint16_t *rois, const int *
 // cnt3 is one device int, list3 holds n ints.
 int plain_path(qcs_ctx *ctx, int kmax, const uint16_t *rois, const int *idx, int n, const int *d_n, float *t1, float *t2,
                float *t3, double *l1, double *l2, double *l3, int *sel, int *cnt3, int *list3, float *rows, int *info1, int *info2,
-               int *info3, hipStream_t s)
+               int *info3, int *kcnt, int *klist, hipStream_t s)
 {
     int rc;
     const int n1 = cf_np(cf_kk(ctx->p, 1)), n2 = cf_np(cf_kk(ctx->p, 2)), n3 = cf_np(cf_kk(ctx->p, 3));
@@ -357,7 +413,7 @@ int plain_path(qcs_ctx *ctx, int kmax, const uint16_t *rois, const int *idx, int
     fill_async(cnt3, 1, 0, s);
     if ((rc = select12_launch(ctx->p, kmax, idx, n, d_n, t1, t2, l1, l2, nullptr, sel, cnt3, list3, s))) return rc;
     if (kmax >= 3 && (rc = cfit_launch(ctx, 3, rois, list3, n, cnt3, CF_INIT_ADD, t2, n2, t3, n3, info3, l3, s))) return rc;
-    return rows_launch(ctx->p, kmax, rois, idx, n, d_n, t1, t2, t3, l2, l3, sel, rows, s);
+    return rows_launch(ctx->p, kmax, rois, idx, n, d_n, t1, t2, t3, l2, l3, sel, rows, kcnt, klist, s);
 }
 
 // ---- learned warm start -----------------------------------------------------------------------
@@ -375,9 +431,10 @@ int plain_path(qcs_ctx *ctx, int kmax, const uint16_t *rois, const int *idx, int
 // from the plain start (the fallback).
 enum { WC_L2W = 0, WC_L2A = 1, WC_F1 = 2, WC_F2 = 3, WC_L3 = 4, WC_N = 16 };
 
-__global__ void warm_decode_kernel(const float *__restrict__ out, const float *__restrict__ norm, int n, int R, float pmin,
-                                   float *__restrict__ tw1, float *__restrict__ tw2, int *__restrict__ cs, int *__restrict__ khat,
-                                   int *__restrict__ cnt, int *__restrict__ l2w, int *__restrict__ l2a)
+__global__ void __launch_bounds__(64 * SEL_WAVES)
+    warm_decode_kernel(const float *__restrict__ out, const float *__restrict__ norm, int n, int R, float pmin, float *__restrict__ tw1,
+                       float *__restrict__ tw2, int *__restrict__ cs, int *__restrict__ khat, int *__restrict__ cnt,
+                       int *__restrict__ l2w, int *__restrict__ l2a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     int cls = -1;   // 0: two-emitter fit from the network start; 1: from the plain start
@@ -418,10 +475,13 @@ __global__ void warm_decode_kernel(const float *__restrict__ out, const float *_
             }
         }
     }
-    int pos = wave_append(&cnt[WC_L2W], cls == 0);
-    if (pos >= 0) l2w[pos] = j;
-    pos = wave_append(&cnt[WC_L2A], cls == 1);
-    if (pos >= 0) l2a[pos] = j;
+    __shared__ BlockAppend<SEL_WAVES, 2> ba;
+    static_assert(WC_L2A == WC_L2W + 1, "the two start lists' counters are adjacent");
+    const bool take[2] = {cls == 0, cls == 1};
+    int pos[2];
+    block_append(ba, cnt + WC_L2W, take, pos);
+    if (pos[0] >= 0) l2w[pos[0]] = j;
+    if (pos[1] >= 0) l2a[pos[1]] = j;
 }
 
 // fallback test of the warm K-emitter fits of the listed ROIs (or all n): not converged, or a loss
@@ -484,7 +544,7 @@ __global__ void label_kernel(int n, const int *__restrict__ sel, const float *__
 // statistics of a multi-emitter call: [0..2] ROIs per final count, [4] iterations and [5] loss
 // evaluations over the fits (info1..info3), [6] ROIs whose count the network capped below 3,
 // [7] and [8] ROIs the network put at two and three emitters
-__global__ void cf_stats_kernel(int n, const int *__restrict__ sel, const int *__restrict__ info1, const int *__restrict__ info2,
+__global__ void __launch_bounds__(64 * SEL_WAVES) cf_stats_kernel(int n, const int *__restrict__ sel, const int *__restrict__ info1, const int *__restrict__ info2,
                                 const int *__restrict__ info3, const int *__restrict__ cs, const int *__restrict__ khat,
                                 long long *__restrict__ st)
 {
@@ -505,10 +565,18 @@ __global__ void cf_stats_kernel(int n, const int *__restrict__ sel, const int *_
             v[8] = khat[j] == 3;
         }
     }
+    // wave sums, then one atomic per statistic and work-group
+    __shared__ long long ws[SEL_WAVES][9];
     for (int i = 0; i < 9; i++) {
         long long x = v[i];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicAdd((unsigned long long *)&st[i], (unsigned long long)x);
+        if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][i] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+        long long x = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) x += ws[w][threadIdx.x];
+        if (x) atomicAdd((unsigned long long *)&st[threadIdx.x], (unsigned long long)x);
     }
 }
 
@@ -518,6 +586,7 @@ struct MeScratch {
     double *l1, *l2, *l3;
     int *cnt;                  // WC_N counters
     int *sel, *list3;
+    int *kcnt, *klist;         // rows: 3 counters per ROI set (cnt + 16 + 4 i), 3 n ints of lists
     int *info1, *info2, *info3;
     // warm path only
     float *out, *norm, *tw1, *tw2;
@@ -530,7 +599,7 @@ int me_scratch(qcs_ctx *ctx, int n, bool warm, bool info, MeScratch &m)
     const size_t n4 = ((size_t)n + 3) & ~(size_t)3;   // keeps every int array 16-byte aligned
     // fit vectors sized for the larger (elliptical) model: 6 + 11 + 16 floats per ROI
     const size_t nf = n4 * (6 + 11 + 16) + (warm ? n4 * (16 + 2 + 5 + 8) : 0);
-    const size_t ni = 64 + n4 * 2 + (info ? n4 * 12 : 0) + (warm ? n4 * 6 : 0);
+    const size_t ni = 64 + n4 * 5 + (info ? n4 * 12 : 0) + (warm ? n4 * 6 : 0);
     int rc;
     if ((rc = ctx->cf_theta.ensure(nf * sizeof(float))) || (rc = ctx->cf_ll.ensure(n4 * 3 * sizeof(double))) ||
         (rc = ctx->cf_list.ensure(ni * sizeof(int))))
@@ -547,11 +616,13 @@ int me_scratch(qcs_ctx *ctx, int n, bool warm, bool info, MeScratch &m)
     m.l3 = d + 2 * n4;
     int *p = ctx->cf_list.as<int>();
     m.cnt = p;
+    m.kcnt = p + 16;
     m.st = reinterpret_cast<long long *>(p + 32);
     p += 64;
     m.sel = p;
     m.list3 = p + n4;
-    p += 2 * n4;
+    m.klist = p + 2 * n4;
+    p += 5 * n4;
     if (info) {
         m.info1 = p;
         m.info2 = p + 4 * n4;
@@ -577,7 +648,7 @@ int stats_copy(qcs_ctx *ctx, const MeScratch &m, int n, const int *sel, long lon
 {
     int rc;
     if ((rc = ctx->pin_cnt.ensure(9 * sizeof(long long) + WC_N * sizeof(int)))) return rc;
-    hipLaunchKernelGGL(cf_stats_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, m.info1, m.info2, m.info3, m.cs, m.khat, m.st);
+    hipLaunchKernelGGL(cf_stats_kernel, dim3((n + 64 * SEL_WAVES - 1) / (64 * SEL_WAVES)), dim3(64 * SEL_WAVES), 0, s, n, sel, m.info1, m.info2, m.info3, m.cs, m.khat, m.st);
     QCS_HIP(hipGetLastError());
     long long *h = ctx->pin_cnt.as<long long>();
     int *hc = reinterpret_cast<int *>(h + 9);
@@ -608,7 +679,7 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     if ((rc = qcs_net_forward_dev(net, &p, d_rois, n, m.out, m.norm, s))) return rc;
     if ((rc = timer_end(ctx->timer, TM_NN, s, ev, n))) return rc;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
-    hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 255) / 256), dim3(256), 0, s, m.out, m.norm, n, R, ctx->tune.warm_pmin, m.tw1,
+    hipLaunchKernelGGL(warm_decode_kernel, dim3((n + 64 * SEL_WAVES - 1) / (64 * SEL_WAVES)), dim3(64 * SEL_WAVES), 0, s, m.out, m.norm, n, R, ctx->tune.warm_pmin, m.tw1,
                        m.tw2, m.cs, m.khat, cnt, m.l2w, m.l2a);
     QCS_HIP(hipGetLastError());
     const CFitConst c2 = cfit_const(p, ctx->tune, 2, CF_INIT_ADD);
@@ -647,7 +718,8 @@ int warm_launch(qcs_ctx *ctx, qcs_net *net, const uint16_t *d_rois, int n, float
     // selection as on the plain path
     if ((rc = select12_launch(p, 3, nullptr, n, nullptr, m.t1, m.t2, m.l1, m.l2, m.cs, sel, cnt + WC_L3, m.list3, s))) return rc;
     if ((rc = cfit_launch(ctx, 3, d_rois, m.list3, n, cnt + WC_L3, CF_INIT_ADD, m.t2, 8, m.t3, 11, m.info3, m.l3, s))) return rc;
-    if ((rc = rows_launch(p, 3, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l2, m.l3, sel, d_loc, s))) return rc;
+    if ((rc = rows_launch(p, 3, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l2, m.l3, sel, d_loc, m.kcnt, m.klist, s)))
+        return rc;
     if ((rc = timer_end(ctx->timer, TM_CFIT, s, ev, n))) return rc;
     if (h_stats) return stats_copy(ctx, m, n, sel, h_stats, s);
     return QCS_OK;
@@ -686,7 +758,7 @@ extern "C" int qcs_fit_mle_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, int 
     hipEvent_t ev;
     if ((rc = timer_begin(ctx->timer, s, &ev))) return rc;
     if ((rc = plain_path(ctx, kmax, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, d_sel ? d_sel : m.sel, m.cnt,
-                         m.list3, d_loc, nullptr, nullptr, nullptr, s)))
+                         m.list3, d_loc, nullptr, nullptr, nullptr, m.kcnt, m.klist, s)))
         return rc;
     return timer_end(ctx->timer, TM_CFIT, s, ev, n);
 }
@@ -757,7 +829,8 @@ extern "C" int qcs_fit_mle_mixed_dev(qcs_ctx *ctx, const qcs_roi_set *sets, int 
         const size_t o = off[i];
         int *sel = sets[i].d_sel ? sets[i].d_sel : m.sel + o;
         if ((rc = rows_launch(p, kmax, sets[i].d_rois, nullptr, sets[i].n, sets[i].d_n, m.t1 + o * n1, m.t2 + o * n2,
-                              m.t3 + o * n3, m.l2 + o, m.l3 + o, sel, sets[i].d_loc, s)))
+                              m.t3 + o * n3, m.l2 + o, m.l3 + o, sel, sets[i].d_loc, m.kcnt + 4 * i,
+                              m.klist + 3 * o, s)))
             return rc;
     }
     return timer_end(ctx->timer, TM_CFIT, s, ev, (long long)total);
@@ -907,7 +980,7 @@ extern "C" int qcs_fit_plain_stats_dev(qcs_ctx *ctx, const uint16_t *d_rois, int
     fill_async(m.cnt, 64, 0, s);
     fill_async(m.info1, (size_t)12 * (((size_t)n + 3) & ~(size_t)3), 0, s);
     if ((rc = plain_path(ctx, kmax, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, sel, m.cnt, m.list3, d_loc,
-                         m.info1, m.info2, m.info3, s)))
+                         m.info1, m.info2, m.info3, m.kcnt, m.klist, s)))
         return rc;
     return stats_copy(ctx, m, n, sel, h_stats, s);
 }
@@ -924,7 +997,7 @@ extern "C" int qcs_label_dev(qcs_ctx *ctx, const uint16_t *d_rois, int n, float 
     if ((rc = ctx->cf_rows.ensure((size_t)n * 3 * QCS_REC_FLOATS * sizeof(float)))) return rc;
     int *sel = d_sel ? d_sel : m.sel;
     if ((rc = plain_path(ctx, 3, d_rois, nullptr, n, nullptr, m.t1, m.t2, m.t3, m.l1, m.l2, m.l3, sel, m.cnt, m.list3,
-                         ctx->cf_rows.as<float>(), nullptr, nullptr, nullptr, s)))
+                         ctx->cf_rows.as<float>(), nullptr, nullptr, nullptr, m.kcnt, m.klist, s)))
         return rc;
     hipLaunchKernelGGL(label_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, sel, m.t1, m.t2, m.t3, d_labels);
     QCS_HIP(hipGetLastError());

@@ -13,6 +13,7 @@
 // lane-interleaved (element i of lane l at [i * 64 + l]).
 #pragma once
 #include <math.h>
+#include <type_traits>
 #include "qcs_internal.h"
 
 namespace qcs {
@@ -631,6 +632,123 @@ __device__ void cf_crlb64(const float (&th)[CP<KK>::P], double *I, float (&cx)[C
     }
 }
 
+// The Fisher sums of the models up to 11 parameters (66 packed doubles) fit the register file.
+template <int KK> __host__ __device__ constexpr bool cf_fisher_in_regs() { return CP<KK>::P <= 11; }
+
+// f(integral_constant<int, i>) for i = B .. E - 1: a loop whose index is a compile-time constant
+// in every iteration, whatever the unroller decides (triangular nests included)
+template <int B, int E, class F>
+__device__ __forceinline__ void cf_sfor(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        cf_sfor<B + 1, E>(f);
+    }
+}
+
+// The same CRLB with I in registers: every loop over parameters is unrolled, so each packed
+// entry has a compile-time index.  The divisions of the model's derivatives move out of the
+// pixel loop (1/s, 1/sx, 1/sy once per ROI; dmu/dN_k is the emitter's normalised Gaussian), and
+// the forward substitution for e_e starts at row e, where z is first non-zero.
+template <int KK, int R>
+__device__ __forceinline__ void cf_crlb64_reg(const float (&th)[CP<KK>::P], float (&cx)[CM<KK>::K], float (&cy)[CM<KK>::K])
+{
+    using M = CM<KK>;
+    constexpr int P = M::P, K = M::K, NS = CP<KK>::NS;
+    double I[NS];
+#pragma unroll
+    for (int i = 0; i < NS; i++) I[i] = 0.0;
+    // per emitter: photons, centre, 1/sx^2, 1/sy^2, 1/(2 pi sx sy), 1/sx, 1/sy (2D: one width)
+    double nk[K], xk[K], yk[K], ax[K], ay[K], nrm[K], jx[K], jy[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const double sx = th[M::isx(k)], sy = th[M::isy(k)];
+        nk[k] = th[M::iN(k)];
+        xk[k] = th[M::ix(k)];
+        yk[k] = th[M::iy(k)];
+        ax[k] = 1.0 / (sx * sx);
+        ay[k] = 1.0 / (sy * sy);
+        nrm[k] = 1.0 / (6.283185307179586 * sx * sy);
+        jx[k] = 1.0 / sx;
+        jy[k] = 1.0 / sy;
+    }
+    const double b = th[M::IB];
+    for (int r = 0; r < R; r++) {
+        const double rpx = r + 0.5;
+        for (int c = 0; c < R; c++) {
+            const double cpx = c + 0.5;
+            double J[P];
+            double mu = b, jw = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const double dx = cpx - xk[k], dy = rpx - yk[k];
+                const double u = dx * dx * ax[k], v = dy * dy * ay[k];
+                const double g = nrm[k] * exp(-0.5 * (u + v));
+                const double mk = nk[k] * g;
+                mu += mk;
+                J[M::iN(k)] = g;
+                J[M::ix(k)] = mk * dx * ax[k];
+                J[M::iy(k)] = mk * dy * ay[k];
+                if constexpr (M::AS) {
+                    J[M::isx(k)] = mk * (u - 1.0) * jx[k];
+                    J[M::isy(k)] = mk * (v - 1.0) * jy[k];
+                } else {
+                    jw += mk * (u + v - 2.0);
+                }
+            }
+            if constexpr (!M::AS) J[3 * K] = jw * jx[0];
+            J[P - 1] = 1.0;
+            const double im = 1.0 / mu;
+#pragma unroll
+            for (int a = 0; a < P; a++) {
+                const double ja = J[a] * im;
+#pragma unroll
+                for (int b2 = a; b2 < P; b2++) I[tri(a, b2, P)] += ja * J[b2];
+            }
+        }
+    }
+    // Cholesky I = U^T U in place; a non-positive pivot marks the matrix singular
+    bool ok = true;
+    cf_sfor<0, P>([&](auto JC) {
+        constexpr int j = decltype(JC)::value;
+        double d = I[tri(j, j, P)];
+        cf_sfor<0, j>([&](auto KC) {
+            constexpr int k = decltype(KC)::value;
+            d -= I[tri(k, j, P)] * I[tri(k, j, P)];
+        });
+        ok = ok && d > 0.0;
+        const double u = sqrt(d), iu = 1.0 / u;
+        I[tri(j, j, P)] = u;
+        cf_sfor<j + 1, P>([&](auto IC) {
+            constexpr int i = decltype(IC)::value;
+            double v = I[tri(j, i, P)];
+            cf_sfor<0, j>([&](auto KC) {
+                constexpr int k = decltype(KC)::value;
+                v -= I[tri(k, j, P)] * I[tri(k, i, P)];
+            });
+            I[tri(j, i, P)] = v * iu;
+        });
+    });
+    // (I^-1)_ee = |U^-T e_e|^2 for e = x_k, y_k; z_i = 0 for i < e
+    cf_sfor<0, 2 * K>([&](auto WC) {
+        constexpr int w = decltype(WC)::value;
+        constexpr int e = (w & 1) ? M::iy(w >> 1) : M::ix(w >> 1);
+        double z[P], var = 0.0;
+        cf_sfor<e, P>([&](auto IC) {
+            constexpr int i = decltype(IC)::value;
+            double v = (i == e) ? 1.0 : 0.0;
+            cf_sfor<e, i>([&](auto KC) {
+                constexpr int k2 = decltype(KC)::value;
+                v -= I[tri(k2, i, P)] * z[k2];
+            });
+            z[i] = v / I[tri(i, i, P)];
+            var += z[i] * z[i];
+        });
+        const float sd = ok && var > 0.0 ? (float)sqrt(var) : -1.0f;
+        if (w & 1) cy[w >> 1] = sd; else cx[w >> 1] = sd;
+    });
+}
+
 struct CSelConst {
     float llr_half;     // accept K+1 over K when ll(K+1) - ll(K) exceeds this (half the chi^2 bound)
     float n_min;        // minimum photons of every emitter of an accepted model
@@ -693,13 +811,17 @@ __device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelCon
     float th[CP<KK>::P];
     for (int i = 0; i < CP<KK>::P; i++) th[i] = t[i];
     float cx[K], cy[K];
-    cf_crlb64<KK, R>(th, I, cx, cy);
+    if constexpr (cf_fisher_in_regs<KK>()) cf_crlb64_reg<KK, R>(th, cx, cy);
+    else cf_crlb64<KK, R>(th, I, cx, cy);
     const float b = th[M::IB];
     const int half = R / 2;
     const float xc = (float)hdr[0], yc = (float)hdr[1];
     const int fr = (int)hdr[2] + (int)hdr[3] * 65536;
+    // a record is 48 bytes: three 16-byte stores when the row array is 16-byte aligned
+    const bool vec = (reinterpret_cast<uintptr_t>(rows) & 15) == 0;
+#pragma unroll
     for (int k = 0; k < K; k++) {
-        float *o = rows + (size_t)k * QCS_REC_FLOATS;
+        float o[QCS_REC_FLOATS];
         const float sx = th[M::isx(k)], sy = th[M::isy(k)];
         const float peak = th[M::iN(k)] * 0.15915494309189535f / (sx * sy);
         o[QCS_POS_PPHO] = peak;
@@ -719,6 +841,15 @@ __device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelCon
         o[QCS_POS_CRBX] = cx[k] > 0.0f ? cx[k] * c.pixel_size : -1.0f;
         o[QCS_POS_CRBY] = cy[k] > 0.0f ? cy[k] * c.pixel_size : -1.0f;
         o[QCS_POS_FRME] = (float)fr;
+        float *dst = rows + (size_t)k * QCS_REC_FLOATS;
+        if (vec) {
+#pragma unroll
+            for (int v = 0; v < QCS_REC_FLOATS / 4; v++)
+                reinterpret_cast<float4 *>(dst)[v] = make_float4(o[4 * v], o[4 * v + 1], o[4 * v + 2], o[4 * v + 3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < QCS_REC_FLOATS; i++) dst[i] = o[i];
+        }
     }
 }
 
@@ -890,6 +1021,44 @@ __device__ __forceinline__ int wave_append(int *count, bool take)
     if (lane == leader) base = atomicAdd(count, __popcll(m));
     base = __shfl(base, leader, 64);
     return take ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// Appends from a whole work-group of W waves to NL lists with counters count[0 .. NL - 1]: lane
+// flags take[k]; pos[k] receives the lane's slot in list k, or -1.  One global atomic per
+// work-group and list instead of one per wave (wave_append): atomics on one address serialise
+// at the L2, at about 10 ns each, which outweighs a light kernel's own work over 10^6 items.
+// Every thread of the work-group calls it (two barriers).
+template <int W, int NL>
+struct BlockAppend {
+    int woff[W][NL];
+    int base[NL];
+};
+
+template <int W, int NL>
+__device__ __forceinline__ void block_append(BlockAppend<W, NL> &sh, int *count, const bool (&take)[NL], int (&pos)[NL])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long m[NL];
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        m[k] = __ballot(take[k]);
+        if (lane == 0) sh.woff[w][k] = __popcll(m[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < NL) {
+        const int k = threadIdx.x;
+        int s = 0;
+        for (int i = 0; i < W; i++) {
+            const int c = sh.woff[i][k];
+            sh.woff[i][k] = s;
+            s += c;
+        }
+        sh.base[k] = s > 0 ? atomicAdd(count + k, s) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NL; k++)
+        pos[k] = take[k] ? sh.base[k] + sh.woff[w][k] + __popcll(m[k] & ((1ull << lane) - 1ull)) : -1;
 }
 
 // ---- host side ----------------------------------------------------------------------------
