@@ -107,6 +107,8 @@ template <> struct CqLayout<3, 11> {   // 60.5 + 66 floats: 5 per CU
 };
 template <class LY> using CqQ = typename std::conditional<LY::QU16, uint16_t, float>::type;
 
+constexpr int CQ_CHUNK = 64;   // most items a wave takes from the queue with one atomic
+
 template <int KK, int R, class LY>
 __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitConst &c, CqQ<LY> *qsh, float *Dsh)
 {
@@ -140,18 +142,41 @@ __device__ __forceinline__ void cfq_run(const CfqSet &S, int *head, const CFitCo
         }
         st = CQ_IDLE;
     };
-    // lanes without work take the next items of the queue (one atomic per wave)
+    // Lanes without work take the next items of the queue.  The wave keeps a chunk of taken but
+    // unstarted items [cb, cb + cl) (wave-uniform) and refills it with one atomic when the idle
+    // lanes need more than it holds: up to CQ_CHUNK items, but no more than the wave's share of
+    // what the queue held at its last refill, so near the end no wave holds back work that others
+    // could run.  Same-address atomics serialise at the L2; one per refill of a few lanes made
+    // the queue head a hot spot.
+    int cb = 0, cl = 0, seen = 0;
+    bool qdone = false;
     auto refill = [&]() {
         const bool want = st == CQ_IDLE;
         const unsigned long long m = __ballot(want);
         if (m == 0ull || dry) return;   // wave-uniform
         const int cnt = __popcll(m), leader = __ffsll((unsigned long long)m) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(head, cnt);
-        base = __shfl(base, leader, 64);
-        if (base + cnt >= n) dry = true;   // the queue holds nothing more
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        int j;
+        if (cnt <= cl) {
+            j = cb + rank;
+            cb += cnt;
+            cl -= cnt;
+        } else {
+            const int need = cnt - cl;
+            const int grab = max(need, min(CQ_CHUNK, (n - seen) / (int)gridDim.x));
+            int base = 0;
+            if (lane == leader) base = atomicAdd(head, grab);
+            base = __shfl(base, leader, 64);
+            j = rank < cl ? cb + rank : base + (rank - cl);
+            const int got = max(0, min(grab, n - base));   // items of the grab inside the queue
+            const int used = min(need, got);
+            cb = base + used;
+            cl = got - used;
+            seen = min(base + grab, n);
+            qdone = base + grab >= n;
+        }
+        if (qdone && cl == 0) dry = true;   // the queue holds nothing more for this wave
         if (!want) return;
-        const int j = base + __popcll(m & ((1ull << lane) - 1ull));
         if (j >= n) return;
         roi = S.idx ? S.idx[j] : j;
         if constexpr (LY::QU16) cf_load_q16<R>(S.rois, roi, qs);
