@@ -10,7 +10,9 @@ are simulated on the GPU and stay resident in HBM.  One step is one pass of the 
     ROI extraction (qcs_extract_dev, batches of 1 frame, 7x7 ROIs + WLE pre-estimation)
     -> single-emitter 2D BFGS MLE fit + CRLB (qcs_fit_dev)
     -> frame-sorted compaction (qcs_compact_sorted_dev)
-    -> 10 nm-pixel rendering into a 20480 x 20480 float image (qcs_render_dev, image cleared first)
+    -> 10 nm-pixel rendering into a 20480 x 20480 float image, a fresh image per step
+       (qcs_render_fresh_dev: the same image as clearing it and calling qcs_render_dev, drawn in
+       one write pass)
 
 `value` = fitted ROIs per second over whole steps (every extracted ROI is fitted).  The roofline
 entry is for the dominant kernel (the 7x7 one-emitter BFGS fit): algorithmic FLOPs per launch
@@ -650,10 +652,11 @@ class Pipeline:
             self._lane(self.lanes[0])
         else:
             list(self.pool.map(self._lane, self.lanes))   # each lane's compaction waits for its stream
-        with self.torch.cuda.stream(self.stream):
-            self.img.zero_()
-        for L in self.lanes:
-            self.ctx.render_dev(L["out"], L["m"], self.img, None, self.lanes[0]["sh"])
+        # a fresh SR image per step: the first lane's rows replace the image (clear and draw in one
+        # write pass), the other lanes' rows add to it
+        for i, L in enumerate(self.lanes):
+            draw = self.ctx.render_fresh_dev if i == 0 else self.ctx.render_dev
+            draw(L["out"], L["m"], self.img, None, self.lanes[0]["sh"])
         return sum(L["n"] for L in self.lanes), sum(L["m"] for L in self.lanes)
 
     def timing_enable(self):

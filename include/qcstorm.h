@@ -246,6 +246,11 @@ int qcs_compact_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_out, int *
  * d_zimg (3D z-weighted plane, may be NULL for 2D) -- FluoRenderWithPrec_2D/3D
  * (ImageRender_2d.cu:55-155, ImageRender_3d.cu:60-147). */
 int qcs_render_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream);
+/* The same rendering into planes whose previous contents are replaced: the result equals clearing
+ * d_img (and d_zimg, 3D) and calling qcs_render_dev (the same sums; float atomics add in the order
+ * they happen to run, as in qcs_render_dev), in one write pass over the planes (no clear, no read).  No counterpart in the reference: for callers that draw a fresh image per
+ * batch of rows, such as a per-batch display or benchmark step. */
+int qcs_render_fresh_dev(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream);
 
 /* The colour-coded 3D image (ImgRender_GetSaveImg_3D / RenderSaveImg_3D,
  * ImgRender/ImageRender_3d.cu:238-403, 424-590).  d_img / d_zimg: the SRImageWidth x SRImageHigh

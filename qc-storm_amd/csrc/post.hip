@@ -126,10 +126,14 @@ __device__ __forceinline__ void splat_tiles(const Splat &q, int &tx0, int &tx1, 
     ty1 = (q.iy + h) / RTILE;
 }
 
-// pass 1 (cursor == null): localizations per tile; pass 2: the per-tile lists
+// A localization drawn into one tile, as the tile kernel reads it: two 16-byte words,
+// {pp, rs, fx, fy} and {z, (iy - y0) << 16 | (ix - x0) & 0xffff (tile-relative), 0, 0}.
+constexpr int RENT = 2;   // float4 words per entry
+
+// pass 1 (cursor == null): localizations per tile; pass 2: the per-tile lists of prepared entries
 __global__ void __launch_bounds__(256) render_bin_kernel(const float *__restrict__ loc, int n, RenderConst c,
                                                          int *__restrict__ tcnt, const int *__restrict__ toff,
-                                                         int *__restrict__ cursor, int *__restrict__ list)
+                                                         int *__restrict__ cursor, float4 *__restrict__ ent)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -140,33 +144,65 @@ __global__ void __launch_bounds__(256) render_bin_kernel(const float *__restrict
     for (int ty = ty0; ty <= ty1; ty++)
         for (int tx = tx0; tx <= tx1; tx++) {
             const int t = ty * c.tiles_x + tx;
-            if (!cursor) atomicAdd(&tcnt[t], 1);
-            else list[toff[t] + atomicAdd(&cursor[t], 1)] = i;
+            if (!cursor) {
+                atomicAdd(&tcnt[t], 1);
+                continue;
+            }
+            const size_t e = (size_t)(toff[t] + atomicAdd(&cursor[t], 1)) * RENT;
+            const int rx = q.ix - tx * RTILE, ry = q.iy - ty * RTILE;   // within [-3, RTILE + 2]
+            ent[e] = make_float4(q.pp, q.rs, q.fx, q.fy);
+            ent[e + 1] = make_float4(q.z, __int_as_float((int)(((unsigned)ry << 16) | ((unsigned)rx & 0xffffu))), 0.0f, 0.0f);
         }
 }
 
 // One work-group of RBLK threads per tile: LDS accumulation, then img += tile (and zimg += z tile
-// with ZON).  Each wave draws one listed localization at a time, one footprint pixel per lane
-// (the row's fields are wave-uniform).  An interior tile of an aligned image (rows 16-byte aligned,
+// with ZON).  The tile's entries are contiguous; each thread stages one in LDS per round, then
+// each wave draws one staged localization at a time, one footprint pixel per lane.  An interior tile of an aligned image (rows 16-byte aligned,
 // width a multiple of 4) reads its image pixels, four per thread, before drawing, so the HBM
-// latency of the read-modify-write overlaps the LDS work.
+// latency of the read-modify-write overlaps the LDS work.  FRESH: the planes' previous contents are
+// replaced (as if cleared first): every tile is written, empty ones with zeros, and nothing is
+// read, so a fresh image costs one write of the planes instead of a clear plus a read-modify-write.
 constexpr int RBLK = 256;
 constexpr int RVEC = RTILE * RTILE / (4 * RBLK);   // float4 image words per thread
-template <bool ZON>
-__global__ void __launch_bounds__(RBLK) render_tile_kernel(const float *__restrict__ loc, RenderConst c, const int *__restrict__ tcnt,
-                                                           const int *__restrict__ toff, const int *__restrict__ list,
-                                                           float *__restrict__ img, float *__restrict__ zimg)
+constexpr int RSTR = RTILE;   // LDS row stride of the tile accumulators
+template <bool ZON, bool FRESH>
+__global__ void __launch_bounds__(RBLK) render_tile_kernel(RenderConst c, const int *__restrict__ tcnt, const int *__restrict__ toff,
+                                                           const float4 *__restrict__ ent, float *__restrict__ img,
+                                                           float *__restrict__ zimg)
 {
-    __shared__ float acc[RTILE * RTILE];
-    __shared__ float zacc[ZON ? RTILE * RTILE : 1];
+    __shared__ __align__(16) float acc[RTILE * RSTR];
+    __shared__ __align__(16) float zacc[ZON ? RTILE * RSTR : 1];
+    // drawing parameters of up to RBLK listed localizations, one loaded per thread
+    __shared__ float s_pp[RBLK], s_rs[RBLK], s_fx[RBLK], s_fy[RBLK], s_z[ZON ? RBLK : 1];
+    __shared__ int s_xy[RBLK];
     const int t = blockIdx.x;
-    const int cnt = tcnt[t];
-    if (cnt == 0) return;   // uniform over the work-group
+    const int cnt = tcnt[t], first = toff[t];
+    if (!FRESH && cnt == 0) return;   // uniform over the work-group
     const int x0 = (t % c.tiles_x) * RTILE, y0 = (t / c.tiles_x) * RTILE;
     const bool aligned = ((reinterpret_cast<uintptr_t>(img) | (ZON ? reinterpret_cast<uintptr_t>(zimg) : 0)) & 15) == 0;
     const bool vec = aligned && (c.srw & 3) == 0 && x0 + RTILE <= c.srw && y0 + RTILE <= c.srh;   // uniform
+    if (FRESH && cnt == 0) {   // an empty tile of a fresh image
+        if (vec) {
+#pragma unroll
+            for (int j = 0; j < RVEC; j++) {
+                const int p = 4 * (threadIdx.x + j * RBLK);
+                const size_t g = (size_t)(y0 + p / RTILE) * c.srw + (x0 + p % RTILE);
+                *reinterpret_cast<float4 *>(img + g) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (ZON) *reinterpret_cast<float4 *>(zimg + g) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            return;
+        }
+        for (int p = threadIdx.x; p < RTILE * RTILE; p += RBLK) {
+            const int gy = y0 + p / RTILE, gx = x0 + p % RTILE;
+            if (gx >= c.srw || gy >= c.srh) continue;
+            const size_t g = (size_t)gy * c.srw + gx;
+            img[g] = 0.0f;
+            if (ZON) zimg[g] = 0.0f;
+        }
+        return;
+    }
     float4 v[RVEC], u[RVEC];
-    if (vec) {
+    if (vec && !FRESH) {
 #pragma unroll
         for (int j = 0; j < RVEC; j++) {
             const int p = 4 * (threadIdx.x + j * RBLK);
@@ -176,45 +212,72 @@ __global__ void __launch_bounds__(RBLK) render_tile_kernel(const float *__restri
         }
     }
     for (int p = threadIdx.x; p < RTILE * RTILE; p += RBLK) {
-        acc[p] = 0.0f;
-        if (ZON) zacc[p] = 0.0f;
+        acc[(p / RTILE) * RSTR + p % RTILE] = 0.0f;
+        if (ZON) zacc[(p / RTILE) * RSTR + p % RTILE] = 0.0f;
     }
     __syncthreads();
-    const int *lst = list + toff[t];
-    const int lane = threadIdx.x & 63;
+    const float4 *te = ent + (size_t)first * RENT;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a = lane / 7, b = lane - 7 * (lane / 7);   // footprint pixel of this lane (lanes < 49)
-    for (int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); k < cnt; k += RBLK / 64) {
-        Splat q;
-        render_prep(loc + (size_t)lst[k] * QCS_REC_FLOATS, c, q);   // listed, hence drawn
-        if (q.one) {
-            if (lane == 0) atomicAdd(&acc[(q.iy - y0) * RTILE + (q.ix - x0)], q.pp);
-            continue;
+    const bool one = !c.is3d && c.mode == QCS_RENDER_PEAKPHOTON_1PIXEL;   // single-pixel splats
+    // Rounds of RBLK localizations: every thread stages one entry (one round of memory latency for
+    // the whole round, coalesced), then each wave draws every fourth staged localization.
+    for (int base = 0; base < cnt; base += RBLK) {
+        const int m = min(RBLK, cnt - base);
+        if ((int)threadIdx.x < m) {
+            const float4 e0 = te[(size_t)(base + threadIdx.x) * RENT], e1 = te[(size_t)(base + threadIdx.x) * RENT + 1];
+            s_pp[threadIdx.x] = e0.x;
+            s_rs[threadIdx.x] = e0.y;
+            s_fx[threadIdx.x] = e0.z;
+            s_fy[threadIdx.x] = e0.w;
+            if (ZON) s_z[threadIdx.x] = e1.x;
+            s_xy[threadIdx.x] = __float_as_int(e1.y);
         }
-        const int py = q.iy - 3 + a - y0, px = q.ix - 3 + b - x0;
-        if (lane < 49 && py >= 0 && py < RTILE && px >= 0 && px < RTILE) {
-            const float dy = (float)a - q.fy, dx = (float)b - q.fx;
-            const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * q.rs);
-            atomicAdd(&acc[py * RTILE + px], q.pp * psf);
-            if (ZON) atomicAdd(&zacc[py * RTILE + px], (q.z * q.pp) * psf);
+        __syncthreads();
+        for (int k = wave; k < m; k += RBLK / 64) {
+            const float pp = s_pp[k];
+            const int xy = s_xy[k];
+            const int rx = (int)(short)(xy & 0xffff), ry = xy >> 16;   // tile-relative centre pixel
+            if (one) {
+                if (lane == 0) atomicAdd(&acc[ry * RSTR + rx], pp);
+                continue;
+            }
+            const int py = ry - 3 + a, px = rx - 3 + b;
+            if (lane < 49 && py >= 0 && py < RTILE && px >= 0 && px < RTILE) {
+                const float dy = (float)a - s_fy[k], dx = (float)b - s_fx[k];
+                const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * s_rs[k]);
+                atomicAdd(&acc[py * RSTR + px], pp * psf);
+                if (ZON) atomicAdd(&zacc[py * RSTR + px], (s_z[k] * pp) * psf);
+            }
         }
+        __syncthreads();   // the next round overwrites the staged parameters; acc is complete after the last (cnt >= 1 here)
     }
-    __syncthreads();
     // the tile is owned by this work-group: plain coalesced writes
     if (vec) {
 #pragma unroll
         for (int j = 0; j < RVEC; j++) {
             const int p = 4 * (threadIdx.x + j * RBLK);
             const size_t g = (size_t)(y0 + p / RTILE) * c.srw + (x0 + p % RTILE);
-            v[j].x += acc[p];
-            v[j].y += acc[p + 1];
-            v[j].z += acc[p + 2];
-            v[j].w += acc[p + 3];
+            const float4 ac = *reinterpret_cast<const float4 *>(&acc[(p / RTILE) * RSTR + p % RTILE]);
+            if (FRESH) {   // 0 + a == a: the same bits as clearing first
+                v[j] = ac;
+            } else {
+                v[j].x += ac.x;
+                v[j].y += ac.y;
+                v[j].z += ac.z;
+                v[j].w += ac.w;
+            }
             *reinterpret_cast<float4 *>(img + g) = v[j];
             if (ZON) {
-                u[j].x += zacc[p];
-                u[j].y += zacc[p + 1];
-                u[j].z += zacc[p + 2];
-                u[j].w += zacc[p + 3];
+                const float4 zc = *reinterpret_cast<const float4 *>(&zacc[(p / RTILE) * RSTR + p % RTILE]);
+                if (FRESH) {
+                    u[j] = zc;
+                } else {
+                    u[j].x += zc.x;
+                    u[j].y += zc.y;
+                    u[j].z += zc.z;
+                    u[j].w += zc.w;
+                }
                 *reinterpret_cast<float4 *>(zimg + g) = u[j];
             }
         }
@@ -224,8 +287,9 @@ __global__ void __launch_bounds__(RBLK) render_tile_kernel(const float *__restri
         const int gy = y0 + p / RTILE, gx = x0 + p % RTILE;
         if (gx >= c.srw || gy >= c.srh) continue;
         const size_t g = (size_t)gy * c.srw + gx;
-        img[g] += acc[p];
-        if (ZON) zimg[g] += zacc[p];
+        const int a2 = (p / RTILE) * RSTR + p % RTILE;
+        img[g] = FRESH ? acc[a2] : img[g] + acc[a2];
+        if (ZON) zimg[g] = FRESH ? zacc[a2] : zimg[g] + zacc[a2];
     }
 }
 
@@ -243,10 +307,19 @@ __global__ void __launch_bounds__(256) max_disp_kernel(const float *__restrict__
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s)
+int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, bool fresh, hipStream_t s)
 {
-    if (n <= 0) return QCS_OK;
     const qcs_params &p = ctx->p;
+    if (n <= 0) {
+        if (fresh) {   // nothing to draw: the planes are cleared (a fill kernel, graph-safe)
+            const size_t npx = (size_t)p.SRImageWidth * p.SRImageHigh;
+            fill_async(reinterpret_cast<int *>(d_img), npx, 0, s);
+            if (d_zimg && (p.LocType == QCS_LOCTYPE_AS3D || p.LocType == QCS_LOCTYPE_DH3D))
+                fill_async(reinterpret_cast<int *>(d_zimg), npx, 0, s);
+            QCS_HIP(hipGetLastError());
+        }
+        return QCS_OK;
+    }
     RenderConst c;
     c.srw = p.SRImageWidth;
     c.srh = p.SRImageHigh;
@@ -262,22 +335,28 @@ int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *
     const size_t ntile = (size_t)c.tiles_x * c.tiles_y;
     if (ntile > (size_t)INT32_MAX / 8 || (size_t)n * 4 > (size_t)INT32_MAX) return fail(QCS_ERR_ARG, "SR image or row count too large");
     const int nst = scan_tile_count(ntile);
-    // tile counts, offsets (+ scan sums), cursors, then the lists (a footprint touches <= 4 tiles)
-    const size_t words = ntile * 3 + (size_t)nst + 1 + (size_t)n * 4;
+    // tile counts, offsets (+ scan sums), cursors, then the entries (16-byte aligned; a footprint
+    // touches <= 4 tiles)
+    const size_t head = (ntile * 3 + (size_t)nst + 1 + 3) & ~(size_t)3;
+    const size_t words = head + (size_t)n * 4 * RENT * 4;
     int rc;
     if ((rc = ctx->rn_buf.ensure(words * sizeof(int)))) return rc;
-    int *tcnt = ctx->rn_buf.as<int>(), *toff = tcnt + ntile, *tsum = toff + ntile, *cursor = tsum + nst + 1,
-        *list = cursor + ntile;
+    int *tcnt = ctx->rn_buf.as<int>(), *toff = tcnt + ntile, *tsum = toff + ntile, *cursor = tsum + nst + 1;
+    float4 *ent = reinterpret_cast<float4 *>(ctx->rn_buf.as<int>() + head);
     fill_async(tcnt, ntile, 0, s);
     fill_async(cursor, ntile, 0, s);
     const dim3 g1((n + 255) / 256), b1(256);
     hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, nullptr, nullptr, nullptr);
     if ((rc = scan_exclusive(tcnt, (int)ntile, toff, tsum, s))) return rc;
-    hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, toff, cursor, list);
-    if (c.is3d && d_zimg)
-        hipLaunchKernelGGL(render_tile_kernel<true>, dim3((unsigned)ntile), dim3(RBLK), 0, s, d_loc, c, tcnt, toff, list, d_img, d_zimg);
-    else
-        hipLaunchKernelGGL(render_tile_kernel<false>, dim3((unsigned)ntile), dim3(RBLK), 0, s, d_loc, c, tcnt, toff, list, d_img, nullptr);
+    hipLaunchKernelGGL(render_bin_kernel, g1, b1, 0, s, d_loc, n, c, tcnt, toff, cursor, ent);
+    const dim3 tg((unsigned)ntile), tb(RBLK);
+    if (c.is3d && d_zimg) {
+        if (fresh) hipLaunchKernelGGL((render_tile_kernel<true, true>), tg, tb, 0, s, c, tcnt, toff, ent, d_img, d_zimg);
+        else hipLaunchKernelGGL((render_tile_kernel<true, false>), tg, tb, 0, s, c, tcnt, toff, ent, d_img, d_zimg);
+    } else {
+        if (fresh) hipLaunchKernelGGL((render_tile_kernel<false, true>), tg, tb, 0, s, c, tcnt, toff, ent, d_img, nullptr);
+        else hipLaunchKernelGGL((render_tile_kernel<false, false>), tg, tb, 0, s, c, tcnt, toff, ent, d_img, nullptr);
+    }
     QCS_HIP(hipGetLastError());
     return QCS_OK;
 }
@@ -372,7 +451,19 @@ extern "C" int qcs_render_dev(qcs_ctx *c, const float *d_loc, int n, float *d_im
     hipEvent_t ev;
     int rc = timer_begin(c->timer, (hipStream_t)stream, &ev);
     if (rc) return rc;
-    rc = render_launch(c, d_loc, n, d_img, d_zimg, (hipStream_t)stream);
+    rc = render_launch(c, d_loc, n, d_img, d_zimg, false, (hipStream_t)stream);
+    if (rc) return rc;
+    return timer_end(c->timer, TM_RENDER, (hipStream_t)stream, ev, n);
+}
+
+extern "C" int qcs_render_fresh_dev(qcs_ctx *c, const float *d_loc, int n, float *d_img, float *d_zimg, void *stream)
+{
+    if (!c || n < 0 || !d_img || (n > 0 && !d_loc)) return fail(QCS_ERR_ARG, "invalid argument");
+    if (c->p.SRImageWidth <= 16 || c->p.SRImageHigh <= 16) return fail(QCS_ERR_ARG, "SR image size not set");
+    hipEvent_t ev;
+    int rc = timer_begin(c->timer, (hipStream_t)stream, &ev);
+    if (rc) return rc;
+    rc = render_launch(c, d_loc, n, d_img, d_zimg, true, (hipStream_t)stream);
     if (rc) return rc;
     return timer_end(c->timer, TM_RENDER, (hipStream_t)stream, ev, n);
 }

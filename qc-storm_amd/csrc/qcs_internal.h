@@ -93,7 +93,7 @@ int noalloc_scratch_init(int device);
 
 // rendering helpers (post.hip): splat rows into the SR image(s); max of (int) image value into
 // *d_out (which must start at 0)
-int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, hipStream_t s);
+int render_launch(qcs_ctx *ctx, const float *d_loc, int n, float *d_img, float *d_zimg, bool fresh, hipStream_t s);
 int max_disp_launch(const float *d_img, size_t npx, int *d_out, hipStream_t s);
 
 // device utilities (extract.hip): exclusive scan of n ints (total at tile_sums[ntiles]),

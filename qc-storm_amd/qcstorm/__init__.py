@@ -211,6 +211,7 @@ PROTOTYPES = {
     "qcs_compact_sorted_dev": (_i, [_vp, _vp, _i, _i, _i, _vp, _P(_i), _vp]),
     "qcs_compact_dev": (_i, [_vp, _vp, _i, _vp, _P(_i), _vp]),
     "qcs_render_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "qcs_render_fresh_dev": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "qcs_fit_rois": (_i, [_vp, _vp, _vp, _i, _vp, _i, _P(_i)]),
     "qcs_cfit_dev": (_i, [_vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qcs_fit_mle_dev": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
@@ -599,6 +600,11 @@ class Context:
     def render_dev(self, d_loc, n, d_img, d_zimg=None, stream=None):
         check(lib().qcs_render_dev(self.h, self._dp(d_loc), n, self._dp(d_img), self._dp(d_zimg), stream),
               "qcs_render_dev")
+
+    def render_fresh_dev(self, d_loc, n, d_img, d_zimg=None, stream=None):
+        """Render into d_img (and d_zimg) replacing their contents (clear + render in one pass)."""
+        check(lib().qcs_render_fresh_dev(self.h, self._dp(d_loc), n, self._dp(d_img), self._dp(d_zimg), stream),
+              "qcs_render_fresh_dev")
 
     def drift_correct_dev(self, d_rows, n, corr_frames, shift_cap=None, stream=None):
         """Drift correction of n frame-sorted device rows in place.  Returns (frame count, per-frame x

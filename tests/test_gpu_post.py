@@ -89,3 +89,48 @@ def test_render_accumulates_into_existing_image(gpu):
     scale = np.maximum(np.abs(want), 1e-3)
     assert np.all(np.abs(got - want) <= 2e-5 * scale + 1e-6)
     assert np.all(got[ref == 0] == np.float32(0.25))
+
+
+@pytest.mark.parametrize("lt,unaligned", [(0, False), (0, True), (1, False)])
+def test_render_fresh_replaces_the_planes(gpu, lt, unaligned):
+    """qcs_render_fresh_dev equals clearing the planes and rendering: pixels no localization touches
+    end at zero whatever the planes held, the others match the oracle's image.  An image that
+    starts 4 bytes past a 16-byte boundary takes the per-pixel path in every tile; the aligned
+    one the 16-byte path inside and the per-pixel path in the partial tiles of the right and
+    bottom edges (2000 x 1500 SR pixels, 64-pixel tiles)."""
+    loc = _rows(20000, 21 + int(unaligned), zrange=600.0 if lt else 0.0)
+    params = Q.default_params(ImageWidth=200, ImageHigh=150, PixelZoom=10.0, LocType=lt, FixedLocPrec=15.0)
+    ref, zref = O.render(params, loc, with_z=(lt == 1))
+    ctx = Q.Context(params)
+    d = torch.from_numpy(loc).cuda()
+    H, W = params.SRImageHigh, params.SRImageWidth
+    off = 1 if unaligned else 0
+    store = torch.full((H * W + 4,), 7.0, dtype=torch.float32, device="cuda")
+    img = store[off:off + H * W].view(H, W)
+    zimg = torch.full((H, W), -3.0, dtype=torch.float32, device="cuda") if lt == 1 else None
+    ctx.render_fresh_dev(d, loc.shape[0], img, zimg)
+    torch.cuda.synchronize()
+    got = img.cpu().numpy()
+    scale = np.maximum(np.abs(ref), 1e-3)
+    print("unaligned", unaligned, "sum", ref.sum(), "max rel", np.max(np.abs(got - ref) / scale))
+    assert np.all(np.abs(got - ref) <= 1e-5 * scale + 1e-6)
+    assert np.all(got[ref == 0] == 0.0) and (ref == 0).sum() > 0
+    assert float(store[:off].sum().item()) == 7.0 * off and float(store[off + H * W:].min().item()) == 7.0
+    if lt == 1:
+        zg = zimg.cpu().numpy()
+        assert np.allclose(zg, zref, rtol=1e-4, atol=1e-2)
+        assert np.all(zg[zref == 0] == 0.0)
+    # the accumulating call on top of a fresh image adds the same sums again
+    ctx.render_dev(d, loc.shape[0], img, zimg)
+    torch.cuda.synchronize()
+    got2 = img.cpu().numpy()
+    assert np.all(np.abs(got2 - 2.0 * ref) <= 2e-5 * scale + 1e-6)
+    # no rows: the planes are cleared
+    img.fill_(7.0)
+    if zimg is not None:
+        zimg.fill_(-3.0)
+    ctx.render_fresh_dev(d, 0, img, zimg)
+    torch.cuda.synchronize()
+    assert float(img.abs().max().item()) == 0.0
+    if zimg is not None:
+        assert float(zimg.abs().max().item()) == 0.0
