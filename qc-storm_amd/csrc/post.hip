@@ -173,7 +173,8 @@ __global__ void __launch_bounds__(RBLK) render_tile_kernel(RenderConst c, const 
     __shared__ __align__(16) float acc[RTILE * RSTR];
     __shared__ __align__(16) float zacc[ZON ? RTILE * RSTR : 1];
     // drawing parameters of up to RBLK listed localizations, one loaded per thread
-    __shared__ float s_pp[RBLK], s_rs[RBLK], s_fx[RBLK], s_fy[RBLK], s_z[ZON ? RBLK : 1];
+    __shared__ float4 s_p[RBLK];   // {pp, rs, fx, fy}: one 16-byte broadcast read per drawn entry
+    __shared__ float s_z[ZON ? RBLK : 1];
     __shared__ int s_xy[RBLK];
     const int t = blockIdx.x;
     const int cnt = tcnt[t], first = toff[t];
@@ -226,16 +227,14 @@ __global__ void __launch_bounds__(RBLK) render_tile_kernel(RenderConst c, const 
         const int m = min(RBLK, cnt - base);
         if ((int)threadIdx.x < m) {
             const float4 e0 = te[(size_t)(base + threadIdx.x) * RENT], e1 = te[(size_t)(base + threadIdx.x) * RENT + 1];
-            s_pp[threadIdx.x] = e0.x;
-            s_rs[threadIdx.x] = e0.y;
-            s_fx[threadIdx.x] = e0.z;
-            s_fy[threadIdx.x] = e0.w;
+            s_p[threadIdx.x] = e0;
             if (ZON) s_z[threadIdx.x] = e1.x;
             s_xy[threadIdx.x] = __float_as_int(e1.y);
         }
         __syncthreads();
         for (int k = wave; k < m; k += RBLK / 64) {
-            const float pp = s_pp[k];
+            const float4 sp = s_p[k];
+            const float pp = sp.x;
             const int xy = s_xy[k];
             const int rx = (int)(short)(xy & 0xffff), ry = xy >> 16;   // tile-relative centre pixel
             if (one) {
@@ -244,8 +243,8 @@ __global__ void __launch_bounds__(RBLK) render_tile_kernel(RenderConst c, const 
             }
             const int py = ry - 3 + a, px = rx - 3 + b;
             if (lane < 49 && py >= 0 && py < RTILE && px >= 0 && px < RTILE) {
-                const float dy = (float)a - s_fy[k], dx = (float)b - s_fx[k];
-                const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * s_rs[k]);
+                const float dy = (float)a - sp.w, dx = (float)b - sp.z;
+                const float psf = qexpf(__builtin_fmaf(dx, dx, dy * dy) * sp.y);
                 atomicAdd(&acc[py * RSTR + px], pp * psf);
                 if (ZON) atomicAdd(&zacc[py * RSTR + px], (s_z[k] * pp) * psf);
             }
