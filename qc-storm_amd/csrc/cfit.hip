@@ -323,19 +323,17 @@ __global__ void __launch_bounds__(64 * SEL_WAVES) cf_kpart_kernel(const int *__r
 }
 
 // rows 3 roi .. 3 roi + K - 1 of the listed ROIs of one model (KK) with the FP64 full-Fisher
-// CRLB; the packed Fisher matrix lives in registers, or in LDS for the 16-parameter model
+// CRLB (the packed Fisher matrix in registers)
 template <int KK, int R>
 __global__ void __launch_bounds__(64) cf_rows_kernel(const uint16_t *__restrict__ rois, const int *__restrict__ list,
                                                      const int *__restrict__ cnt, int cap, const float *__restrict__ t,
                                                      CSelConst c, float *__restrict__ rows)
 {
-    __shared__ double Ish[cf_fisher_in_regs<KK>() ? 1 : CP<KK>::NS * 64];
-    const int lane = threadIdx.x;
-    const int j = blockIdx.x * 64 + lane;
+    const int j = blockIdx.x * 64 + threadIdx.x;
     if (j >= min(*cnt, cap)) return;
     const int roi = list[j];
     const uint16_t *hdr = rois + (size_t)roi * R * (R + 1) + R * R;
-    cf_write_rows<KK, R>(t + (size_t)roi * CM<KK>::P, hdr, c, Ish + lane, rows + (size_t)roi * 3 * QCS_REC_FLOATS);
+    cf_write_rows<KK, R>(t + (size_t)roi * CM<KK>::P, hdr, c, rows + (size_t)roi * 3 * QCS_REC_FLOATS);
 }
 
 int select12_launch(const qcs_params &p, int kmax, const int *idx, int n, const int *d_n, const float *t1, const float *t2,

@@ -62,6 +62,17 @@ __device__ __forceinline__ float cf_scale(int i)
 // packed upper-triangular index of (i, j), i <= j
 __device__ __forceinline__ constexpr int tri(int i, int j, int P) { return i * P - (i * (i - 1)) / 2 + (j - i); }
 
+// f(integral_constant<int, i>) for i = B .. E - 1: a loop whose index is a compile-time constant
+// in every iteration, whatever the unroller decides (triangular nests included)
+template <int B, int E, class F>
+__device__ __forceinline__ void cf_sfor(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        cf_sfor<B + 1, E>(f);
+    }
+}
+
 // Where a lane keeps its observed photons q and its packed inverse Hessian D.  Every D access
 // uses a compile-time index (unrolled loops), so the entries a DSplit keeps in registers stay
 // there.
@@ -390,6 +401,89 @@ __device__ __forceinline__ bool cf_fisher_inverse(const float (&th)[CP<KK>::P], 
     return true;
 }
 
+// The same inverse with the packed matrix in registers for any model size (the persistent
+// queue's prep pass, whose output goes to HBM): every loop is a compile-time loop, and the
+// arithmetic and its order are those of cf_fisher_inverse, so the result is bit-identical.
+template <int KK, int R>
+__device__ __forceinline__ bool cf_fisher_inverse_regs(const float (&th)[CP<KK>::P], float (&F)[CP<KK>::NS])
+{
+    constexpr int P = CP<KK>::P, NS = CP<KK>::NS;
+    cf_sfor<0, NS>([&](auto IC) { F[decltype(IC)::value] = 0.0f; });
+#pragma unroll 1
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+        for (int c = 0; c < R; c++) {
+            float J[P];
+            const float mu = cf_jacobian<KK>(th, (float)c + 0.5f, (float)r + 0.5f, J);
+            const float im = __builtin_amdgcn_rcpf(fmaxf(mu, 1e-6f));
+            cf_sfor<0, P>([&](auto AC) {
+                constexpr int a = decltype(AC)::value;
+                const float ja = J[a] * im;
+                cf_sfor<a, P>([&](auto BC) {
+                    constexpr int b2 = decltype(BC)::value;
+                    F[tri(a, b2, P)] = fmaf(ja, J[b2], F[tri(a, b2, P)]);
+                });
+            });
+        }
+    }
+    // F = U^T U (U upper, in place); a non-positive pivot ends it
+    bool ok = true;
+    cf_sfor<0, P>([&](auto JC) {
+        constexpr int j = decltype(JC)::value;
+        if (!ok) return;
+        float d = F[tri(j, j, P)];
+        cf_sfor<0, j>([&](auto KC) {
+            constexpr int k = decltype(KC)::value;
+            d = fmaf(-F[tri(k, j, P)], F[tri(k, j, P)], d);
+        });
+        if (!(d > 0.0f)) {
+            ok = false;
+            return;
+        }
+        const float u = sqrtf(d), iu = 1.0f / u;
+        F[tri(j, j, P)] = u;
+        cf_sfor<j + 1, P>([&](auto IC) {
+            constexpr int i = decltype(IC)::value;
+            float v = F[tri(j, i, P)];
+            cf_sfor<0, j>([&](auto KC) {
+                constexpr int k = decltype(KC)::value;
+                v = fmaf(-F[tri(k, j, P)], F[tri(k, i, P)], v);
+            });
+            F[tri(j, i, P)] = v * iu;
+        });
+    });
+    if (!ok) return false;
+    // U^-1 in place, rows bottom-up, each row right to left
+    cf_sfor<0, P>([&](auto IC) {
+        constexpr int i = P - 1 - decltype(IC)::value;
+        const float iu = 1.0f / F[tri(i, i, P)];
+        cf_sfor<0, P - 1 - i>([&](auto JC) {
+            constexpr int j = P - 1 - decltype(JC)::value;
+            float v = 0.0f;
+            cf_sfor<i + 1, j + 1>([&](auto KC) {
+                constexpr int k = decltype(KC)::value;
+                v = fmaf(F[tri(i, k, P)], F[tri(k, j, P)], v);
+            });
+            F[tri(i, j, P)] = -v * iu;
+        });
+        F[tri(i, i, P)] = iu;
+    });
+    // F^-1 = U^-1 U^-T
+    cf_sfor<0, P>([&](auto IC) {
+        constexpr int i = decltype(IC)::value;
+        cf_sfor<i, P>([&](auto JC) {
+            constexpr int j = decltype(JC)::value;
+            float v = 0.0f;
+            cf_sfor<j, P>([&](auto KC) {
+                constexpr int k = decltype(KC)::value;
+                v = fmaf(F[tri(i, k, P)], F[tri(j, k, P)], v);
+            });
+            F[tri(i, j, P)] = v;
+        });
+    });
+    return true;
+}
+
 // mu of one pixel in double (the FP64 likelihood and CRLB)
 template <int KK>
 __device__ __forceinline__ double cf_mu64(const float (&th)[CP<KK>::P], double cpx, double rpx, double *J)
@@ -579,77 +673,13 @@ __device__ __forceinline__ void cf_init(const float *qs, const float *init, int 
 }
 
 // ---- FP64 full-Fisher CRLB of one model (standard deviations of x_k and y_k, px) ----------
-// I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed in LDS per lane; Cholesky; then
-// (I^-1)_ii = |L^-1 e_i|^2 for the position indices.
-template <int KK, int R>
-__device__ void cf_crlb64(const float (&th)[CP<KK>::P], double *I, float (&cx)[CM<KK>::K], float (&cy)[CM<KK>::K])
-{
-    using M = CM<KK>;
-    constexpr int P = M::P, K = M::K;
-    for (int i = 0; i < CP<KK>::NS; i++) I[i * 64] = 0.0;
-    for (int r = 0; r < R; r++)
-        for (int c = 0; c < R; c++) {
-            double J[P];
-            const double mu = cf_mu64<KK>(th, c + 0.5, r + 0.5, J);
-            const double im = 1.0 / mu;
-            for (int a = 0; a < P; a++) {
-                const double ja = J[a] * im;
-                for (int b2 = a; b2 < P; b2++) I[tri(a, b2, P) * 64] += ja * J[b2];
-            }
-        }
-    // in-place Cholesky of the packed upper triangle: I = U^T U
-    bool ok = true;
-    for (int j = 0; j < P; j++) {
-        double d = I[tri(j, j, P) * 64];
-        for (int k = 0; k < j; k++) d -= I[tri(k, j, P) * 64] * I[tri(k, j, P) * 64];
-        if (!(d > 0.0)) { ok = false; break; }
-        const double u = sqrt(d);
-        I[tri(j, j, P) * 64] = u;
-        for (int i = j + 1; i < P; i++) {
-            double v = I[tri(j, i, P) * 64];
-            for (int k = 0; k < j; k++) v -= I[tri(k, j, P) * 64] * I[tri(k, i, P) * 64];
-            I[tri(j, i, P) * 64] = v / u;
-        }
-    }
-    for (int k = 0; k < K; k++) {
-        for (int w = 0; w < 2; w++) {
-            const int e = w == 0 ? M::ix(k) : M::iy(k);
-            double var = -1.0;
-            if (ok) {
-                // (I^-1)_ee = |U^-T e_e|^2: forward substitution U^T z = e_e
-                double z[P];
-                for (int i = 0; i < P; i++) {
-                    double v = (i == e) ? 1.0 : 0.0;
-                    for (int k2 = 0; k2 < i; k2++) v -= I[tri(k2, i, P) * 64] * z[k2];
-                    z[i] = v / I[tri(i, i, P) * 64];
-                }
-                var = 0.0;
-                for (int i = 0; i < P; i++) var += z[i] * z[i];
-            }
-            const float sd = var > 0.0 ? (float)sqrt(var) : -1.0f;
-            if (w == 0) cx[k] = sd; else cy[k] = sd;
-        }
-    }
-}
-
-// The Fisher sums of the models up to 11 parameters (66 packed doubles) fit the register file.
-template <int KK> __host__ __device__ constexpr bool cf_fisher_in_regs() { return CP<KK>::P <= 11; }
-
-// f(integral_constant<int, i>) for i = B .. E - 1: a loop whose index is a compile-time constant
-// in every iteration, whatever the unroller decides (triangular nests included)
-template <int B, int E, class F>
-__device__ __forceinline__ void cf_sfor(F &&f)
-{
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        cf_sfor<B + 1, E>(f);
-    }
-}
-
-// The same CRLB with I in registers: every loop over parameters is unrolled, so each packed
-// entry has a compile-time index.  The divisions of the model's derivatives move out of the
-// pixel loop (1/s, 1/sx, 1/sy once per ROI; dmu/dN_k is the emitter's normalised Gaussian), and
-// the forward substitution for e_e starts at row e, where z is first non-zero.
+// I = sum_pixels grad(mu) grad(mu)^T / mu in double, packed (upper triangle) in registers: every
+// loop over parameters is a compile-time loop, so each entry has a fixed register.  The 16-parameter
+// elliptical model's 136 doubles spill into the accumulation registers (one wave per SIMD).
+// Cholesky I = U^T U, then (I^-1)_ee = |U^-T e_e|^2 for the position indices e.  The divisions of
+// the model's derivatives move out of the pixel loop (1/s, 1/sx, 1/sy once per ROI; dmu/dN_k is
+// the emitter's normalised Gaussian), and the forward substitution for e_e starts at row e,
+// where z is first non-zero.
 template <int KK, int R>
 __device__ __forceinline__ void cf_crlb64_reg(const float (&th)[CP<KK>::P], float (&cx)[CM<KK>::K], float (&cy)[CM<KK>::K])
 {
@@ -804,15 +834,14 @@ __device__ __forceinline__ float cf_zpoly(const float (&pc)[5], float d, float c
 }
 
 template <int KK, int R>
-__device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, double *I, float *rows)
+__device__ void cf_write_rows(const float *t, const uint16_t *hdr, const CSelConst &c, float *rows)
 {
     using M = CM<KK>;
     constexpr int K = M::K;
     float th[CP<KK>::P];
     for (int i = 0; i < CP<KK>::P; i++) th[i] = t[i];
     float cx[K], cy[K];
-    if constexpr (cf_fisher_in_regs<KK>()) cf_crlb64_reg<KK, R>(th, cx, cy);
-    else cf_crlb64<KK, R>(th, I, cx, cy);
+    cf_crlb64_reg<KK, R>(th, cx, cy);
     const float b = th[M::IB];
     const int half = R / 2;
     const float xc = (float)hdr[0], yc = (float)hdr[1];
