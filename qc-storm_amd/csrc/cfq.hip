@@ -40,46 +40,29 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
 {
     using Q = CQ<KK>;
     constexpr int P = Q::P, NS = Q::NS;
-    // models past 66 packed entries factor their start matrix in registers (the LDS matrix of
-    // 136 floats per lane left two work-groups per CU); the others keep cf_fisher_inverse's own choice
-    constexpr bool DREG = NS > 66;
+    // the start matrix is factored in registers and written to the record from there: LDS holds
+    // only the observed image (the 16-parameter model's LDS matrix had left two work-groups per CU)
     __shared__ float qsh[R * R * 64];
-    __shared__ float Dsh[DREG ? 1 : NS * 64];
     const int lane = threadIdx.x, j = blockIdx.x * 64 + lane;
     const int n = S.n_dev ? min(*S.n_dev, S.n) : S.n;   // a device count never exceeds the capacity
     if (j >= n) return;   // no cross-lane work here
     const int roi = S.idx ? S.idx[j] : j;
-    float *qs = qsh + lane, *D = Dsh + lane;
+    float *qs = qsh + lane;
     cf_load_q<R>(S.rois, roi, c, qs);
     float th[P], g[P];
     cf_init<KK, R>(qs, S.init ? S.init + (size_t)roi * S.istride : nullptr, S.init ? S.init_mode : CF_INIT_HEUR, c.sigma0, th);
     cf_project<KK, R>(th);
     const float L = cf_lossgrad<KK, R, true>(qs, th, g);
-    float Dr[DREG ? NS : 1];
-    bool first;
-    if constexpr (DREG) {
-        first = !cf_fisher_inverse_regs<KK, R>(th, Dr);
-        if (first)
-            cf_sfor<0, P>([&](auto IC) {
-                constexpr int i = decltype(IC)::value;
-                cf_sfor<i, P>([&](auto JC) {
-                    constexpr int jj = decltype(JC)::value;
-                    Dr[tri(i, jj, P)] = (i == jj) ? 1.0f : 0.0f;
-                });
+    float Dr[NS];
+    const bool first = !cf_fisher_inverse_regs<KK, R>(th, Dr);
+    if (first)
+        cf_sfor<0, P>([&](auto IC) {
+            constexpr int i = decltype(IC)::value;
+            cf_sfor<i, P>([&](auto JC) {
+                constexpr int jj = decltype(JC)::value;
+                Dr[tri(i, jj, P)] = (i == jj) ? 1.0f : 0.0f;
             });
-    } else {
-        first = !cf_fisher_inverse<KK, R>(th, D);
-        if (first) {
-#pragma unroll
-            for (int i = 0; i < P; i++)
-#pragma unroll
-                for (int jj = i; jj < P; jj++) D[tri(i, jj, P) * 64] = (i == jj) ? 1.0f : 0.0f;
-        }
-    }
-    const auto dval = [&](int i) -> float {
-        if constexpr (DREG) return Dr[i];
-        else return D[i * 64];
-    };
+        });
     float4 *rec = reinterpret_cast<float4 *>(S.rec + (size_t)j * Q::W);
 #pragma unroll
     for (int v = 0; v < Q::W / 4; v++) {
@@ -91,7 +74,7 @@ __global__ void __launch_bounds__(64) cfq_prep_kernel(CfqSet S, CFitConst c)
                  : i == Q::OL ? L
                  : i < Q::OF ? g[(i > Q::OL && i < Q::OF) ? i - Q::OG : 0]
                  : i == Q::OF ? (first ? 1.0f : 0.0f)
-                 : i < Q::OD + NS ? dval((i >= Q::OD && i < Q::OD + NS) ? i - Q::OD : 0) : 0.0f;
+                 : i < Q::OD + NS ? Dr[(i >= Q::OD && i < Q::OD + NS) ? i - Q::OD : 0] : 0.0f;
         }
         rec[v] = make_float4(e[0], e[1], e[2], e[3]);
     }
